@@ -1,0 +1,50 @@
+#!/usr/bin/env bash
+# Builds the spray-simulation reference with flang, linked to the project's
+# golden-vector driver (oracle/ref/fvk_ref_driver.f90).  TEST INFRASTRUCTURE,
+# run in the development container only (the reference is not on GPU hosts).
+#
+# Objects and module files go to a scratch directory outside the repository;
+# only the two driver executables land in oracle/_ref/:
+#   _ref/shipped/fvk_ref_driver   reference sources compiled where they lie
+#   _ref/relaxed/fvk_ref_driver   same, except that cfd_setup.f90 is taken
+#       from a scratch copy with two hard-coded locals switched:
+#         simpler=.true.  -> simpler=.false.   (cfd_setup.f90:581; enables the
+#                                                relaxed update at :675-683)
+#         call solv_cg(1, -> call solv_cg(0,   (cfd_setup.f90:662,742; Jacobi)
+#       which is the SIMPLE/Jacobi-PCG configuration the reference cannot
+#       select without editing a literal.
+set -euo pipefail
+REF=${FVK_REFERENCE_DIR:-/root/reference/spray-simulation}
+HERE=$(cd "$(dirname "$0")" && pwd)
+OUT=$HERE/_ref
+FC=${FC:-flang}
+FFLAGS=${FFLAGS:--O2}
+MODS="precision solv_data solv_res solv_precond solv_direct solv_bicg solv_cg
+      mom_data mom_math mom_closure mom_solve cfd_data grd_data grd_solv grd_math
+      grd_constr grd_read grd_conn grd_geom cfd_constr spr_data spr_inject
+      spr_impact spr_hydro solv_conv cfd_define cfd_setup"
+[ -d "$REF" ] || { echo "reference not found at $REF" >&2; exit 1; }
+SCRATCH=$(mktemp -d /tmp/fvk_ref_build.XXXXXX)
+trap 'rm -rf "$SCRATCH"' EXIT
+
+build() {  # $1 variant name, $2 directory holding cfd_setup.f90
+  local v=$1 setup_dir=$2 w=$SCRATCH/$1 objs=""
+  mkdir -p "$w" "$OUT/$v"
+  for m in $MODS; do
+    local src=$REF/$m.f90
+    [ "$m" = cfd_setup ] && src=$setup_dir/cfd_setup.f90
+    (cd "$w" && $FC $FFLAGS -c "$src" -o "$m.o")
+    objs="$objs $w/$m.o"
+  done
+  (cd "$w" && $FC $FFLAGS -c "$HERE/ref/fvk_ref_driver.f90" -o driver.o)
+  $FC $FFLAGS -o "$OUT/$v/fvk_ref_driver" "$w/driver.o" $objs
+}
+
+build shipped "$REF"
+mkdir -p "$SCRATCH/relaxed_src"
+sed -e 's/simpler=\.true\./simpler=.false./' -e 's/call solv_cg(1,/call solv_cg(0,/' \
+    "$REF/cfd_setup.f90" > "$SCRATCH/relaxed_src/cfd_setup.f90"
+[ "$(grep -c 'simpler=.false.' "$SCRATCH/relaxed_src/cfd_setup.f90")" -ge 2 ]
+[ "$(grep -c 'call solv_cg(0,' "$SCRATCH/relaxed_src/cfd_setup.f90")" -eq 2 ]
+build relaxed "$SCRATCH/relaxed_src"
+echo "built $OUT/shipped/fvk_ref_driver $OUT/relaxed/fvk_ref_driver"

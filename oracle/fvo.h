@@ -1,0 +1,139 @@
+/*
+ * fvo — CPU FP64 oracle of the spray-simulation SIMPLE hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a clean-room C restatement of the
+ * reference algorithm (DominicJones/programs, spray-simulation Fortran sources), used
+ * by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the
+ * parity checker.  Nothing in programs_amd/ may include, link or call it.
+ *
+ * Conventions mirror the reference so that loops read alike:
+ *   - every entity array is 1-based (element 0 unused); cell ids 1..n_vol,
+ *     face ids 1..n_fce, boundary slots ib = n_vol+1..n_vbf;
+ *   - `in == 0` marks a boundary face (grd_data.f90:23);
+ *   - geometric vectors are stored with stride 3 even in 2D (only n_dx
+ *     used); field components phi(i)%vc(c) with stride n_vc;
+ *   - floating-point expressions keep the reference's evaluation order and
+ *     the file is compiled with -ffp-contract=off (no FMA) like the x86-64
+ *     baseline Fortran build.
+ *
+ * Parity status of each function is stated in its comment:
+ *   "pinned: ..." names the checks that fix it.
+ */
+#ifndef FVO_H
+#define FVO_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* precision.f90:8-9 — small = 10**(-20), large = 1/small */
+double fvo_small(void);
+double fvo_large(void);
+
+typedef struct fvo_mesh {
+  int n_dx;
+  int n_vrt, n_vol, n_fce, n_bf, n_vbf;
+  int n_slot; /* sum of faces over cells */
+  double *vx; /* 3*(n_vrt+1) */
+  /* cells */
+  int *v_ty, *v_nvrt, *v_nfce; /* n_vol+1 */
+  int *v_vptr, *v_vrt;         /* vertex list of cell i: v_vrt[v_vptr[i] .. v_vptr[i+1]-1] */
+  int *v_fptr;                 /* slot range of cell i: v_fptr[i] .. v_fptr[i+1]-1 */
+  int *v_bnd, *v_fce, *v_ng;   /* per slot: bnd_lst (by local face), fce_lst, ng_lst */
+  double *v_x, *v_v, *v_vr, *v_sf;
+  /* faces */
+  int *f_ip, *f_in, *f_ib, *f_bty, *f_nvrt, *f_vptr, *f_vrt;
+  double *f_x, *f_n, *f_dxp, *f_dxn, *f_s, *f_lpn, *f_wfp;
+} fvo_mesh;
+
+/* ---- mesh construction (grd_read / grd_conn / grd_geom) ---- */
+/* Element face table lookup (gambit.vrt restated; see fvo.c). */
+int fvo_elem_nfce(int v_ty);
+int fvo_elem_face(int v_ty, int ivf, int *vrt_local /* out, 1-based */);
+
+/* Raw cell description: cell i (1-based) has vertex list
+   v_vrt[v_vptr[i-1] .. v_vptr[i]-1] (0-based pointer array of n_vol+1),
+   boundary codes bnd[s_ptr[i-1] + ivf-1] per local face (0 = none). */
+fvo_mesh *fvo_mesh_raw(int n_dx, int n_vrt, const double *vx /* 3*n_vrt, 0-based */,
+                       int n_vol, const int *v_ty, const int *v_vptr,
+                       const int *v_vrt /* 1-based vertex ids */, const int *bnd);
+fvo_mesh *fvo_read_neu(const char *path, double l_sc);
+/* Uniform or jittered box of nx*ny*nz hexes (nz=0: 2D quads). */
+fvo_mesh *fvo_gen_box(int nx, int ny, int nz, double lx, double ly, double lz,
+                      double jitter, uint64_t seed, int lid_code, int wall_code);
+int fvo_conn_literal(fvo_mesh *m); /* grd_conn O(n^2) restated */
+int fvo_conn_hash(fvo_mesh *m);    /* O(N), identical output */
+int fvo_geom(fvo_mesh *m);         /* grd_geom restated */
+void fvo_mesh_free(fvo_mesh *m);
+
+/* ---- linear solvers on the face-based matrix (solv_*.f90) ---- */
+typedef struct fvo_sys {
+  const fvo_mesh *m;
+  double *pl;       /* n_vol+1 */
+  double *ng_p, *ng_n; /* n_fce+1 */
+} fvo_sys;
+
+/* phi(i)%vc(c) is phi[i*ld + c] (i 1-based, as the reference's pointer
+   array); res0 is column-major like res0(n_vol, n_vc): element (i, c) at
+   [c*n_vol + i-1]. */
+void fvo_solv_res(const fvo_sys *A, int n_vc, const double *phi, int ld, double *res0,
+                  double *res0_n, int not_r0);
+void fvo_solv_precond(const fvo_sys *A, int imat, double *c, double *c_r);
+void fvo_solv_direct(const fvo_sys *A, int imat, const double *c, const double *c_r,
+                     double *x, const double *b);
+void fvo_spmv(const fvo_sys *A, const double *x, double *y);
+int fvo_solv_cg(const fvo_sys *A, int imat, double reduc, int n_iter, int n_vc,
+                double *phi, int ld, double *res0, double *res0_n, int not_r0, int *iters,
+                double *hist /* n_vc*n_iter relative L1, may be NULL */);
+int fvo_solv_bicg(const fvo_sys *A, int imat, double reduc, int n_iter, int n_vc,
+                  double *phi, int ld, double *res0, double *res0_n, int not_r0, int *iters,
+                  double *hist);
+
+/* ---- flow state and SIMPLE (cfd_setup.f90 / cfd_solve.f90) ---- */
+typedef struct fvo_cfg {
+  int simpler;   /* cfd_setup.f90:581 */
+  int imat_u, imat_p;
+  int n_bicg, n_cg, n_grad, c_sch;
+  double urf_u, urf_p, reduc_u, reduc_p, c_bl;
+  double rho, mu;
+} fvo_cfg;
+
+typedef struct fvo_flow {
+  fvo_mesh *m;
+  fvo_cfg cfg;
+  int n_vc;
+  double *u;      /* velocity, (n_vbf+1)*n_vc: cells then boundary slots */
+  double *p;      /* n_vol+1 */
+  double *dpdx;   /* (n_vol+1)*3 */
+  double *spd, *frc_dns;  /* n_fce+1 */
+  double *cs, *frc_vol;   /* n_vol+1 */
+  double *frc_c, *dns_c, *vis_c; /* per cell (n_vol+1) */
+  double *frc_b, *dns_b, *vis_b; /* per boundary slot (n_vbf+1), index ib */
+  double *co_num;                /* per cell, 0 in steady runs */
+  fvo_sys A;
+  double res_n_u;   /* ieq%res_n(1) */
+  double pp_res_n;  /* saved local of cfd_setup_pp */
+  double mx_res_n, c_in, c_out;
+  /* last-iteration diagnostics (the reference's print lines) */
+  double h_reduc, h_redpp, h_redpp_sum, h_red_p, h_red_p_sum;
+  double res0_n_u[3], res0_n_pp, res0_n_p;
+  int it_u[3], it_pp, it_p;
+} fvo_flow;
+
+fvo_flow *fvo_flow_new(fvo_mesh *m, const fvo_cfg *cfg);
+void fvo_flow_free(fvo_flow *f);
+/* Set the velocity of every boundary slot whose bnd_ty equals code. */
+void fvo_flow_set_bnd_u(fvo_flow *f, int code, const double *u3);
+void fvo_grad(const fvo_mesh *m, int n_vc, const double *phi /* (n_vbf+1)*n_vc */,
+              double *grad /* (n_vol+1)*n_vc*3 */, int n_iter, int neu);
+int fvo_setup_eqn(fvo_flow *f, int it);
+int fvo_setup_pp(fvo_flow *f, int it);
+/* One SIMPLE outer iteration (cfd_solve.f90:134-178, plain branch). */
+int fvo_outer(fvo_flow *f, int it, double *hist5 /* reduc, redpp, red_p, REDUC, sum */);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

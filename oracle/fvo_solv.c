@@ -1,0 +1,201 @@
+/*
+ * fvo_solv.c — oracle restatement of the linear-solver layer
+ * (TEST INFRASTRUCTURE).
+ *
+ *   solv_res      spray-simulation/solv_res.f90:5-30
+ *   solv_precond  spray-simulation/solv_precond.f90:5-30
+ *   solv_direct   spray-simulation/solv_direct.f90:5-43
+ *   solv_cg       spray-simulation/solv_cg.f90:7-87
+ *   solv_bicg     spray-simulation/solv_bicg.f90:7-121
+ *
+ * The matrix is the reference's face-based form: diagonal pl(i) and, per
+ * interior face f with owner ip < neighbour in, ng(f)%p in row ip / column in
+ * and ng(f)%n in row in / column ip (solv_data.f90:14-22).
+ *
+ * pinned: golden vectors of the flang-built reference calling each routine
+ * (tests/golden), a dense solve for CG/BiCGStab on small systems, the dense
+ * (D+L)D^-1(D+U) reconstruction for DIC/DILU, and textbook ILU(0) on
+ * triangle-free hex graphs.
+ */
+#include "fvo.h"
+
+#include <math.h>
+#include <stdlib.h>
+
+static double dot1(const double *a, const double *b, int n) {
+  double s = 0.0;
+  for (int i = 1; i <= n; ++i) s = s + a[i] * b[i];
+  return s;
+}
+
+void fvo_solv_res(const fvo_sys *A, int n_vc, const double *phi, int ld, double *res0,
+                  double *res0_n, int not_r0) {
+  const fvo_mesh *m = A->m;
+  int n = m->n_vol;
+  if (!not_r0) {
+    for (int ip = 1; ip <= n; ++ip)
+      for (int c = 0; c < n_vc; ++c)
+        res0[c * n + ip - 1] = res0[c * n + ip - 1] - A->pl[ip] * phi[ip * ld + c];
+    for (int f = 1; f <= m->n_fce; ++f) {
+      int ip = m->f_ip[f], in = m->f_in[f];
+      if (in <= 0) continue;
+      for (int c = 0; c < n_vc; ++c) {
+        res0[c * n + ip - 1] = res0[c * n + ip - 1] - A->ng_p[f] * phi[in * ld + c];
+        res0[c * n + in - 1] = res0[c * n + in - 1] - A->ng_n[f] * phi[ip * ld + c];
+      }
+    }
+  }
+  for (int c = 0; c < n_vc; ++c) res0_n[c] = 0.0;
+  for (int ip = 1; ip <= n; ++ip)
+    for (int c = 0; c < n_vc; ++c) res0_n[c] = res0_n[c] + fabs(res0[c * n + ip - 1]);
+}
+
+void fvo_solv_precond(const fvo_sys *A, int imat, double *c, double *c_r) {
+  const fvo_mesh *m = A->m;
+  double sm = fvo_small();
+  for (int ip = 1; ip <= m->n_vol; ++ip) {
+    c[ip] = A->pl[ip];
+    if (imat > 0) {
+      for (int s = m->v_fptr[ip]; s < m->v_fptr[ip + 1]; ++s) {
+        int in = m->v_ng[s];
+        if (in > 0 && in < ip) {
+          int f = m->v_fce[s];
+          if (imat == 1) c[ip] = c[ip] - A->ng_n[f] * A->ng_n[f] * c[in];
+          else if (imat == 2) c[ip] = c[ip] - A->ng_p[f] * c[in] * A->ng_n[f];
+        }
+      }
+    }
+    c_r[ip] = c[ip];
+    c[ip] = 1.0 / (c[ip] + sm);
+  }
+}
+
+void fvo_solv_direct(const fvo_sys *A, int imat, const double *c, const double *c_r, double *x,
+                     const double *b) {
+  const fvo_mesh *m = A->m;
+  int n = m->n_vol;
+  if (imat > 0) {
+    for (int ip = 1; ip <= n; ++ip) {
+      x[ip] = b[ip];
+      for (int s = m->v_fptr[ip]; s < m->v_fptr[ip + 1]; ++s) {
+        int in = m->v_ng[s];
+        if (in > 0 && in < ip) x[ip] = x[ip] - A->ng_n[m->v_fce[s]] * x[in];
+      }
+      x[ip] = x[ip] * c[ip];
+    }
+    for (int ip = 1; ip <= n; ++ip) x[ip] = x[ip] * c_r[ip];
+    for (int ip = n; ip >= 1; --ip) {
+      for (int s = m->v_fptr[ip]; s < m->v_fptr[ip + 1]; ++s) {
+        int in = m->v_ng[s];
+        if (in > 0 && in > ip) x[ip] = x[ip] - A->ng_p[m->v_fce[s]] * x[in];
+      }
+      x[ip] = x[ip] * c[ip];
+    }
+  } else {
+    for (int ip = 1; ip <= n; ++ip) x[ip] = b[ip] * c[ip];
+  }
+}
+
+/* z = pl*p, then the face scatter (solv_cg.f90:56-64) */
+void fvo_spmv(const fvo_sys *A, const double *x, double *y) {
+  const fvo_mesh *m = A->m;
+  for (int i = 1; i <= m->n_vol; ++i) y[i] = A->pl[i] * x[i];
+  for (int f = 1; f <= m->n_fce; ++f) {
+    int ip = m->f_ip[f], in = m->f_in[f];
+    if (in <= 0) continue;
+    y[ip] = y[ip] + A->ng_p[f] * x[in];
+    y[in] = y[in] + A->ng_n[f] * x[ip];
+  }
+}
+
+#define V1(n) ((double *)calloc((size_t)(n) + 1, sizeof(double)))
+
+int fvo_solv_cg(const fvo_sys *A, int imat, double reduc, int n_iter, int n_vc, double *phi,
+                int ld, double *res0, double *res0_n, int not_r0, int *iters, double *hist) {
+  int n = A->m->n_vol;
+  double sm = fvo_small(), lg = fvo_large();
+  fvo_solv_res(A, n_vc, phi, ld, res0, res0_n, not_r0);
+  double mx = res0_n[0];
+  for (int c = 1; c < n_vc; ++c) mx = res0_n[c] > mx ? res0_n[c] : mx;
+  for (int c = 0; c < n_vc; ++c) if (iters) iters[c] = 0;
+  if (mx < sm) return 0;
+  double *cc = V1(n), *cr = V1(n), *r = V1(n), *p = V1(n), *z = V1(n);
+  fvo_solv_precond(A, imat, cc, cr);
+  for (int ivc = 0; ivc < n_vc; ++ivc) {
+    double res_n = 0.0, alpha = 0.0, beta = 0.0, s = lg;
+    for (int i = 1; i <= n; ++i) { r[i] = res0[ivc * n + i - 1]; p[i] = 0.0; z[i] = 0.0; }
+    int num = 0;
+    for (int it = 1; it <= n_iter; ++it) {
+      ++num;
+      fvo_solv_direct(A, imat, cc, cr, z, r);
+      beta = 1.0 / s;
+      s = dot1(r, z, n);
+      beta = beta * s;
+      for (int i = 1; i <= n; ++i) p[i] = z[i] + beta * p[i];
+      fvo_spmv(A, p, z);
+      alpha = dot1(p, z, n);
+      alpha = s / (alpha + sm);
+      for (int i = 1; i <= n; ++i) phi[i * ld + ivc] = phi[i * ld + ivc] + alpha * p[i];
+      for (int i = 1; i <= n; ++i) r[i] = r[i] - alpha * z[i];
+      res_n = 0.0;
+      for (int i = 1; i <= n; ++i) res_n = res_n + fabs(r[i]);
+      res_n = res_n / (res0_n[ivc] + sm);
+      if (hist) hist[ivc * n_iter + it - 1] = res_n;
+      if (res_n < reduc) break;
+      res_n = 0.0;
+    }
+    if (iters) iters[ivc] = num;
+  }
+  free(cc); free(cr); free(r); free(p); free(z);
+  return 0;
+}
+
+int fvo_solv_bicg(const fvo_sys *A, int imat, double reduc, int n_iter, int n_vc, double *phi,
+                  int ld, double *res0, double *res0_n, int not_r0, int *iters, double *hist) {
+  int n = A->m->n_vol;
+  double sm = fvo_small();
+  fvo_solv_res(A, n_vc, phi, ld, res0, res0_n, not_r0);
+  double mx = res0_n[0];
+  for (int c = 1; c < n_vc; ++c) mx = res0_n[c] > mx ? res0_n[c] : mx;
+  for (int c = 0; c < n_vc; ++c) if (iters) iters[c] = 0;
+  if (mx < sm) return 0;
+  double *cc = V1(n), *cr = V1(n);
+  double *r = V1(n), *p = V1(n), *u = V1(n), *v = V1(n), *z = V1(n);
+  fvo_solv_precond(A, imat, cc, cr);
+  for (int ivc = 0; ivc < n_vc; ++ivc) {
+    /* w_sc: res_n=1 alpha=2 beta0=3 beta=4 gam=5 omeg=6 */
+    double w1 = 0.0, w2 = 1.0, w3 = 1.0, w4 = 0.0, w5 = 1.0, w6 = 0.0;
+    const double *r0 = &res0[ivc * n - 1]; /* r0[i], i 1-based */
+    for (int i = 1; i <= n; ++i) { r[i] = r0[i]; p[i] = u[i] = v[i] = z[i] = 0.0; }
+    int num = 0;
+    for (int it = 1; it <= n_iter; ++it) {
+      ++num;
+      w4 = dot1(r, r0, n);
+      w6 = w4 * w5 / (w2 * w3 + sm);
+      w3 = w4;
+      for (int i = 1; i <= n; ++i) p[i] = r[i] + w6 * (p[i] - w2 * u[i]);
+      fvo_solv_direct(A, imat, cc, cr, z, p);
+      fvo_spmv(A, z, u);
+      w5 = dot1(u, r0, n);
+      w5 = w4 / (w5 + sm);
+      for (int i = 1; i <= n; ++i) phi[i * ld + ivc] = phi[i * ld + ivc] + w5 * z[i];
+      for (int i = 1; i <= n; ++i) r[i] = r[i] - w5 * u[i];
+      fvo_solv_direct(A, imat, cc, cr, z, r);
+      fvo_spmv(A, z, v);
+      w1 = dot1(v, r, n);
+      w2 = dot1(v, v, n);
+      w2 = w1 / (w2 + sm);
+      for (int i = 1; i <= n; ++i) phi[i * ld + ivc] = phi[i * ld + ivc] + w2 * z[i];
+      for (int i = 1; i <= n; ++i) r[i] = r[i] - w2 * v[i];
+      w1 = 0.0;
+      for (int i = 1; i <= n; ++i) w1 = w1 + fabs(r[i]);
+      w1 = w1 / (res0_n[ivc] + sm);
+      if (hist) hist[ivc * n_iter + it - 1] = w1;
+      if (w1 < reduc) break;
+      w1 = 0.0;
+    }
+    if (iters) iters[ivc] = num;
+  }
+  free(cc); free(cr); free(r); free(p); free(u); free(v); free(z);
+  return 0;
+}

@@ -1,0 +1,145 @@
+"""Generate the golden fixtures under tests/golden/ from the flang-built reference.
+
+Run in the development container only (needs /root/reference and flang):
+
+    python tests/golden/make_golden.py
+
+Steps: build oracle/_ref/{shipped,relaxed}/fvk_ref_driver with
+oracle/build_ref.sh, write each case's mesh as a Gambit neutral file
+(generated boxes) or use the reference's own .neu fixture, run the driver,
+and store inputs plus the driver's dump and parsed print-line histories as
+<case>.npz.  The fixtures are data (inputs and expected outputs); no reference
+source is stored.
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+from oracle import fvo  # noqa: E402
+
+REF = os.environ.get("FVK_REFERENCE_DIR", "/root/reference/spray-simulation")
+
+# case: (mesh, variant, n_outer, rho, mu, urf_p, boundary velocities)
+CASES = {
+    "cav5_shipped": dict(mesh=("neu", "cavity_3d.neu"), variant="shipped", n_outer=10,
+                         rho=1.0, mu=0.01, urf_p=0.1, bc=[(31, (1.0, 0.0, 0.0))]),
+    "cav5_relaxed": dict(mesh=("neu", "cavity_3d.neu"), variant="relaxed", n_outer=20,
+                         rho=1.0, mu=0.01, urf_p=0.3, bc=[(31, (1.0, 0.0, 0.0))]),
+    "chan2d_shipped": dict(mesh=("neu", "channel_2d.neu"), variant="shipped", n_outer=6,
+                           rho=1.0, mu=0.05, urf_p=0.1, bc=[(11, (1.0, 0.0, 0.0))]),
+    "chan2d_relaxed": dict(mesh=("neu", "channel_2d.neu"), variant="relaxed", n_outer=12,
+                           rho=1.0, mu=0.05, urf_p=0.3, bc=[(11, (1.0, 0.0, 0.0))]),
+    "box8j_shipped": dict(mesh=("box", (8, 8, 8, 0.2, 0x5EED0001)), variant="shipped", n_outer=6,
+                          rho=1.0, mu=0.01, urf_p=0.1, bc=[(31, (1.0, 0.0, 0.0))]),
+    "box12_relaxed": dict(mesh=("box", (12, 12, 12, 0.0, 0)), variant="relaxed", n_outer=30,
+                          rho=1.0, mu=0.01, urf_p=0.3, bc=[(31, (1.0, 0.0, 0.0))]),
+    "box10j_relaxed": dict(mesh=("box", (10, 10, 10, 0.1, 0x5EED0002)), variant="relaxed", n_outer=15,
+                           rho=1.0, mu=0.01, urf_p=0.3, bc=[(31, (1.0, 0.0, 0.0))]),
+    "quad16_relaxed": dict(mesh=("box", (16, 16, 0, 0.2, 0x5EED0003)), variant="relaxed", n_outer=20,
+                           rho=1.0, mu=0.01, urf_p=0.3, bc=[(31, (1.0, 0.0, 0.0))]),
+}
+
+
+def box_raw(nx, ny, nz, jitter, seed):
+    """Cells/vertices of the oracle's box generator, coordinates rounded to the
+    13 significant digits a Gambit E20.12 field carries."""
+    m = fvo.OMesh.box(nx, ny, nz, jitter=jitter, seed=seed)
+    mm = m.m
+    vx = fvo.arr(mm.vx, 3 * (mm.n_vrt + 1)).reshape(-1, 3)[1:]
+    vx = np.array([[float("%.12e" % v) for v in row] for row in vx])
+    nv = mm.n_vol
+    vptr = fvo.arr(mm.v_vptr, nv + 2, np.int32)[1:] - 1
+    vrt = fvo.arr(mm.v_vrt, vptr[-1] + 1, np.int32)[1:]
+    ty = fvo.arr(mm.v_ty, nv + 1, np.int32)[1:]
+    fptr = fvo.arr(mm.v_fptr, nv + 2, np.int32)[1:] - 1
+    bnd = fvo.arr(mm.v_bnd, fptr[-1] + 1, np.int32)[1:]
+    n_dx = mm.n_dx
+    m.free()
+    return dict(n_dx=n_dx, vx=vx, v_ty=ty, v_vptr=vptr, v_vrt=vrt, bnd=bnd, bnd_ptr=fptr)
+
+
+def write_neu(path, r):
+    """Gambit neutral file in the fixed formats grd_read expects
+    (grd_read.f90:28,42,59,83,85)."""
+    n_dx, vx, ty, vptr, vrt = r["n_dx"], r["vx"], r["v_ty"], r["v_vptr"], r["v_vrt"]
+    bnd, bptr = r["bnd"], r["bnd_ptr"]
+    nv = len(ty)
+    codes = sorted(set(int(c) for c in bnd if c > 0))
+    names = {c: {1: "inlet", 2: "outlet", 3: "wall", 4: "symmetry", 5: "pressure"}[c // 10] + "_%d" % (c % 10)
+             for c in codes}
+    L = ["        CONTROL INFO 2.3.16", "** GAMBIT NEUTRAL FILE", "generated", "PROGRAM:  fvk", "",
+         "     NUMNP     NELEM     NGRPS    NBSETS     NDFCD     NDFVL",
+         "".join(" %9d" % v for v in (len(vx), nv, 1, len(codes), n_dx, n_dx)), "ENDOFSECTION",
+         "   NODAL COORDINATES 2.3.16"]
+    for k, x in enumerate(vx):
+        L.append("%10d " % (k + 1) + "".join("%20.12E" % x[d] for d in range(n_dx)))
+    L += ["ENDOFSECTION", "      ELEMENTS/CELLS 2.3.16"]
+    for i in range(nv):
+        vs = vrt[vptr[i]:vptr[i + 1]]
+        s = "%8d %2d %2d " % (i + 1, ty[i], len(vs))
+        first, rest = vs[:7], vs[7:]
+        L.append(s + "".join("%8d" % v for v in first))
+        while len(rest):
+            L.append(" " * 15 + "".join("%8d" % v for v in rest[:7]))
+            rest = rest[7:]
+    L.append("ENDOFSECTION")
+    for c in codes:
+        ent = []
+        for i in range(nv):
+            for k, cc in enumerate(bnd[bptr[i]:bptr[i + 1]]):
+                if cc == c:
+                    ent.append((i + 1, ty[i], k + 1))
+        L.append(" BOUNDARY CONDITIONS 2.3.16")
+        L.append("%32s%10d%10d%10d%10d" % (names[c], 1, len(ent), 0, 6))
+        for e in ent:
+            L.append("%10d%5d%5d" % e)
+        L.append("ENDOFSECTION")
+    with open(path, "w") as fh:
+        fh.write("\n".join(L) + "\n")
+
+
+def run_case(name, spec, tmp):
+    kind, arg = spec["mesh"]
+    if kind == "neu":
+        neu = os.path.join(REF, arg)
+        raw = None
+    else:
+        raw = box_raw(*arg)
+        neu = os.path.join(tmp, name + ".neu")
+        write_neu(neu, raw)
+    drv = os.path.join(ROOT, "oracle", "_ref", spec["variant"], "fvk_ref_driver")
+    out = os.path.join(tmp, name + ".bin")
+    args = [drv, os.path.join(REF, "gambit.vrt"), neu, out, str(spec["n_outer"]),
+            repr(spec["rho"]), repr(spec["mu"]), repr(spec["urf_p"])]
+    for code, u in spec["bc"]:
+        args += [str(code)] + [repr(float(v)) for v in u]
+    p = subprocess.run(args, capture_output=True, text=True, check=True, cwd=tmp)
+    dump = fvo.read_dump(out)
+    hist = fvo.parse_history(p.stdout)
+    rec = {"d_" + k: v for k, v in dump.items()}
+    rec.update({"h_" + k: v for k, v in hist.items()})
+    rec["case_json"] = np.array(repr({k: v for k, v in spec.items()}))
+    if raw is not None:
+        rec.update({"mesh_" + k: np.asarray(v) for k, v in raw.items()})
+    else:
+        rec["mesh_neu"] = np.array(arg)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **rec)
+    print("%-16s n_vol=%d n_fce=%d outer=%d -> %s.npz" % (name, dump["n_vol"][0], dump["n_fce"][0],
+                                                       spec["n_outer"], name))
+
+
+def main():
+    subprocess.run([os.path.join(ROOT, "oracle", "build_ref.sh")], check=True)
+    with tempfile.TemporaryDirectory() as tmp:
+        for name, spec in CASES.items():
+            run_case(name, spec, tmp)
+
+
+if __name__ == "__main__":
+    main()
