@@ -1,0 +1,254 @@
+/*
+ * fvk — MI355X (gfx950) implementation of the spray-simulation SIMPLE hot
+ * path behind a plain C ABI.
+ *
+ * Conventions
+ *   - Every function returns an int status: FVK_OK (0), > 0 recoverable
+ *     (a fallback was taken), < 0 fatal (nothing was changed that the caller
+ *     can rely on).  fvk_last_error() describes the last failure on the
+ *     calling thread.  No exception crosses the ABI.
+ *   - Entity numbering on the ABI is the reference's: cells 1..n_vol,
+ *     faces 1..n_fce in grd_conn order, in = 0 for a boundary face,
+ *     boundary values addressed by ib = n_vol+1 .. n_vol+n_bf
+ *     (spray-simulation/grd_data.f90:22-53, grd_geom.f90:46-69).
+ *   - Vectors per entity are contiguous in the reference's order
+ *     (x(1:n_dx) of face 1, then face 2, ...).  Multi-component fields
+ *     phi(i)%vc(c) are passed entity-major, stride n_vc; residual arrays
+ *     res0(n_vol, n_vc) column-major as the reference declares them.
+ *   - Host buffers are borrowed for the duration of a call; device memory is
+ *     owned by the context.  A context is used from one host thread at a
+ *     time and runs on its own HIP stream.
+ */
+#ifndef FVK_H
+#define FVK_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FVK_OK 0
+#define FVK_FALLBACK 1        /* learned/mixed path fell back to plain PCG */
+#define FVK_NOT_CONVERGED 2   /* iteration cap reached before the tolerance */
+#define FVK_ERR_ARG (-1)
+#define FVK_ERR_HIP (-2)
+#define FVK_ERR_STATE (-3)
+#define FVK_ERR_DIVERGED (-4) /* |normalised residual| > large: the reference stops here */
+#define FVK_ERR_IO (-5)
+#define FVK_ERR_NODEV (-6)
+
+const char *fvk_last_error(void);
+int fvk_version(void);
+
+/* ------------------------------------------------------------------------
+ * Mesh (host): the grd_read / grd_conn / grd_geom layer.  Output arrays are
+ * bit-identical to the reference's grd structure; grd_conn's O(n^2) vertex
+ * matching is replaced by an O(N) hash with the same face ordering.
+ * ---------------------------------------------------------------------- */
+typedef struct fvk_mesh fvk_mesh;
+
+typedef struct fvk_mesh_info {
+  int n_dx, n_vrt, n_vol, n_fce, n_bf, n_slot, max_fce;
+} fvk_mesh_info;
+
+/* grd_read (spray-simulation/grd_read.f90:9-110) + grd_conn + grd_geom.
+   l_sc scales coordinates (grd_data.f90:9). */
+int fvk_mesh_read_neu(const char *path, double l_sc, fvk_mesh **out);
+/* Box of nx*ny*nz hexes (nz = 0: nx*ny quads) on [0,lx]x[0,ly]x[0,lz].
+   Interior vertices are moved by U(-jitter, jitter)*h per coordinate, drawn
+   from SplitMix64(seed) in vertex order x, y, z.  Faces on z = lz (y = ly in
+   2D) get boundary code lid_code, the others wall_code. */
+int fvk_mesh_box(int nx, int ny, int nz, double lx, double ly, double lz, double jitter,
+                 uint64_t seed, int lid_code, int wall_code, fvk_mesh **out);
+/* Cells given directly: cell i (1-based) has Gambit element type v_ty[i-1]
+   and vertices v_vrt[v_vptr[i-1] .. v_vptr[i]-1] (1-based vertex ids);
+   bnd holds one boundary code per (cell, local face), 0 = none. */
+int fvk_mesh_from_cells(int n_dx, int n_vrt, const double *vx3, int n_vol, const int *v_ty,
+                        const int *v_vptr, const int *v_vrt, const int *bnd, fvk_mesh **out);
+int fvk_mesh_get_info(const fvk_mesh *m, fvk_mesh_info *info);
+/* Topology as the reference holds it: fce%ip, fce%in, fce%ib, fce%bnd_ty
+   (n_fce each), cell_ptr (n_vol+1, 0-based offsets) and the per-cell
+   fce_lst / ng_lst (n_slot each, grd_conn.f90:128-152, 218-238). */
+int fvk_mesh_get_topology(const fvk_mesh *m, int *fce_ip, int *fce_in, int *fce_ib, int *fce_bty,
+                          int *cell_ptr, int *fce_lst, int *ng_lst);
+/* Geometry (grd_geom.f90): vol_x (n_vol*n_dx), vol_v, fce_x, fce_n, fce_dxp,
+   fce_dxn (n_fce*n_dx; dx_n is 0 on boundary faces), fce_s, fce_lpn, fce_wfp. */
+int fvk_mesh_get_geometry(const fvk_mesh *m, double *vol_x, double *vol_v, double *fce_x,
+                          double *fce_n, double *fce_dxp, double *fce_dxn, double *fce_s,
+                          double *fce_lpn, double *fce_wfp);
+void fvk_mesh_free(fvk_mesh *m);
+
+/* ------------------------------------------------------------------------
+ * Device context: topology and geometry resident in HBM.
+ * Replaces grd_solv (spray-simulation/grd_solv.f90:7-24): the cell-face
+ * lists become a slot-major ELL table, plus level sets for the triangular
+ * sweeps of solv_precond / solv_direct.
+ * ---------------------------------------------------------------------- */
+typedef struct fvk_ctx fvk_ctx;
+
+#define FVK_CTX_RCM 1 /* store cells in reverse Cuthill-McKee order */
+
+int fvk_ctx_create(const fvk_mesh *m, int device, int flags, fvk_ctx **out);
+/* Same, from the arrays the reference's grd structure holds after grd_geom
+   (the Fortran drop-in path; layouts as fvk_mesh_get_topology/geometry). */
+int fvk_ctx_create_arrays(int n_dx, int n_vol, int n_fce, int n_bf, const int *fce_ip,
+                          const int *fce_in, const int *fce_ib, const int *fce_bty,
+                          const int *cell_ptr, const int *fce_lst, const double *vol_x,
+                          const double *vol_v, const double *fce_x, const double *fce_n,
+                          const double *fce_dxp, const double *fce_dxn, const double *fce_s,
+                          const double *fce_lpn, const double *fce_wfp, int device, int flags,
+                          fvk_ctx **out);
+void fvk_ctx_destroy(fvk_ctx *ctx);
+int fvk_ctx_sync(fvk_ctx *ctx);
+/* Cell order used on the device: perm[k] = reference cell id (1-based) of
+   device row k; identity unless FVK_CTX_RCM. */
+int fvk_ctx_get_perm(const fvk_ctx *ctx, int *perm);
+/* Number of forward level sets of the lower-triangular sweep. */
+int fvk_ctx_levels(const fvk_ctx *ctx, int *n_fwd, int *n_bwd);
+
+/* ------------------------------------------------------------------------
+ * Linear solvers on the face-based matrix: diagonal pl(n_vol) and per face
+ * ng%p / ng%n (spray-simulation/solv_data.f90:14-22).
+ * ---------------------------------------------------------------------- */
+/* Upload pl, ng%p, ng%n (host, reference order). */
+int fvk_set_matrix(fvk_ctx *ctx, const double *pl, const double *ng_p, const double *ng_n);
+/* solv_precond (spray-simulation/solv_precond.f90:5-30): imat 0 Jacobi,
+   1 DIC, 2 DILU; c = 1/(d+small), c_r = d. */
+int fvk_solv_precond(fvk_ctx *ctx, int imat, double *c, double *c_r);
+/* solv_direct (spray-simulation/solv_direct.f90:5-43): x = M^-1 b with the
+   factor c, c_r as solv_precond returns it; c = c_r = NULL builds the factor
+   of the current matrix. */
+int fvk_solv_direct(fvk_ctx *ctx, int imat, const double *c, const double *c_r, const double *b,
+                    double *x);
+/* solv_res (spray-simulation/solv_res.f90:5-30): res0 <- res0 - A phi unless
+   not_r0, res0_n = per-component L1 norm. */
+int fvk_solv_res(fvk_ctx *ctx, int n_vc, const double *phi, double *res0, double *res0_n,
+                 int not_r0);
+
+typedef struct fvk_stats {
+  int iters[3];      /* iterations per component */
+  double res_rel[3]; /* final relative L1 residual per component */
+  int fallback;      /* learned/mixed path fell back */
+  double seconds;    /* device time of the solve */
+} fvk_stats;
+
+/* solv_cg (spray-simulation/solv_cg.f90:7-87) and solv_bicg
+   (spray-simulation/solv_bicg.f90:7-121).  phi: n_vol x n_vc entity-major
+   (in/out), res0: column-major (in: RHS, out: initial residual),
+   res0_n: out.  The n_vc components share the matrix and factor; they are
+   solved as one batched launch sequence, each with its own scalars. */
+int fvk_solv_cg(fvk_ctx *ctx, int imat, double reduc, int n_iter, int n_vc, double *phi,
+                double *res0, double *res0_n, int not_r0, fvk_stats *st);
+int fvk_solv_bicg(fvk_ctx *ctx, int imat, double reduc, int n_iter, int n_vc, double *phi,
+                  double *res0, double *res0_n, int not_r0, fvk_stats *st);
+
+/* solv_grad (spray-simulation/cfd_setup.f90:765-866): phi has n_vol+n_bf
+   entries of n_vc components (cells, then boundary slots ib); grad is
+   n_vol x n_vc x n_dx, entity-major (grad[(i*n_vc+c)*n_dx+d]). */
+int fvk_solv_grad(fvk_ctx *ctx, int n_vc, const double *phi, double *grad, int n_iter, int neu);
+
+/* ------------------------------------------------------------------------
+ * Stand-alone pressure systems held in HBM (no host transfer inside a
+ * solve).  The matrix is the symmetric face-based form of the pressure
+ * correction (cfd_setup.f90:623-634): A_ij = -a_f, A_ii = sum of a_f.
+ * ---------------------------------------------------------------------- */
+typedef struct fvk_solve_info {
+  int iters;
+  int fallback;          /* learned/mixed path fell back to plain PCG */
+  double seconds;        /* device time of the solve (HIP events) */
+  double res_rec;        /* final relative L1 norm of the recurrence residual */
+  double res_true_l1;    /* |b - A x|_1 / |b|_1 in FP64 */
+  double res_true_l2;    /* |b - A x|_2 / |b|_2 in FP64 */
+  double kern_seconds;   /* mean duration of the dominant kernel (time_kernel != 0) */
+  double kern_bytes;     /* its algorithmic bytes per launch */
+  long kern_launches;
+} fvk_solve_info;
+
+#define FVK_SOLVE_PCG64 0 /* the reference PCG recurrence in FP64 (solv_cg.f90) */
+
+/* a_f = s/l_pn * exp(sigma*z_f), z_f ~ N(0,1) from SplitMix64(seed) by face
+   index; b ~ N(0,1) by cell, projected to zero mean.  b_out (reference
+   order, may be NULL) receives the RHS. */
+int fvk_sys_random(fvk_ctx *ctx, uint64_t seed, double sigma, double *b_out);
+/* Symmetric system from host arrays: pl, a (= -ng%p = -ng%n per face), b. */
+int fvk_sys_set(fvk_ctx *ctx, const double *pl, const double *ng, const double *b);
+/* Solve from x = 0 until the method's relative L1 residual is below tol (or
+   max_iter).  imat selects the plain preconditioner (0 Jacobi, 1 DIC). */
+int fvk_sys_solve(fvk_ctx *ctx, int method, int imat, double tol, int max_iter, int time_kernel,
+                  fvk_solve_info *info);
+int fvk_sys_get_x(fvk_ctx *ctx, double *x);
+/* The loaded system in reference order: pl (n_vol), a (n_fce, 0 on boundary
+   faces), b (n_vol); NULL pointers skip. */
+int fvk_sys_get(fvk_ctx *ctx, double *pl, double *a, double *b);
+/* Mean duration of one FP64 SpMV launch over reps launches (HIP events). */
+int fvk_bench_spmv(fvk_ctx *ctx, int reps, double *sec_per_launch);
+/* Mean duration of a device copy of nbytes (kernel k_copy, 16-byte vector
+   loads and stores): the achievable-bandwidth reference and the calibration
+   kernel for hardware byte counters.  Allocates two scratch buffers. */
+int fvk_bench_copy(int device, long nbytes, int reps, double *sec_per_launch);
+
+/* ------------------------------------------------------------------------
+ * SIMPLE: momentum assembly + BiCGStab (cfd_setup_eqn,
+ * spray-simulation/cfd_setup.f90:22-544) and pressure correction
+ * (cfd_setup_pp, :549-760), plain single-phase steady branch.
+ * ---------------------------------------------------------------------- */
+typedef struct fvk_flow fvk_flow;
+
+typedef struct fvk_flow_cfg {
+  int simpler;      /* 1: SIMPLER pressure solve (shipped, cfd_setup.f90:581) */
+  int imat_u;       /* momentum preconditioner (shipped 2 = DILU) */
+  int imat_p;       /* pressure preconditioner (shipped 1 = DIC) */
+  int n_bicg;       /* shipped 5 */
+  int n_cg;         /* shipped 20 */
+  int n_grad;       /* shipped 2 */
+  int c_sch;        /* 0 CDS, 1 minmod, 2 van Leer, 3 superbee, >3 HRIC */
+  double urf_u, urf_p, reduc_u, reduc_p, c_bl;
+  double rho, mu;
+  int p_solver;     /* 0: reference PCG recurrence (parity); 1: FP32-inner
+                       defect correction; 2: learned preconditioner */
+  double p_tol;     /* p_solver > 0: FP64 true relative residual target */
+} fvk_flow_cfg;
+
+typedef struct fvk_outer_hist {
+  double reduc;     /* 'reduc:' line, cfd_setup.f90:540 */
+  double redpp;     /* 'redpp:' line, :672 */
+  double redpp_sum; /* sum(rhs) on that line */
+  double red_p;     /* 'red_p:' line, :751 (SIMPLER only) */
+  double red_p_sum;
+  double mx_res_n;  /* 'REDUC:' line, cfd_solve.f90:176-177 */
+  int it_u[3], it_pp, it_p;
+  double res0_n_u[3]; /* initial L1 norms of the momentum solve */
+  double res0_n_pp;   /* ... of the p' solve */
+  double res0_n_p;    /* ... of the SIMPLER pressure solve */
+} fvk_outer_hist;
+
+int fvk_flow_create(fvk_ctx *ctx, const fvk_flow_cfg *cfg, fvk_flow **out);
+void fvk_flow_destroy(fvk_flow *f);
+/* Replace the configuration (e.g. the urf argument of cfd_setup_pp). */
+int fvk_flow_configure(fvk_flow *f, const fvk_flow_cfg *cfg);
+/* The module variables c_in / c_out (cfd_data.f90:15): set = 1 copies the
+   given values to the device, set = 0 reads them back. */
+int fvk_flow_scalars(fvk_flow *f, int set, double *c_in, double *c_out);
+/* Boundary velocity of every boundary face whose bnd_ty equals code. */
+int fvk_flow_set_bnd_u(fvk_flow *f, int code, const double *u);
+/* Full state in reference order: u_cell (n_vol*n_dx), u_bnd (n_bf*n_dx, by
+   ib), p (n_vol), dpdx (n_vol*n_dx), spd (n_fce).  NULL pointers skip. */
+int fvk_flow_set_state(fvk_flow *f, const double *u_cell, const double *u_bnd, const double *p,
+                       const double *dpdx, const double *spd);
+int fvk_flow_get_state(fvk_flow *f, double *u_cell, double *u_bnd, double *p, double *dpdx,
+                       double *spd, double *cs, double *frc_dns);
+/* Last assembled matrix (pl, ng%p, ng%n in reference order). */
+int fvk_flow_get_matrix(fvk_flow *f, double *pl, double *ng_p, double *ng_n);
+int fvk_flow_setup_eqn(fvk_flow *f, int it, fvk_outer_hist *h);
+int fvk_flow_setup_pp(fvk_flow *f, int it, fvk_outer_hist *h);
+/* One outer iteration (cfd_solve.f90:134-178, plain branch). */
+int fvk_flow_outer(fvk_flow *f, int it, fvk_outer_hist *h);
+/* n outer iterations it0 .. it0+n-1 with the state kept in HBM; h has n
+   entries (may be NULL). */
+int fvk_flow_run(fvk_flow *f, int it0, int n, fvk_outer_hist *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

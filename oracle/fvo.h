@@ -84,6 +84,8 @@ void fvo_solv_precond(const fvo_sys *A, int imat, double *c, double *c_r);
 void fvo_solv_direct(const fvo_sys *A, int imat, const double *c, const double *c_r,
                      double *x, const double *b);
 void fvo_spmv(const fvo_sys *A, const double *x, double *y);
+/* 0: sequential reductions (the reference); 1: pairwise (sensitivity probe) */
+void fvo_set_sum_order(int order);
 int fvo_solv_cg(const fvo_sys *A, int imat, double reduc, int n_iter, int n_vc,
                 double *phi, int ld, double *res0, double *res0_n, int not_r0, int *iters,
                 double *hist /* n_vc*n_iter relative L1, may be NULL */);

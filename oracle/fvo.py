@@ -104,12 +104,27 @@ def lib():
             getattr(L, f).argtypes = [_P(Sys), C.c_int, C.c_double, C.c_int, C.c_int, dp, C.c_int,
                                       dp, dp, C.c_int, ip, dp]
         L.fvo_solv_res.argtypes = [_P(Sys), C.c_int, dp, C.c_int, dp, dp, C.c_int]
+        L.fvo_set_sum_order.argtypes = [C.c_int]
         _lib = L
     return _lib
 
 
 def small():
     return lib().fvo_small()
+
+
+class sum_order:
+    """Context manager: run the oracle with pairwise reductions (1) instead of
+    the reference's sequential ones (0)."""
+
+    def __init__(self, order):
+        self.order = order
+
+    def __enter__(self):
+        lib().fvo_set_sum_order(self.order)
+
+    def __exit__(self, *a):
+        lib().fvo_set_sum_order(0)
 
 
 def _d(a):

@@ -22,11 +22,38 @@
 #include <math.h>
 #include <stdlib.h>
 
-static double dot1(const double *a, const double *b, int n) {
-  double s = 0.0;
-  for (int i = 1; i <= n; ++i) s = s + a[i] * b[i];
+/* Summation order of dot products and L1 norms: 0 = sequential, as the
+   reference (the default); 1 = pairwise over blocks of 256, a different but
+   equally valid rounding used only to measure how sensitive a result is to
+   the order of its reductions. */
+static int g_sum_order = 0;
+void fvo_set_sum_order(int order) { g_sum_order = order; }
+
+static double pairwise(const double *t, int n) {
+  if (n <= 256) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s = s + t[i];
+    return s;
+  }
+  int h = n / 2;
+  return pairwise(t, h) + pairwise(t + h, n - h);
+}
+
+static double reduce1(const double *a, const double *b, int n, int absval) {
+  if (g_sum_order == 0) {
+    double s = 0.0;
+    for (int i = 1; i <= n; ++i) s = s + (absval ? fabs(a[i]) : a[i] * b[i]);
+    return s;
+  }
+  double *t = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  for (int i = 1; i <= n; ++i) t[i - 1] = absval ? fabs(a[i]) : a[i] * b[i];
+  double s = pairwise(t, n);
+  free(t);
   return s;
 }
+
+static double dot1(const double *a, const double *b, int n) { return reduce1(a, b, n, 0); }
+static double l1(const double *a, int n) { return reduce1(a, a, n, 1); }
 
 void fvo_solv_res(const fvo_sys *A, int n_vc, const double *phi, int ld, double *res0,
                   double *res0_n, int not_r0) {
@@ -137,8 +164,7 @@ int fvo_solv_cg(const fvo_sys *A, int imat, double reduc, int n_iter, int n_vc, 
       alpha = s / (alpha + sm);
       for (int i = 1; i <= n; ++i) phi[i * ld + ivc] = phi[i * ld + ivc] + alpha * p[i];
       for (int i = 1; i <= n; ++i) r[i] = r[i] - alpha * z[i];
-      res_n = 0.0;
-      for (int i = 1; i <= n; ++i) res_n = res_n + fabs(r[i]);
+      res_n = l1(r, n);
       res_n = res_n / (res0_n[ivc] + sm);
       if (hist) hist[ivc * n_iter + it - 1] = res_n;
       if (res_n < reduc) break;
@@ -187,8 +213,7 @@ int fvo_solv_bicg(const fvo_sys *A, int imat, double reduc, int n_iter, int n_vc
       w2 = w1 / (w2 + sm);
       for (int i = 1; i <= n; ++i) phi[i * ld + ivc] = phi[i * ld + ivc] + w2 * z[i];
       for (int i = 1; i <= n; ++i) r[i] = r[i] - w2 * v[i];
-      w1 = 0.0;
-      for (int i = 1; i <= n; ++i) w1 = w1 + fabs(r[i]);
+      w1 = l1(r, n);
       w1 = w1 / (res0_n[ivc] + sm);
       if (hist) hist[ivc * n_iter + it - 1] = w1;
       if (w1 < reduc) break;

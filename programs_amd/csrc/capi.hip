@@ -1,0 +1,997 @@
+// C ABI of the fvk library (include/fvk.h): mesh, device context, solver,
+// gradient and SIMPLE entry points.  Host arrays use the reference's
+// numbering and layouts; everything on the device is in row order.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "flow.h"
+#include "mesh.h"
+
+struct fvk_mesh {
+  fvk::Mesh m;
+};
+struct fvk_ctx {
+  fvk::Ctx c;
+};
+struct fvk_flow {
+  fvk::Flow f;
+};
+
+namespace fvk {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+
+int mom_setup(Flow &F, int it, bool &skipped);
+int pp_setup(Flow &F, int it);
+int outer_begin(Flow &F);
+
+// ---------------------------------------------------------------- helpers
+struct Topo {  // host topology + geometry, 0-based, vectors stride 3
+  int n_dx = 0, n = 0, nf = 0, nbf = 0;
+  std::vector<int> fip, fin, fib, fbty, cptr, clst;
+  std::vector<double> vx, vv, vvr, fx, fn, fdp, fdn, fs, flpn, fwfp;
+};
+
+static Topo topo_from_mesh(const Mesh &m) {
+  Topo t;
+  t.n_dx = m.n_dx; t.n = m.n_vol; t.nf = m.n_fce; t.nbf = m.n_bf;
+  t.fip = m.f_ip; t.fin = m.f_in; t.fib = m.f_ib; t.fbty = m.f_bty;
+  t.cptr = m.v_fptr; t.clst = m.fce_lst;
+  t.vx = m.vol_x; t.vv = m.vol_v; t.vvr = m.vol_vr;
+  t.fx = m.f_x; t.fn = m.f_n; t.fdp = m.f_dxp; t.fdn = m.f_dxn;
+  t.fs = m.f_s; t.flpn = m.f_lpn; t.fwfp = m.f_wfp;
+  return t;
+}
+
+// Reverse Cuthill-McKee order of the cell graph (George & Liu): start from a
+// pseudo-peripheral cell of minimum degree, visit neighbours by ascending
+// (degree, id), reverse the visit order.  Deterministic.
+static std::vector<int> rcm_order(const Topo &t) {
+  const int n = t.n;
+  std::vector<int> deg(n, 0), xadj(n + 1, 0), adj;
+  for (int f = 0; f < t.nf; ++f)
+    if (t.fin[f] >= 0) { deg[t.fip[f]]++; deg[t.fin[f]]++; }
+  for (int i = 0; i < n; ++i) xadj[i + 1] = xadj[i] + deg[i];
+  adj.resize(xadj[n]);
+  std::vector<int> fill(n, 0);
+  for (int f = 0; f < t.nf; ++f)
+    if (t.fin[f] >= 0) {
+      int a = t.fip[f], b = t.fin[f];
+      adj[xadj[a] + fill[a]++] = b;
+      adj[xadj[b] + fill[b]++] = a;
+    }
+  for (int i = 0; i < n; ++i)
+    std::sort(adj.begin() + xadj[i], adj.begin() + xadj[i + 1],
+              [&](int a, int b) { return deg[a] != deg[b] ? deg[a] < deg[b] : a < b; });
+  std::vector<int> order, lvl(n, -1);
+  order.reserve(n);
+  std::vector<char> seen(n, 0);
+  auto bfs_last = [&](int s, std::vector<int> &q) {
+    q.clear();
+    q.push_back(s);
+    std::vector<int> mark;
+    lvl[s] = 0;
+    mark.push_back(s);
+    for (size_t h = 0; h < q.size(); ++h) {
+      int v = q[h];
+      for (int k = xadj[v]; k < xadj[v + 1]; ++k)
+        if (lvl[adj[k]] < 0) { lvl[adj[k]] = lvl[v] + 1; q.push_back(adj[k]); mark.push_back(adj[k]); }
+    }
+    int last = lvl[q.back()], best = q.back();
+    for (int v : q)
+      if (lvl[v] == last && (deg[v] < deg[best] || (deg[v] == deg[best] && v < best))) best = v;
+    int ecc = last;
+    for (int v : mark) lvl[v] = -1;
+    return std::make_pair(best, ecc);
+  };
+  std::vector<int> q;
+  for (int s0 = 0; s0 < n; ++s0) {
+    if (seen[s0]) continue;
+    // component of s0: pick its minimum-degree cell, then walk to a
+    // pseudo-peripheral one
+    bfs_last(s0, q);
+    int s = s0;
+    for (int v : q)
+      if (deg[v] < deg[s] || (deg[v] == deg[s] && v < s)) s = v;
+    int ecc = -1;
+    for (int rep = 0; rep < 4; ++rep) {
+      auto pr = bfs_last(s, q);
+      if (pr.second <= ecc) break;
+      ecc = pr.second;
+      s = pr.first;
+    }
+    size_t base = order.size();
+    order.push_back(s);
+    seen[s] = 1;
+    for (size_t h = base; h < order.size(); ++h) {
+      int v = order[h];
+      for (int k = xadj[v]; k < xadj[v + 1]; ++k)
+        if (!seen[adj[k]]) { seen[adj[k]] = 1; order.push_back(adj[k]); }
+    }
+  }
+  std::reverse(order.begin(), order.end());
+  return order;
+}
+
+template <class T>
+static int upload(Ctx &c, T **dst, const std::vector<T> &src) {
+  int rc = dalloc(c, dst, src.size());
+  if (rc) return rc;
+  if (!src.empty()) FVK_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+static int build_levels(Ctx &c, const Topo &t, const std::vector<int> &iperm, bool fwd, Levels &L) {
+  const int n = t.n;
+  std::vector<int> lev(n, 0);
+  int nl = 0;
+  for (int s = 0; s < n; ++s) {
+    const int o = fwd ? s : n - 1 - s;
+    int l = 0;
+    for (int k = t.cptr[o]; k < t.cptr[o + 1]; ++k) {
+      const int f = t.clst[k];
+      if (t.fin[f] < 0) continue;
+      const int j = t.fip[f] == o ? t.fin[f] : t.fip[f];
+      if (fwd ? j < o : j > o) l = std::max(l, lev[j] + 1);
+    }
+    lev[o] = l;
+    nl = std::max(nl, l + 1);
+  }
+  L.ptr.assign(nl + 1, 0);
+  for (int o = 0; o < n; ++o) L.ptr[lev[o] + 1]++;
+  for (int l = 0; l < nl; ++l) L.ptr[l + 1] += L.ptr[l];
+  std::vector<int> rows(n), pos(L.ptr.begin(), L.ptr.end() - 1);
+  // rows of a level in ascending row order
+  std::vector<int> byrow(n);
+  for (int o = 0; o < n; ++o) byrow[iperm[o]] = o;
+  for (int r = 0; r < n; ++r) {
+    int o = byrow[r];
+    rows[pos[lev[o]]++] = r;
+  }
+  return upload(c, &L.rows, rows);
+}
+
+static int ctx_build(Ctx &c, const Topo &t, int device, int flags) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device) {
+    set_error("no HIP device");
+    return FVK_ERR_NODEV;
+  }
+  c.device = device;
+  FVK_HIP(hipSetDevice(device));
+  FVK_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  FVK_HIP(hipEventCreate(&c.ev0));
+  FVK_HIP(hipEventCreate(&c.ev1));
+  const int n = t.n, nf = t.nf;
+  c.n = n; c.nf = nf; c.nbf = t.nbf; c.ndx = t.n_dx;
+  // row order
+  if (flags & FVK_CTX_RCM) c.perm = rcm_order(t);
+  else { c.perm.resize(n); std::iota(c.perm.begin(), c.perm.end(), 0); }
+  c.iperm.assign(n, 0);
+  for (int r = 0; r < n; ++r) c.iperm[c.perm[r]] = r;
+  // cell-face table and interior-slot ELL
+  int W = 0, Ws = 0;
+  for (int o = 0; o < n; ++o) {
+    int a = t.cptr[o + 1] - t.cptr[o], b = 0;
+    for (int k = t.cptr[o]; k < t.cptr[o + 1]; ++k) b += t.fin[t.clst[k]] >= 0;
+    W = std::max(W, a);
+    Ws = std::max(Ws, b);
+  }
+  if (Ws == 0) Ws = 1;
+  c.W = W; c.Ws = Ws;
+  std::vector<int> cf((size_t)W * n, PAD), spc((size_t)Ws * n, -1), spf((size_t)Ws * n, -1);
+  for (int r = 0; r < n; ++r) {
+    const int o = c.perm[r];
+    int k = 0, ks = 0;
+    for (int s = t.cptr[o]; s < t.cptr[o + 1]; ++s, ++k) {
+      const int f = t.clst[s];
+      const bool own = t.fip[f] == o;
+      cf[(size_t)k * n + r] = own ? f : ~f;
+      if (t.fin[f] >= 0) {
+        const int j = own ? t.fin[f] : t.fip[f];
+        spc[(size_t)ks * n + r] = c.iperm[j];
+        spf[(size_t)ks * n + r] = f;
+        ++ks;
+      }
+    }
+  }
+  std::vector<int> fip(nf), fin(nf);
+  for (int f = 0; f < nf; ++f) {
+    fip[f] = c.iperm[t.fip[f]];
+    fin[f] = t.fin[f] >= 0 ? c.iperm[t.fin[f]] : -1;
+  }
+  c.h_fip = fip; c.h_fin = fin; c.h_fib = t.fib; c.h_fbty = t.fbty;
+  std::vector<double> cx(n), cy(n), cz(n), v(n), vr(n);
+  for (int r = 0; r < n; ++r) {
+    const int o = c.perm[r];
+    cx[r] = t.vx[3 * (size_t)o]; cy[r] = t.vx[3 * (size_t)o + 1]; cz[r] = t.vx[3 * (size_t)o + 2];
+    v[r] = t.vv[o]; vr[r] = t.vvr[o];
+  }
+  auto comp = [&](const std::vector<double> &a, int d) {
+    std::vector<double> o(nf);
+    for (int f = 0; f < nf; ++f) o[f] = a[3 * (size_t)f + d];
+    return o;
+  };
+  DMesh &M = c.M;
+  M.n = n; M.nf = nf; M.nbf = t.nbf; M.W = W; M.Ws = Ws; M.ndx = t.n_dx;
+  double *p;
+  int *ip;
+  int rc = 0;
+#define UP(field, vec) do { if ((rc = upload(c, &p, vec))) return rc; M.field = p; } while (0)
+#define UPI(field, vec) do { if ((rc = upload(c, &ip, vec))) return rc; M.field = ip; } while (0)
+  UP(cx, cx); UP(cy, cy); UP(cz, cz); UP(v, v); UP(vr, vr);
+  UPI(rank, c.perm); UPI(fip, fip); UPI(fin, fin); UPI(fib, t.fib); UPI(fbty, t.fbty);
+  UP(fs, t.fs); UP(flpn, t.flpn); UP(fwfp, t.fwfp);
+  UP(fx, comp(t.fx, 0)); UP(fy, comp(t.fx, 1)); UP(fz, comp(t.fx, 2));
+  UP(fnx, comp(t.fn, 0)); UP(fny, comp(t.fn, 1)); UP(fnz, comp(t.fn, 2));
+  UP(dpx, comp(t.fdp, 0)); UP(dpy, comp(t.fdp, 1)); UP(dpz, comp(t.fdp, 2));
+  UP(dnx, comp(t.fdn, 0)); UP(dny, comp(t.fdn, 1)); UP(dnz, comp(t.fdn, 2));
+  UPI(cf, cf); UPI(spc, spc); UPI(spf, spf);
+#undef UP
+#undef UPI
+  if ((rc = build_levels(c, t, c.iperm, true, c.lf))) return rc;
+  if ((rc = build_levels(c, t, c.iperm, false, c.lb))) return rc;
+  if ((rc = dalloc(c, &c.diag, n))) return rc;
+  if ((rc = dalloc(c, &c.spv, (size_t)Ws * n))) return rc;
+  if ((rc = dalloc(c, &c.ngp, nf))) return rc;
+  if ((rc = dalloc(c, &c.ngn, nf))) return rc;
+  if ((rc = dalloc(c, &c.pc_c, n))) return rc;
+  if ((rc = dalloc(c, &c.pc_cr, n))) return rc;
+  if ((rc = dalloc(c, &c.wk, (size_t)8 * 3 * n))) return rc;
+  if ((rc = dalloc(c, &c.ks, 3))) return rc;
+  if ((rc = dalloc(c, &c.red.part, (size_t)MAX_RED_BLOCKS * 8))) return rc;
+  if ((rc = dalloc(c, &c.red.ticket, 4))) return rc;
+  if ((rc = dalloc(c, &c.stage, (size_t)3 * std::max(n, nf)))) return rc;
+  FVK_HIP(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+static void ctx_free(Ctx &c) {
+  if (c.stream) hipStreamSynchronize(c.stream);
+  for (void *p : c.owned) hipFree(p);
+  c.owned.clear();
+  if (c.ev0) hipEventDestroy(c.ev0);
+  if (c.ev1) hipEventDestroy(c.ev1);
+  if (c.stream) hipStreamDestroy(c.stream);
+  c.stream = nullptr;
+}
+
+// reference entity-major host array (stride ld, ncomp components) <-> device
+// rows (component-major)
+static int to_dev(Ctx &c, const double *h, int ncomp, int ld, int count, const std::vector<int> *perm, double *d) {
+  std::vector<double> tmp((size_t)ncomp * count);
+  for (int r = 0; r < count; ++r) {
+    const int o = perm ? (*perm)[r] : r;
+    for (int q = 0; q < ncomp; ++q) tmp[(size_t)q * count + r] = h[(size_t)o * ld + q];
+  }
+  FVK_HIP(hipStreamSynchronize(c.stream));
+  FVK_HIP(hipMemcpy(d, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice));
+  return 0;
+}
+static int to_host(Ctx &c, const double *d, int ncomp, int ld, int count, const std::vector<int> *perm, double *h) {
+  std::vector<double> tmp((size_t)ncomp * count);
+  FVK_HIP(hipStreamSynchronize(c.stream));
+  FVK_HIP(hipMemcpy(tmp.data(), d, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int r = 0; r < count; ++r) {
+    const int o = perm ? (*perm)[r] : r;
+    for (int q = 0; q < ncomp; ++q) h[(size_t)o * ld + q] = tmp[(size_t)q * count + r];
+  }
+  return 0;
+}
+
+__global__ __launch_bounds__(BS) void k_spv_from_faces(DMesh M, const double *ngp, const double *ngn, double *spv) {
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i >= M.n) return;
+  int ks = 0;
+  for (int k = 0; k < M.W; ++k) {
+    const int e = M.cf[k * M.n + i];
+    if (e == PAD) break;
+    const int f = e >= 0 ? e : ~e;
+    if (M.fin[f] < 0) continue;
+    spv[ks * M.n + i] = e >= 0 ? ngp[f] : ngn[f];
+    ++ks;
+  }
+}
+
+}  // namespace fvk
+
+using namespace fvk;
+
+#define GUARD(x) \
+  do {           \
+    int rc_ = (x); \
+    if (rc_) return rc_; \
+  } while (0)
+
+extern "C" {
+
+const char *fvk_last_error(void) { return g_err.c_str(); }
+int fvk_version(void) { return 1; }
+
+// ---------------------------------------------------------------- mesh
+static int mesh_done(fvk_mesh *m, const std::string &e, fvk_mesh **out) {
+  if (!e.empty()) {
+    set_error(e);
+    delete m;
+    return FVK_ERR_ARG;
+  }
+  *out = m;
+  return FVK_OK;
+}
+
+int fvk_mesh_read_neu(const char *path, double l_sc, fvk_mesh **out) {
+  if (!path || !out) { set_error("null argument"); return FVK_ERR_ARG; }
+  fvk_mesh *m = new fvk_mesh;
+  std::string e = read_neu(m->m, path, l_sc);
+  if (!e.empty()) {
+    set_error(e);
+    delete m;
+    return e.rfind("cannot open", 0) == 0 ? FVK_ERR_IO : FVK_ERR_ARG;
+  }
+  return mesh_done(m, finish(m->m), out);
+}
+
+int fvk_mesh_box(int nx, int ny, int nz, double lx, double ly, double lz, double jitter, uint64_t seed, int lid_code,
+                 int wall_code, fvk_mesh **out) {
+  if (!out) { set_error("null argument"); return FVK_ERR_ARG; }
+  fvk_mesh *m = new fvk_mesh;
+  std::string e = make_box(m->m, nx, ny, nz, lx, ly, lz, jitter, seed, lid_code, wall_code);
+  if (e.empty()) e = finish(m->m);
+  return mesh_done(m, e, out);
+}
+
+int fvk_mesh_from_cells(int n_dx, int n_vrt, const double *vx3, int n_vol, const int *v_ty, const int *v_vptr,
+                        const int *v_vrt, const int *bnd, fvk_mesh **out) {
+  if (!vx3 || !v_ty || !v_vptr || !v_vrt || !bnd || !out || n_vol < 1 || (n_dx != 2 && n_dx != 3)) {
+    set_error("bad argument");
+    return FVK_ERR_ARG;
+  }
+  fvk_mesh *m = new fvk_mesh;
+  std::string e = init_cells(m->m, n_dx, n_vrt, vx3, n_vol, v_ty, v_vptr, v_vrt, bnd);
+  if (e.empty()) e = finish(m->m);
+  return mesh_done(m, e, out);
+}
+
+int fvk_mesh_get_info(const fvk_mesh *mm, fvk_mesh_info *info) {
+  if (!mm || !info) { set_error("null argument"); return FVK_ERR_ARG; }
+  const Mesh &m = mm->m;
+  info->n_dx = m.n_dx; info->n_vrt = m.n_vrt; info->n_vol = m.n_vol; info->n_fce = m.n_fce;
+  info->n_bf = m.n_bf; info->n_slot = m.n_slot; info->max_fce = m.max_fce;
+  return FVK_OK;
+}
+
+int fvk_mesh_get_topology(const fvk_mesh *mm, int *fce_ip, int *fce_in, int *fce_ib, int *fce_bty, int *cell_ptr,
+                          int *fce_lst, int *ng_lst) {
+  if (!mm) { set_error("null argument"); return FVK_ERR_ARG; }
+  const Mesh &m = mm->m;
+  for (int f = 0; f < m.n_fce; ++f) {
+    if (fce_ip) fce_ip[f] = m.f_ip[f] + 1;
+    if (fce_in) fce_in[f] = m.f_in[f] + 1;
+    if (fce_ib) fce_ib[f] = m.f_ib[f] >= 0 ? m.n_vol + 1 + m.f_ib[f] : 0;
+    if (fce_bty) fce_bty[f] = m.f_bty[f];
+  }
+  if (cell_ptr) for (int c = 0; c <= m.n_vol; ++c) cell_ptr[c] = m.v_fptr[c];
+  for (int s = 0; s < m.n_slot; ++s) {
+    if (fce_lst) fce_lst[s] = m.fce_lst[s] + 1;
+    if (ng_lst) ng_lst[s] = m.ng_lst[s] + 1;
+  }
+  return FVK_OK;
+}
+
+int fvk_mesh_get_geometry(const fvk_mesh *mm, double *vol_x, double *vol_v, double *fce_x, double *fce_n,
+                          double *fce_dxp, double *fce_dxn, double *fce_s, double *fce_lpn, double *fce_wfp) {
+  if (!mm) { set_error("null argument"); return FVK_ERR_ARG; }
+  const Mesh &m = mm->m;
+  const int nd = m.n_dx;
+  for (int c = 0; c < m.n_vol; ++c) {
+    for (int d = 0; d < nd; ++d)
+      if (vol_x) vol_x[(size_t)c * nd + d] = m.vol_x[3 * (size_t)c + d];
+    if (vol_v) vol_v[c] = m.vol_v[c];
+  }
+  for (int f = 0; f < m.n_fce; ++f) {
+    for (int d = 0; d < nd; ++d) {
+      const size_t o = (size_t)f * nd + d, i = 3 * (size_t)f + d;
+      if (fce_x) fce_x[o] = m.f_x[i];
+      if (fce_n) fce_n[o] = m.f_n[i];
+      if (fce_dxp) fce_dxp[o] = m.f_dxp[i];
+      if (fce_dxn) fce_dxn[o] = m.f_dxn[i];
+    }
+    if (fce_s) fce_s[f] = m.f_s[f];
+    if (fce_lpn) fce_lpn[f] = m.f_lpn[f];
+    if (fce_wfp) fce_wfp[f] = m.f_wfp[f];
+  }
+  return FVK_OK;
+}
+
+void fvk_mesh_free(fvk_mesh *m) { delete m; }
+
+// ---------------------------------------------------------------- context
+int fvk_ctx_create(const fvk_mesh *m, int device, int flags, fvk_ctx **out) {
+  if (!m || !out) { set_error("null argument"); return FVK_ERR_ARG; }
+  fvk_ctx *c = new fvk_ctx;
+  int rc = ctx_build(c->c, topo_from_mesh(m->m), device, flags);
+  if (rc) {
+    ctx_free(c->c);
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return FVK_OK;
+}
+
+int fvk_ctx_create_arrays(int n_dx, int n_vol, int n_fce, int n_bf, const int *fce_ip, const int *fce_in,
+                          const int *fce_ib, const int *fce_bty, const int *cell_ptr, const int *fce_lst,
+                          const double *vol_x, const double *vol_v, const double *fce_x, const double *fce_n,
+                          const double *fce_dxp, const double *fce_dxn, const double *fce_s, const double *fce_lpn,
+                          const double *fce_wfp, int device, int flags, fvk_ctx **out) {
+  if (!out || !fce_ip || !fce_in || !fce_ib || !fce_bty || !cell_ptr || !fce_lst || !vol_x || !vol_v || !fce_x ||
+      !fce_n || !fce_dxp || !fce_dxn || !fce_s || !fce_lpn || !fce_wfp || (n_dx != 2 && n_dx != 3) || n_vol < 1) {
+    set_error("bad argument");
+    return FVK_ERR_ARG;
+  }
+  Topo t;
+  t.n_dx = n_dx; t.n = n_vol; t.nf = n_fce; t.nbf = n_bf;
+  t.fip.resize(n_fce); t.fin.resize(n_fce); t.fib.resize(n_fce); t.fbty.assign(fce_bty, fce_bty + n_fce);
+  for (int f = 0; f < n_fce; ++f) {
+    t.fip[f] = fce_ip[f] - 1;
+    t.fin[f] = fce_in[f] - 1;
+    t.fib[f] = fce_ib[f] > 0 ? fce_ib[f] - n_vol - 1 : -1;
+    if (t.fip[f] < 0 || t.fip[f] >= n_vol || t.fin[f] >= n_vol || (t.fin[f] < 0 && (t.fib[f] < 0 || t.fib[f] >= n_bf))) {
+      set_error("face index out of range");
+      return FVK_ERR_ARG;
+    }
+  }
+  t.cptr.assign(cell_ptr, cell_ptr + n_vol + 1);
+  t.clst.resize(cell_ptr[n_vol]);
+  for (int s = 0; s < cell_ptr[n_vol]; ++s) {
+    t.clst[s] = fce_lst[s] - 1;
+    if (t.clst[s] < 0 || t.clst[s] >= n_fce) { set_error("fce_lst entry out of range"); return FVK_ERR_ARG; }
+  }
+  auto v3 = [&](const double *a, int cnt) {
+    std::vector<double> o(3 * (size_t)cnt, 0.0);
+    for (int k = 0; k < cnt; ++k)
+      for (int d = 0; d < n_dx; ++d) o[3 * (size_t)k + d] = a[(size_t)k * n_dx + d];
+    return o;
+  };
+  t.vx = v3(vol_x, n_vol);
+  t.vv.assign(vol_v, vol_v + n_vol);
+  t.vvr.resize(n_vol);
+  for (int c = 0; c < n_vol; ++c) t.vvr[c] = 1.0 / t.vv[c];  // grd_geom.f90:216
+  t.fx = v3(fce_x, n_fce); t.fn = v3(fce_n, n_fce); t.fdp = v3(fce_dxp, n_fce); t.fdn = v3(fce_dxn, n_fce);
+  t.fs.assign(fce_s, fce_s + n_fce); t.flpn.assign(fce_lpn, fce_lpn + n_fce); t.fwfp.assign(fce_wfp, fce_wfp + n_fce);
+  fvk_ctx *c = new fvk_ctx;
+  int rc = ctx_build(c->c, t, device, flags);
+  if (rc) {
+    ctx_free(c->c);
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return FVK_OK;
+}
+
+void fvk_ctx_destroy(fvk_ctx *c) {
+  if (!c) return;
+  ctx_free(c->c);
+  delete c;
+}
+
+int fvk_ctx_sync(fvk_ctx *c) {
+  if (!c) return FVK_ERR_ARG;
+  FVK_HIP(hipStreamSynchronize(c->c.stream));
+  return FVK_OK;
+}
+
+int fvk_ctx_get_perm(const fvk_ctx *c, int *perm) {
+  if (!c || !perm) return FVK_ERR_ARG;
+  for (int r = 0; r < c->c.n; ++r) perm[r] = c->c.perm[r] + 1;
+  return FVK_OK;
+}
+
+int fvk_ctx_levels(const fvk_ctx *c, int *n_fwd, int *n_bwd) {
+  if (!c) return FVK_ERR_ARG;
+  if (n_fwd) *n_fwd = c->c.lf.count();
+  if (n_bwd) *n_bwd = c->c.lb.count();
+  return FVK_OK;
+}
+
+// ---------------------------------------------------------------- solvers
+int fvk_set_matrix(fvk_ctx *cc, const double *pl, const double *ng_p, const double *ng_n) {
+  if (!cc || !pl || !ng_p || !ng_n) { set_error("null argument"); return FVK_ERR_ARG; }
+  Ctx &c = cc->c;
+  GUARD(to_dev(c, pl, 1, 1, c.n, &c.perm, c.diag));
+  FVK_HIP(hipStreamSynchronize(c.stream));
+  FVK_HIP(hipMemcpy(c.ngp, ng_p, sizeof(double) * c.nf, hipMemcpyHostToDevice));
+  FVK_HIP(hipMemcpy(c.ngn, ng_n, sizeof(double) * c.nf, hipMemcpyHostToDevice));
+  k_spv_from_faces<<<grid_for(c.n), BS, 0, c.stream>>>(c.M, c.ngp, c.ngn, c.spv);
+  FVK_HIP(hipGetLastError());
+  c.pc_valid = false;
+  return FVK_OK;
+}
+
+int fvk_solv_precond(fvk_ctx *cc, int imat, double *cv, double *c_r) {
+  if (!cc || imat < 0 || imat > 2) { set_error("bad argument"); return FVK_ERR_ARG; }
+  Ctx &c = cc->c;
+  GUARD(precond_build(c, imat));
+  if (cv) GUARD(to_host(c, c.pc_c, 1, 1, c.n, &c.perm, cv));
+  if (c_r) GUARD(to_host(c, c.pc_cr, 1, 1, c.n, &c.perm, c_r));
+  return FVK_OK;
+}
+
+int fvk_solv_direct(fvk_ctx *cc, int imat, const double *cv, const double *c_r, const double *b, double *x) {
+  if (!cc || !b || !x || imat < 0 || imat > 2 || (!cv) != (!c_r)) { set_error("bad argument"); return FVK_ERR_ARG; }
+  Ctx &c = cc->c;
+  if (cv) {
+    GUARD(to_dev(c, cv, 1, 1, c.n, &c.perm, c.pc_c));
+    GUARD(to_dev(c, c_r, 1, 1, c.n, &c.perm, c.pc_cr));
+    c.pc_imat = imat;
+    c.pc_valid = true;
+  } else if (!c.pc_valid || c.pc_imat != imat) {
+    GUARD(precond_build(c, imat));
+  }
+  double *db = c.wk + (size_t)5 * 3 * c.n, *dx = c.wk + (size_t)6 * 3 * c.n;
+  GUARD(to_dev(c, b, 1, 1, c.n, &c.perm, db));
+  GUARD(precond_apply(c, imat, 1, db, dx, nullptr));
+  GUARD(to_host(c, dx, 1, 1, c.n, &c.perm, x));
+  return FVK_OK;
+}
+
+static int col_to_dev(Ctx &c, const double *res0, int nvc, double *d) {
+  // res0(n_vol, n_vc) column-major -> component-major rows
+  std::vector<double> tmp((size_t)nvc * c.n);
+  for (int q = 0; q < nvc; ++q)
+    for (int r = 0; r < c.n; ++r) tmp[(size_t)q * c.n + r] = res0[(size_t)q * c.n + c.perm[r]];
+  FVK_HIP(hipStreamSynchronize(c.stream));
+  FVK_HIP(hipMemcpy(d, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice));
+  return 0;
+}
+static int col_to_host(Ctx &c, const double *d, int nvc, double *res0) {
+  std::vector<double> tmp((size_t)nvc * c.n);
+  FVK_HIP(hipStreamSynchronize(c.stream));
+  FVK_HIP(hipMemcpy(tmp.data(), d, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int q = 0; q < nvc; ++q)
+    for (int r = 0; r < c.n; ++r) res0[(size_t)q * c.n + c.perm[r]] = tmp[(size_t)q * c.n + r];
+  return 0;
+}
+
+int fvk_solv_res(fvk_ctx *cc, int n_vc, const double *phi, double *res0, double *res0_n, int not_r0) {
+  if (!cc || !res0 || !res0_n || n_vc < 1 || n_vc > 3 || (!not_r0 && !phi)) { set_error("bad argument"); return FVK_ERR_ARG; }
+  Ctx &c = cc->c;
+  double *dx = c.wk + (size_t)5 * 3 * c.n, *dr = c.wk + (size_t)6 * 3 * c.n;
+  if (!not_r0) GUARD(to_dev(c, phi, n_vc, n_vc, c.n, &c.perm, dx));
+  GUARD(col_to_dev(c, res0, n_vc, dr));
+  GUARD(residual(c, n_vc, dx, dr, not_r0 != 0));
+  GUARD(col_to_host(c, dr, n_vc, res0));
+  KS h[3];
+  FVK_HIP(hipMemcpy(h, c.ks, sizeof h, hipMemcpyDeviceToHost));
+  for (int q = 0; q < n_vc; ++q) res0_n[q] = h[q].res0_n;
+  return FVK_OK;
+}
+
+static int solve_common(fvk_ctx *cc, int kind, int imat, double reduc, int n_iter, int n_vc, double *phi,
+                        double *res0, double *res0_n, int not_r0, fvk_stats *st) {
+  if (!cc || !phi || !res0 || !res0_n || n_vc < 1 || n_vc > 3 || imat < 0 || imat > 2 || n_iter < 0) {
+    set_error("bad argument");
+    return FVK_ERR_ARG;
+  }
+  Ctx &c = cc->c;
+  double *dx = c.wk + (size_t)5 * 3 * c.n, *dr = c.wk + (size_t)6 * 3 * c.n;
+  GUARD(to_dev(c, phi, n_vc, n_vc, c.n, &c.perm, dx));
+  GUARD(col_to_dev(c, res0, n_vc, dr));
+  FVK_HIP(hipEventRecord(c.ev0, c.stream));
+  GUARD(krylov(c, kind, imat, reduc, n_iter, n_vc, dx, dr, not_r0 != 0));
+  FVK_HIP(hipEventRecord(c.ev1, c.stream));
+  GUARD(to_host(c, dx, n_vc, n_vc, c.n, &c.perm, phi));
+  GUARD(col_to_host(c, dr, n_vc, res0));
+  KS h[3];
+  FVK_HIP(hipMemcpy(h, c.ks, sizeof h, hipMemcpyDeviceToHost));
+  for (int q = 0; q < n_vc; ++q) res0_n[q] = h[q].res0_n;
+  if (st) {
+    std::memset(st, 0, sizeof *st);
+    for (int q = 0; q < n_vc; ++q) { st->iters[q] = h[q].iter; st->res_rel[q] = h[q].res_n; }
+    float ms = 0;
+    hipEventElapsedTime(&ms, c.ev0, c.ev1);
+    st->seconds = ms * 1e-3;
+  }
+  return FVK_OK;
+}
+
+int fvk_solv_cg(fvk_ctx *c, int imat, double reduc, int n_iter, int n_vc, double *phi, double *res0, double *res0_n,
+                int not_r0, fvk_stats *st) {
+  return solve_common(c, 0, imat, reduc, n_iter, n_vc, phi, res0, res0_n, not_r0, st);
+}
+int fvk_solv_bicg(fvk_ctx *c, int imat, double reduc, int n_iter, int n_vc, double *phi, double *res0,
+                  double *res0_n, int not_r0, fvk_stats *st) {
+  return solve_common(c, 1, imat, reduc, n_iter, n_vc, phi, res0, res0_n, not_r0, st);
+}
+
+int fvk_solv_grad(fvk_ctx *cc, int n_vc, const double *phi, double *gout, int n_iter, int neu) {
+  if (!cc || !phi || !gout || n_vc < 1 || n_vc > 3 || n_iter < 1) { set_error("bad argument"); return FVK_ERR_ARG; }
+  Ctx &c = cc->c;
+  const int n = c.n, nb = c.nbf, nd = c.ndx;
+  double *dphi = c.wk, *dphib = c.wk + (size_t)3 * n, *g = c.wk + (size_t)2 * 3 * n, *g0 = c.wk + (size_t)5 * 3 * n;
+  if ((size_t)3 * nb > (size_t)3 * n) { set_error("more boundary faces than cells"); return FVK_ERR_ARG; }
+  GUARD(to_dev(c, phi, n_vc, n_vc, n, &c.perm, dphi));
+  if (nb) GUARD(to_dev(c, phi + (size_t)n * n_vc, n_vc, n_vc, nb, nullptr, dphib));
+  GUARD(grad(c, n_vc, dphi, dphib, g, n_iter, neu, nullptr, g0));
+  std::vector<double> tmp((size_t)9 * n);
+  FVK_HIP(hipStreamSynchronize(c.stream));
+  FVK_HIP(hipMemcpy(tmp.data(), g, sizeof(double) * 3 * n_vc * n, hipMemcpyDeviceToHost));
+  for (int r = 0; r < n; ++r) {
+    const int o = c.perm[r];
+    for (int q = 0; q < n_vc; ++q)
+      for (int d = 0; d < nd; ++d) gout[((size_t)o * n_vc + q) * nd + d] = tmp[(size_t)(q * 3 + d) * n + r];
+  }
+  return FVK_OK;
+}
+
+// ---------------------------------------------------------------- systems
+static int sys_alloc(Ctx &c) {
+  if (c.sys_b) return 0;
+  GUARD(dalloc(c, &c.sys_b, c.n));
+  GUARD(dalloc(c, &c.sys_x, c.n));
+  GUARD(dalloc(c, &c.sys_r, (size_t)c.n + 8));
+  return 0;
+}
+
+static uint64_t sm64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ULL;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+  return x ^ (x >> 31);
+}
+// standard normal from two counter-based uniforms (Box-Muller)
+static double normal_at(uint64_t seed, uint64_t k) {
+  const double u1 = ((double)(sm64(seed ^ (2 * k)) >> 11) + 0.5) * 0x1.0p-53;
+  const double u2 = (double)(sm64(seed ^ (2 * k + 1)) >> 11) * 0x1.0p-53;
+  return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+}
+
+int fvk_sys_set(fvk_ctx *cc, const double *pl, const double *ng, const double *b) {
+  if (!cc || !pl || !ng || !b) { set_error("null argument"); return FVK_ERR_ARG; }
+  Ctx &c = cc->c;
+  GUARD(sys_alloc(c));
+  GUARD(fvk_set_matrix(cc, pl, ng, ng));
+  GUARD(to_dev(c, b, 1, 1, c.n, &c.perm, c.sys_b));
+  return FVK_OK;
+}
+
+int fvk_sys_random(fvk_ctx *cc, uint64_t seed, double sigma, double *b_out) {
+  if (!cc) { set_error("null argument"); return FVK_ERR_ARG; }
+  Ctx &c = cc->c;
+  // geometry factor s/l_pn per face, on the host
+  std::vector<double> s(c.nf), l(c.nf);
+  FVK_HIP(hipMemcpy(s.data(), c.M.fs, sizeof(double) * c.nf, hipMemcpyDeviceToHost));
+  FVK_HIP(hipMemcpy(l.data(), c.M.flpn, sizeof(double) * c.nf, hipMemcpyDeviceToHost));
+  std::vector<double> ng(c.nf, 0.0), pl(c.n, 0.0), b(c.n);
+  std::vector<int> orig_ip(c.nf), orig_in(c.nf);
+  for (int f = 0; f < c.nf; ++f) {
+    if (c.h_fin[f] < 0) continue;
+    const double a = s[f] / l[f] * std::exp(sigma * normal_at(seed, (uint64_t)f));
+    ng[f] = -a;
+    pl[c.perm[c.h_fip[f]]] += a;
+    pl[c.perm[c.h_fin[f]]] += a;
+  }
+  double mean = 0.0;
+  for (int i = 0; i < c.n; ++i) { b[i] = normal_at(seed ^ 0x5DEECE66DULL, (uint64_t)i + 0x100000000ULL); mean += b[i]; }
+  mean /= c.n;
+  for (int i = 0; i < c.n; ++i) b[i] -= mean;
+  if (b_out) std::memcpy(b_out, b.data(), sizeof(double) * c.n);
+  return fvk_sys_set(cc, pl.data(), ng.data(), b.data());
+}
+
+int fvk_sys_get_x(fvk_ctx *cc, double *x) {
+  if (!cc || !x || !cc->c.sys_x) { set_error("no system"); return FVK_ERR_STATE; }
+  return to_host(cc->c, cc->c.sys_x, 1, 1, cc->c.n, &cc->c.perm, x);
+}
+
+int fvk_sys_get(fvk_ctx *cc, double *pl, double *a, double *b) {
+  if (!cc || !cc->c.sys_b) { set_error("no system"); return FVK_ERR_STATE; }
+  Ctx &c = cc->c;
+  if (pl) GUARD(to_host(c, c.diag, 1, 1, c.n, &c.perm, pl));
+  if (a) {
+    FVK_HIP(hipStreamSynchronize(c.stream));
+    FVK_HIP(hipMemcpy(a, c.ngp, sizeof(double) * c.nf, hipMemcpyDeviceToHost));
+    for (int f = 0; f < c.nf; ++f) a[f] = -a[f];
+  }
+  if (b) GUARD(to_host(c, c.sys_b, 1, 1, c.n, &c.perm, b));
+  return FVK_OK;
+}
+
+int fvk_sys_solve(fvk_ctx *cc, int method, int imat, double tol, int max_iter, int time_kernel,
+                  fvk_solve_info *info) {
+  if (!cc || !info || imat < 0 || imat > 2 || max_iter < 1 || method != FVK_SOLVE_PCG64) {
+    set_error("bad argument");
+    return FVK_ERR_ARG;
+  }
+  Ctx &c = cc->c;
+  if (!c.sys_b) { set_error("no system loaded"); return FVK_ERR_STATE; }
+  std::memset(info, 0, sizeof *info);
+  const int n = c.n;
+  double *r0 = c.wk + (size_t)6 * 3 * n;
+  FVK_HIP(hipMemsetAsync(c.sys_x, 0, sizeof(double) * n, c.stream));
+  FVK_HIP(hipMemcpyAsync(r0, c.sys_b, sizeof(double) * n, hipMemcpyDeviceToDevice, c.stream));
+  KTimer T;
+  if (time_kernel) T.init(64);
+  FVK_HIP(hipEventRecord(c.ev0, c.stream));
+  GUARD(krylov(c, 0, imat, tol, max_iter, 1, c.sys_x, r0, true, 64, time_kernel ? &T : nullptr));
+  FVK_HIP(hipEventRecord(c.ev1, c.stream));
+  FVK_HIP(hipEventSynchronize(c.ev1));
+  float ms = 0.f;
+  FVK_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+  KS h;
+  FVK_HIP(hipMemcpy(&h, c.ks, sizeof h, hipMemcpyDeviceToHost));
+  double tr[4];
+  GUARD(true_residual(c, c.sys_x, c.sys_b, c.sys_r, tr));
+  info->iters = h.iter;
+  info->seconds = ms * 1e-3;
+  info->res_rec = h.res_n;
+  info->res_true_l1 = tr[0] / tr[2];
+  info->res_true_l2 = std::sqrt(tr[1] / tr[3]);
+  if (time_kernel && T.launches) {
+    long nnz_off = 0;
+    for (int f = 0; f < c.nf; ++f) nnz_off += c.h_fin[f] >= 0 ? 2 : 0;
+    info->kern_seconds = T.total_ms * 1e-3 / T.launches;
+    // k_cg_pq: z, p_old, diag read; p_new, q written; column + value per off-diagonal
+    info->kern_bytes = 40.0 * n + 12.0 * (double)nnz_off;
+    info->kern_launches = T.launches;
+  }
+  return h.done && h.res_n < tol ? FVK_OK : FVK_NOT_CONVERGED;
+}
+
+int fvk_bench_spmv(fvk_ctx *cc, int reps, double *sec) {
+  if (!cc || !sec || reps < 1) { set_error("bad argument"); return FVK_ERR_ARG; }
+  Ctx &c = cc->c;
+  double *x = c.wk, *y = c.wk + (size_t)3 * c.n;
+  for (int k = 0; k < 3; ++k) GUARD(spmv(c, 1, x, y));
+  FVK_HIP(hipEventRecord(c.ev0, c.stream));
+  for (int k = 0; k < reps; ++k) GUARD(spmv(c, 1, x, y));
+  FVK_HIP(hipEventRecord(c.ev1, c.stream));
+  FVK_HIP(hipEventSynchronize(c.ev1));
+  float ms = 0.f;
+  FVK_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+  *sec = ms * 1e-3 / reps;
+  return FVK_OK;
+}
+
+__global__ __launch_bounds__(BS) void k_copy(long n4, const double4 *__restrict__ a, double4 *__restrict__ b) {
+  for (long i = blockIdx.x * (long)BS + threadIdx.x; i < n4; i += (long)gridDim.x * BS) b[i] = a[i];
+}
+
+int fvk_bench_copy(int device, long nbytes, int reps, double *sec) {
+  if (!sec || reps < 1 || nbytes < 32) { set_error("bad argument"); return FVK_ERR_ARG; }
+  FVK_HIP(hipSetDevice(device));
+  const long n4 = nbytes / 32;
+  void *a = nullptr, *b = nullptr;
+  FVK_HIP(hipMalloc(&a, n4 * 32));
+  if (hipMalloc(&b, n4 * 32) != hipSuccess) { hipFree(a); set_error("hipMalloc failed"); return FVK_ERR_HIP; }
+  hipMemset(a, 0, n4 * 32);
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  const int G = 256 * 8 * 4;
+  for (int k = 0; k < 3; ++k) k_copy<<<G, BS>>>(n4, (const double4 *)a, (double4 *)b);
+  hipEventRecord(e0, 0);
+  for (int k = 0; k < reps; ++k) k_copy<<<G, BS>>>(n4, (const double4 *)a, (double4 *)b);
+  hipEventRecord(e1, 0);
+  hipEventSynchronize(e1);
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipError_t err = hipGetLastError();
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(a);
+  hipFree(b);
+  if (err != hipSuccess) { set_error(hipGetErrorString(err)); return FVK_ERR_HIP; }
+  *sec = ms * 1e-3 / reps;
+  return FVK_OK;
+}
+
+// ---------------------------------------------------------------- flow
+int fvk_flow_create(fvk_ctx *cc, const fvk_flow_cfg *cfg, fvk_flow **out) {
+  if (!cc || !cfg || !out) { set_error("null argument"); return FVK_ERR_ARG; }
+  if (cfg->imat_u < 0 || cfg->imat_u > 2 || cfg->imat_p < 0 || cfg->imat_p > 2 || cfg->n_grad < 1 ||
+      cfg->urf_u <= 0.0 || cfg->n_bicg < 0 || cfg->n_cg < 0) {
+    set_error("bad flow configuration");
+    return FVK_ERR_ARG;
+  }
+  Ctx &c = cc->c;
+  fvk_flow *ff = new fvk_flow;
+  Flow &F = ff->f;
+  F.c = &c;
+  F.cfg = *cfg;
+  F.nvc = c.ndx;
+  const size_t n = c.n, nf = c.nf, nb = std::max(c.nbf, 1);
+  int rc = 0;
+  auto A = [&](double **p, size_t cnt) { if (!rc) rc = dalloc(c, p, cnt); };
+  A(&F.u, 3 * n); A(&F.ub, 3 * nb); A(&F.p, n); A(&F.dpdx, 3 * n); A(&F.spd, nf); A(&F.frc_dns, nf);
+  A(&F.cs, n); A(&F.frc_vol, n); A(&F.ddx, 9 * n); A(&F.g0, 9 * n); A(&F.rhs, 3 * n); A(&F.rhs_p, n);
+  A(&F.cor, 3 * nf); A(&F.cfl, nf); A(&F.cpp, nf); A(&F.cinout, nf); A(&F.pl_r, n); A(&F.pc, n);
+  A(&F.ddxp, 3 * n); A(&F.red_scratch, 8);
+  if (!rc) rc = dalloc(c, &F.sc, 1);
+  // inlet / outlet face lists and outlet cells
+  std::vector<int> fin_l, fout_l;
+  for (int f = 0; f < c.nf; ++f) {
+    if (c.h_fin[f] >= 0) continue;
+    int cls = c.h_fbty[f] - c.h_fbty[f] % 10;
+    if (cls == 10) fin_l.push_back(f);
+    if (cls == 20) fout_l.push_back(f);
+  }
+  F.n_in = (int)fin_l.size();
+  F.n_out = (int)fout_l.size();
+  std::vector<int> io(fin_l);
+  io.insert(io.end(), fout_l.begin(), fout_l.end());
+  if (!rc && !io.empty()) rc = upload(c, &F.io_faces, io);
+  if (!rc && !fout_l.empty()) {
+    std::vector<int> cell_oc(c.n, -1), oc_cell, oc_face;
+    for (int f : fout_l) {  // ascending face order
+      int r = c.h_fip[f];
+      if (cell_oc[r] < 0) {
+        cell_oc[r] = (int)oc_cell.size();
+        oc_cell.push_back(r);
+        oc_face.insert(oc_face.end(), {-1, -1, -1, -1});
+      }
+      int m = cell_oc[r], k = 0;
+      while (k < 4 && oc_face[4 * m + k] >= 0) ++k;
+      if (k == 4) { rc = FVK_ERR_ARG; set_error("cell with more than 4 outlet faces"); break; }
+      oc_face[4 * m + k] = f;
+    }
+    F.n_oc = (int)oc_cell.size();
+    if (!rc) rc = upload(c, &F.cell_oc, cell_oc);
+    if (!rc) rc = upload(c, &F.oc_cell, oc_cell);
+    if (!rc) rc = upload(c, &F.oc_face, oc_face);
+    if (!rc) rc = dalloc(c, &F.oc_u, (size_t)F.n_oc * 12);
+  }
+  F.row_of_cell1 = c.iperm[0];
+  if (!rc) {
+    hipError_t e = hipStreamSynchronize(c.stream);
+    if (e != hipSuccess) { set_error(hipGetErrorString(e)); rc = FVK_ERR_HIP; }
+  }
+  if (rc) {
+    delete ff;
+    return rc;
+  }
+  *out = ff;
+  return FVK_OK;
+}
+
+void fvk_flow_destroy(fvk_flow *f) { delete f; }  // device memory is owned by the context
+
+int fvk_flow_configure(fvk_flow *ff, const fvk_flow_cfg *cfg) {
+  if (!ff || !cfg) { set_error("null argument"); return FVK_ERR_ARG; }
+  if (cfg->imat_u < 0 || cfg->imat_u > 2 || cfg->imat_p < 0 || cfg->imat_p > 2 || cfg->n_grad < 1 ||
+      cfg->urf_u <= 0.0 || cfg->n_bicg < 0 || cfg->n_cg < 0) {
+    set_error("bad flow configuration");
+    return FVK_ERR_ARG;
+  }
+  ff->f.cfg = *cfg;
+  return FVK_OK;
+}
+
+int fvk_flow_scalars(fvk_flow *ff, int set, double *c_in, double *c_out) {
+  if (!ff || !c_in || !c_out) { set_error("null argument"); return FVK_ERR_ARG; }
+  Flow &F = ff->f;
+  FVK_HIP(hipStreamSynchronize(F.c->stream));
+  FlowSc s;
+  FVK_HIP(hipMemcpy(&s, F.sc, sizeof s, hipMemcpyDeviceToHost));
+  if (set) {
+    s.c_in = *c_in;
+    s.c_out = *c_out;
+    FVK_HIP(hipMemcpy(F.sc, &s, sizeof s, hipMemcpyHostToDevice));
+  } else {
+    *c_in = s.c_in;
+    *c_out = s.c_out;
+  }
+  return FVK_OK;
+}
+
+int fvk_flow_set_bnd_u(fvk_flow *ff, int code, const double *u) {
+  if (!ff || !u) { set_error("null argument"); return FVK_ERR_ARG; }
+  Flow &F = ff->f;
+  Ctx &c = *F.c;
+  const int nb = c.nbf, nvc = F.nvc;
+  std::vector<double> tmp((size_t)3 * std::max(nb, 1));
+  FVK_HIP(hipStreamSynchronize(c.stream));
+  FVK_HIP(hipMemcpy(tmp.data(), F.ub, sizeof(double) * 3 * std::max(nb, 1), hipMemcpyDeviceToHost));
+  for (int f = 0; f < c.nf; ++f)
+    if (c.h_fin[f] < 0 && c.h_fbty[f] == code)
+      for (int q = 0; q < nvc; ++q) tmp[(size_t)q * nb + c.h_fib[f]] = u[q];
+  FVK_HIP(hipMemcpy(F.ub, tmp.data(), sizeof(double) * 3 * std::max(nb, 1), hipMemcpyHostToDevice));
+  return FVK_OK;
+}
+
+int fvk_flow_set_state(fvk_flow *ff, const double *u_cell, const double *u_bnd, const double *p, const double *dpdx,
+                       const double *spd) {
+  if (!ff) return FVK_ERR_ARG;
+  Flow &F = ff->f;
+  Ctx &c = *F.c;
+  FVK_HIP(hipStreamSynchronize(c.stream));
+  if (u_cell) GUARD(to_dev(c, u_cell, F.nvc, F.nvc, c.n, &c.perm, F.u));
+  if (u_bnd && c.nbf) GUARD(to_dev(c, u_bnd, F.nvc, F.nvc, c.nbf, nullptr, F.ub));
+  if (p) GUARD(to_dev(c, p, 1, 1, c.n, &c.perm, F.p));
+  if (dpdx) GUARD(to_dev(c, dpdx, c.ndx, c.ndx, c.n, &c.perm, F.dpdx));
+  if (spd) FVK_HIP(hipMemcpy(F.spd, spd, sizeof(double) * c.nf, hipMemcpyHostToDevice));
+  return FVK_OK;
+}
+
+int fvk_flow_get_state(fvk_flow *ff, double *u_cell, double *u_bnd, double *p, double *dpdx, double *spd, double *cs,
+                       double *frc_dns) {
+  if (!ff) return FVK_ERR_ARG;
+  Flow &F = ff->f;
+  Ctx &c = *F.c;
+  if (u_cell) GUARD(to_host(c, F.u, F.nvc, F.nvc, c.n, &c.perm, u_cell));
+  if (u_bnd && c.nbf) GUARD(to_host(c, F.ub, F.nvc, F.nvc, c.nbf, nullptr, u_bnd));
+  if (p) GUARD(to_host(c, F.p, 1, 1, c.n, &c.perm, p));
+  if (dpdx) GUARD(to_host(c, F.dpdx, c.ndx, c.ndx, c.n, &c.perm, dpdx));
+  if (cs) GUARD(to_host(c, F.cs, 1, 1, c.n, &c.perm, cs));
+  FVK_HIP(hipStreamSynchronize(c.stream));
+  if (spd) FVK_HIP(hipMemcpy(spd, F.spd, sizeof(double) * c.nf, hipMemcpyDeviceToHost));
+  if (frc_dns) FVK_HIP(hipMemcpy(frc_dns, F.frc_dns, sizeof(double) * c.nf, hipMemcpyDeviceToHost));
+  return FVK_OK;
+}
+
+int fvk_flow_get_matrix(fvk_flow *ff, double *pl, double *ng_p, double *ng_n) {
+  if (!ff) return FVK_ERR_ARG;
+  Flow &F = ff->f;
+  Ctx &c = *F.c;
+  if (pl) GUARD(to_host(c, c.diag, 1, 1, c.n, &c.perm, pl));
+  FVK_HIP(hipStreamSynchronize(c.stream));
+  if (ng_p) FVK_HIP(hipMemcpy(ng_p, c.ngp, sizeof(double) * c.nf, hipMemcpyDeviceToHost));
+  if (ng_n) FVK_HIP(hipMemcpy(ng_n, c.ngn, sizeof(double) * c.nf, hipMemcpyDeviceToHost));
+  return FVK_OK;
+}
+
+static int read_hist(Flow &F, fvk_outer_hist *h) {
+  FlowSc s;
+  FVK_HIP(hipStreamSynchronize(F.c->stream));
+  FVK_HIP(hipMemcpy(&s, F.sc, sizeof s, hipMemcpyDeviceToHost));
+  if (h) {
+    h->reduc = s.h_reduc; h->redpp = s.h_redpp; h->redpp_sum = s.h_redpp_sum;
+    h->red_p = s.h_red_p; h->red_p_sum = s.h_red_p_sum; h->mx_res_n = s.mx_res_n;
+    for (int q = 0; q < 3; ++q) h->it_u[q] = s.it_u[q];
+    h->it_pp = s.it_pp; h->it_p = s.it_p;
+    for (int q = 0; q < 3; ++q) h->res0_n_u[q] = s.res0_n_u[q];
+    h->res0_n_pp = s.res0_n_pp;
+    h->res0_n_p = s.res0_n_p;
+  }
+  if (s.diverged) { set_error("momentum residual exceeded large"); return FVK_ERR_DIVERGED; }
+  return FVK_OK;
+}
+
+int fvk_flow_setup_eqn(fvk_flow *ff, int it, fvk_outer_hist *h) {
+  if (!ff || it < 1) return FVK_ERR_ARG;
+  bool skipped;
+  GUARD(mom_setup(ff->f, it, skipped));
+  return read_hist(ff->f, h);
+}
+
+int fvk_flow_setup_pp(fvk_flow *ff, int it, fvk_outer_hist *h) {
+  if (!ff || it < 1) return FVK_ERR_ARG;
+  GUARD(pp_setup(ff->f, it));
+  return read_hist(ff->f, h);
+}
+
+int fvk_flow_outer(fvk_flow *ff, int it, fvk_outer_hist *h) {
+  if (!ff || it < 1) return FVK_ERR_ARG;
+  Flow &F = ff->f;
+  GUARD(outer_begin(F));
+  bool skipped;
+  GUARD(mom_setup(F, it, skipped));
+  GUARD(pp_setup(F, it));
+  return read_hist(F, h);
+}
+
+int fvk_flow_run(fvk_flow *ff, int it0, int n, fvk_outer_hist *h) {
+  if (!ff || it0 < 1 || n < 0) return FVK_ERR_ARG;
+  for (int k = 0; k < n; ++k) GUARD(fvk_flow_outer(ff, it0 + k, h ? h + k : nullptr));
+  return FVK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,125 @@
+// Host-side state of an fvk device context.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/fvk.h"
+#include "dev.h"
+
+namespace fvk {
+
+void set_error(const std::string &msg);
+
+#define FVK_HIP(call)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess) {                                                              \
+      ::fvk::set_error(std::string(#call) + ": " + hipGetErrorString(e_));              \
+      return FVK_ERR_HIP;                                                                \
+    }                                                                                    \
+  } while (0)
+
+// Per-component Krylov scalars, kept in device memory so that a whole solve
+// runs without host round trips (done flags gate every kernel).
+struct KS {
+  double res0_n;  // initial L1 norm
+  double res_n;   // current relative L1 norm
+  double s, alpha, beta;                 // PCG (solv_cg.f90:28)
+  double w1, w2, w3, w4, w5, w6;         // BiCGStab w_sc(1..6) (solv_bicg.f90:30)
+  double reduc;
+  int iter, n_iter, done, pad;
+};
+
+struct Levels {
+  std::vector<int> ptr;  // host: level offsets into rows
+  int *rows = nullptr;   // device
+  int count() const { return ptr.empty() ? 0 : (int)ptr.size() - 1; }
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int n = 0, nf = 0, nbf = 0, W = 0, Ws = 0, ndx = 0;
+  std::vector<int> perm, iperm;  // perm[row] = reference cell (0-based)
+  std::vector<int> h_fip, h_fin, h_fib, h_fbty;  // rows
+  DMesh M{};
+  std::vector<void *> owned;      // device allocations freed at destroy
+  // matrix on the device
+  double *diag = nullptr, *spv = nullptr;  // ELL values (Ws*n)
+  double *ngp = nullptr, *ngn = nullptr;   // face coefficients (nf)
+  // preconditioner (per imat: c, c_r)
+  double *pc_c = nullptr, *pc_cr = nullptr;
+  int pc_imat = -1;
+  bool pc_valid = false;
+  Levels lf, lb;                  // forward / backward level sets
+  // Krylov workspace (3 components)
+  double *wk = nullptr;           // 8 vectors x 3 components x n
+  KS *ks = nullptr;               // 3 components
+  Red red{};
+  double *h_part = nullptr;
+  // scratch for reference-order transfers
+  double *stage = nullptr;        // max(n, nf) * 3 doubles
+  double *sys_b = nullptr, *sys_x = nullptr, *sys_r = nullptr;  // stand-alone system
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+template <class T>
+int dalloc(Ctx &c, T **p, size_t n) {
+  void *q = nullptr;
+  FVK_HIP(hipMalloc(&q, (n ? n : 1) * sizeof(T)));
+  c.owned.push_back(q);
+  FVK_HIP(hipMemsetAsync(q, 0, (n ? n : 1) * sizeof(T), c.stream));
+  FVK_HIP(hipStreamSynchronize(c.stream));
+  *p = (T *)q;
+  return 0;
+}
+
+// krylov.hip
+int precond_build(Ctx &c, int imat);
+int precond_apply(Ctx &c, int imat, int nvc, const double *b, double *x, const KS *gate);
+int spmv(Ctx &c, int nvc, const double *x, double *y);
+int residual(Ctx &c, int nvc, const double *x, double *r, bool not_r0);
+// Times one kernel per iteration with event pairs on the launch stream.
+struct KTimer {
+  std::vector<hipEvent_t> ev;
+  int used = 0;
+  double total_ms = 0.0;
+  long launches = 0;
+  void init(int pairs) {
+    ev.resize(2 * pairs);
+    for (auto &e : ev) hipEventCreate(&e);
+  }
+  void begin(hipStream_t s) { if (used < (int)ev.size() / 2) hipEventRecord(ev[2 * used], s); }
+  void end(hipStream_t s) { if (used < (int)ev.size() / 2) { hipEventRecord(ev[2 * used + 1], s); ++used; } }
+  int flush() {
+    for (int k = 0; k < used; ++k) {
+      float ms = 0.f;
+      if (hipEventSynchronize(ev[2 * k + 1]) != hipSuccess) return 1;
+      if (hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]) != hipSuccess) return 1;
+      total_ms += ms;
+      ++launches;
+    }
+    used = 0;
+    return 0;
+  }
+  ~KTimer() {
+    for (auto &e : ev) hipEventDestroy(e);
+  }
+};
+
+// kind 0: PCG, 1: BiCGStab.  x, r0: nvc component vectors in device rows;
+// r0 holds the RHS on entry and the initial residual on exit.  chunk > 0:
+// stop queueing iterations once converged (checked every chunk iterations).
+int krylov(Ctx &c, int kind, int imat, double reduc, int n_iter, int nvc, double *x, double *r0, bool not_r0,
+           int chunk = 0, KTimer *T = nullptr);
+int true_residual(Ctx &c, const double *x, const double *b, double *dev_out4, double *host_out4);
+
+// flow.hip
+// solv_grad: n_iter Green-Gauss sweeps; g0 is scratch of the size of g;
+// on_flag (device, may be null): all-zero gradient unless *on_flag > small.
+int grad(Ctx &c, int nvc, const double *phi, const double *phib, double *g, int n_iter, int neu,
+         const double *on_flag, double *g0);
+
+}  // namespace fvk
