@@ -66,7 +66,7 @@ def main():
     ap.add_argument("--jitter", type=float, default=0.2)
     ap.add_argument("--sigma", type=float, default=1.0)
     ap.add_argument("--tol", type=float, default=1e-8)
-    ap.add_argument("--cpu-sample", type=int, default=150)
+    ap.add_argument("--cpu-sample", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -246,8 +246,8 @@ static int ctx_build(Ctx &c, const Topo &t, int device, int flags) {
   if ((rc = dalloc(c, &c.pc_cr, n))) return rc;
   if ((rc = dalloc(c, &c.wk, (size_t)8 * 3 * n))) return rc;
   if ((rc = dalloc(c, &c.ks, 3))) return rc;
-  if ((rc = dalloc(c, &c.red.part, (size_t)MAX_RED_BLOCKS * 8))) return rc;
-  if ((rc = dalloc(c, &c.red.ticket, 4))) return rc;
+  if ((rc = dalloc(c, &c.ksr, 6))) return rc;
+  if ((rc = dalloc(c, &c.part, (size_t)4 * RG * 8))) return rc;
   if ((rc = dalloc(c, &c.stage, (size_t)3 * std::max(n, nf)))) return rc;
   FVK_HIP(hipStreamSynchronize(c.stream));
   return 0;
@@ -616,8 +616,8 @@ int fvk_solv_grad(fvk_ctx *cc, int n_vc, const double *phi, double *gout, int n_
   if (!cc || !phi || !gout || n_vc < 1 || n_vc > 3 || n_iter < 1) { set_error("bad argument"); return FVK_ERR_ARG; }
   Ctx &c = cc->c;
   const int n = c.n, nb = c.nbf, nd = c.ndx;
-  double *dphi = c.wk, *dphib = c.wk + (size_t)3 * n, *g = c.wk + (size_t)2 * 3 * n, *g0 = c.wk + (size_t)5 * 3 * n;
-  if ((size_t)3 * nb > (size_t)3 * n) { set_error("more boundary faces than cells"); return FVK_ERR_ARG; }
+  if (!c.bnd_buf) GUARD(dalloc(c, &c.bnd_buf, (size_t)3 * std::max(nb, 1)));
+  double *dphi = c.wk, *dphib = c.bnd_buf, *g = c.wk + (size_t)2 * 3 * n, *g0 = c.wk + (size_t)5 * 3 * n;
   GUARD(to_dev(c, phi, n_vc, n_vc, n, &c.perm, dphi));
   if (nb) GUARD(to_dev(c, phi + (size_t)n * n_vc, n_vc, n_vc, nb, nullptr, dphib));
   GUARD(grad(c, n_vc, dphi, dphib, g, n_iter, neu, nullptr, g0));

@@ -56,12 +56,13 @@ struct Ctx {
   Levels lf, lb;                  // forward / backward level sets
   // Krylov workspace (3 components)
   double *wk = nullptr;           // 8 vectors x 3 components x n
-  KS *ks = nullptr;               // 3 components
-  Red red{};
-  double *h_part = nullptr;
+  KS *ks = nullptr;               // 3 components: state after the last kernel / solve
+  KS *ksr = nullptr;              // 2 x 3: scalar ring indexed by iteration parity
+  double *part = nullptr;         // 4 partial buffers of RG x 8 doubles
   // scratch for reference-order transfers
   double *stage = nullptr;        // max(n, nf) * 3 doubles
   double *sys_b = nullptr, *sys_x = nullptr, *sys_r = nullptr;  // stand-alone system
+  double *bnd_buf = nullptr;      // 3 x n_bf boundary values (fvk_solv_grad)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 

@@ -7,9 +7,10 @@
 
 namespace fvk {
 
-constexpr int BS = 256;          // threads per block (4 waves)
+constexpr int BS = 256;          // threads per block of the row kernels (4 waves)
 constexpr int WAVE = 64;
-constexpr int MAX_RED_BLOCKS = 2048;  // grid of the reduction kernels
+constexpr int RB = 512;          // threads per block of the reduction kernels (8 waves)
+constexpr int RG = 512;          // at most RG blocks (partials) per reduction kernel
 // small = ten**(-20) as the flang build of spray-simulation/precision.f90:8
 // evaluates it (bit pattern 0x3BC79CA10C924224); large = one/small.
 constexpr double SMALL = 0x1.79ca10c924224p-67;
@@ -90,49 +91,73 @@ __device__ __forceinline__ void block_max(double (&v)[NV]) {
     for (int j = 0; j < NV; ++j) v[j] = fmax(fmax(red_mx[j][0], red_mx[j][1]), fmax(red_mx[j][2], red_mx[j][3]));
 }
 
-// Grid-wide sum: every block stores its partial, the last block to arrive
-// (ticket) adds the partials in block order and returns true in thread 0
-// with the totals in v.  The ticket is reset by that block, so consecutive
-// kernels on one stream can share it.
-struct Red {
-  double *part;       // MAX_RED_BLOCKS * 8
-  unsigned *ticket;
-};
+// ---- fence-free grid reductions ----
+// A producer kernel leaves one partial per block; the next kernel reads them
+// (kernel boundaries make plain stores visible) and every block adds them in
+// the same fixed tree.
 
-template <int NV>
-__device__ __forceinline__ bool grid_sum(double (&v)[NV], Red R) {
-  block_sum<NV>(v);
-  __shared__ int red_last;
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int j = 0; j < NV; ++j)
-      __hip_atomic_store(&R.part[blockIdx.x * NV + j], v[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    unsigned t = atomicAdd(R.ticket, 1u);
-    red_last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!red_last) return false;
-  __threadfence();
-  double acc[NV];
-#pragma unroll
-  for (int j = 0; j < NV; ++j) acc[j] = 0.0;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += BS)
-#pragma unroll
-    for (int j = 0; j < NV; ++j)
-      acc[j] += __hip_atomic_load(&R.part[b * NV + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  block_sum<NV>(acc);
-  if (threadIdx.x == 0) {
-    *R.ticket = 0u;
-#pragma unroll
-    for (int j = 0; j < NV; ++j) v[j] = acc[j];
-  }
-  return threadIdx.x == 0;
+// Row range of a reduction kernel's block: contiguous chunks, and with a
+// grid that is a multiple of 8 the blocks that land on one XCD (dispatched
+// round-robin by block id) own one contiguous eighth of the rows, so
+// neighbour gathers stay in that XCD's L2.
+__device__ __forceinline__ void block_rows(int n, int &r0, int &r1) {
+  const int G = gridDim.x;
+  const int lb = (G % 8 == 0) ? (int)(blockIdx.x % 8) * (G / 8) + (int)(blockIdx.x / 8) : (int)blockIdx.x;
+  const int chunk = ((n + G - 1) / G + RB - 1) / RB * RB;
+  r0 = lb * chunk;
+  r1 = min(n, r0 + chunk);
 }
 
+template <int NV>
+__device__ __forceinline__ void rblock_sum(double (&v)[NV]) {
+  __shared__ double sm[NV][RB / WAVE];
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1)
+#pragma unroll
+    for (int j = 0; j < NV; ++j) v[j] += __shfl_xor(v[j], o, WAVE);
+  const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int j = 0; j < NV; ++j) sm[j][w] = v[j];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < NV; ++j) {
+    double a = 0.0;
+#pragma unroll
+    for (int k = 0; k < RB / WAVE; ++k) a += sm[j][k];
+    v[j] = a;  // every thread holds the block total
+  }
+}
+
+// This block's partials -> part[blockIdx.x*NV + j]
+template <int NV>
+__device__ __forceinline__ void store_partials(double (&v)[NV], double *part) {
+  rblock_sum<NV>(v);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int j = 0; j < NV; ++j) part[blockIdx.x * NV + j] = v[j];
+}
+
+// Totals of a previous kernel's P partials, identical in every block.
+template <int NV>
+__device__ __forceinline__ void load_totals(const double *part, int P, double (&v)[NV]) {
+#pragma unroll
+  for (int j = 0; j < NV; ++j) v[j] = 0.0;
+  for (int b = threadIdx.x; b < P; b += RB)
+#pragma unroll
+    for (int j = 0; j < NV; ++j) v[j] += part[b * NV + j];
+  rblock_sum<NV>(v);
+}
+
+// Grid of the reduction kernels: at most RG blocks of RB threads, a multiple
+// of 8 once there are enough rows (XCD-contiguous row chunks).
 inline int red_grid(int n) {
-  int g = (n + BS - 1) / BS;
-  return g < MAX_RED_BLOCKS ? (g < 1 ? 1 : g) : MAX_RED_BLOCKS;
+  int g = (n + RB - 1) / RB;
+  if (g < 1) g = 1;
+  if (g > RG) g = RG;
+  if (g >= 16) g = g / 8 * 8;
+  return g;
 }
 inline int grid_for(int n) { return n > 0 ? (n + BS - 1) / BS : 1; }
 

@@ -129,10 +129,27 @@ __global__ __launch_bounds__(BS) void k_absmax(int n, int nvc, const double *__r
   }
 }
 
-__global__ __launch_bounds__(BS) void k_sum(int n, const double *__restrict__ a, Red R, double *out) {
+__global__ __launch_bounds__(RB) void k_sum(int n, const double *__restrict__ a, double *part) {
   double s[1] = {0.0};
-  for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) s[0] += a[i];
-  if (grid_sum<1>(s, R)) *out = s[0];
+  int r0, r1;
+  block_rows(n, r0, r1);
+  for (int i = r0 + threadIdx.x; i < r1; i += RB) s[0] += a[i];
+  store_partials<1>(s, part);
+}
+
+__global__ __launch_bounds__(RB) void k_sum_total(const double *part, int P, double *out) {
+  double s[1];
+  load_totals<1>(part, P, s);
+  if (threadIdx.x == 0) *out = s[0];
+}
+
+static int sum_to(Ctx &c, const double *a, double *out) {
+  const int G = red_grid(c.n);
+  double *part = c.part + (size_t)3 * RG * 8;
+  k_sum<<<G, RB, 0, c.stream>>>(c.n, a, part);
+  k_sum_total<<<1, RB, 0, c.stream>>>(part, G, out);
+  FVK_HIP(hipGetLastError());
+  return 0;
 }
 
 // ---------------------------------------------------------------- momentum
@@ -634,7 +651,7 @@ int mom_setup(Flow &F, int it, bool &skipped) {
   MomPar P{F.cfg.rho, F.cfg.mu, F.cfg.c_bl, F.cfg.urf_u, F.cfg.c_sch};
   skipped = false;
   FVK_HIP(hipMemsetAsync(&F.sc->phi_mx_cell, 0, 2 * sizeof(double), c.stream));
-  k_absmax<<<red_grid(n > c.nbf ? n : c.nbf), BS, 0, c.stream>>>(n, nvc, F.u, c.nbf, F.ub, &F.sc->phi_mx_cell,
+  k_absmax<<<grid_for(n > c.nbf ? n : c.nbf) < 2048 ? grid_for(n > c.nbf ? n : c.nbf) : 2048, BS, 0, c.stream>>>(n, nvc, F.u, c.nbf, F.ub, &F.sc->phi_mx_cell,
                                                                  &F.sc->phi_mx_all);
   // the early return when every velocity is zero (cfd_setup.f90:170-171)
   double mx[2];
@@ -686,7 +703,7 @@ int pp_setup(Flow &F, int it) {
   const int chunk = F.cfg.n_cg > 64 ? 64 : 0;  // long solves stop queueing once converged
   int rc = krylov(c, 0, F.cfg.imat_p, F.cfg.reduc_p, F.cfg.n_cg, 1, F.pc, F.rhs, true, chunk);
   if (rc) return rc;
-  k_sum<<<red_grid(n), BS, 0, c.stream>>>(n, F.rhs, c.red, F.red_scratch);
+  if ((rc = sum_to(c, F.rhs, F.red_scratch))) return rc;
   k_pp_norm<<<1, 1, 0, c.stream>>>(it, 0, c.ks, F.red_scratch, F.sc);
   if (!F.cfg.simpler) k_p_relax<<<grid_for(n), BS, 0, c.stream>>>(n, F.cfg.urf_p, F.pc, F.row_of_cell1, F.p);
   rc = grad(c, 1, F.pc, nullptr, F.ddxp, F.cfg.n_grad, 2, nullptr, F.g0);
@@ -700,7 +717,7 @@ int pp_setup(Flow &F, int it) {
     k_flux_cell<<<grid_for(n), BS, 0, c.stream>>>(c.M, F.cpp, F.rhs, F.rhs_p);
     rc = krylov(c, 0, F.cfg.imat_p, F.cfg.reduc_p, F.cfg.n_cg, 1, F.p, F.rhs_p, true, chunk);
     if (rc) return rc;
-    k_sum<<<red_grid(n), BS, 0, c.stream>>>(n, F.rhs_p, c.red, F.red_scratch);
+    if ((rc = sum_to(c, F.rhs_p, F.red_scratch))) return rc;
     k_pp_norm<<<1, 1, 0, c.stream>>>(it, 1, c.ks, F.red_scratch, F.sc);
   }
   rc = grad(c, 1, F.p, nullptr, F.dpdx, F.cfg.n_grad, 2, nullptr, F.g0);

@@ -12,19 +12,34 @@
 // Row sums run over a row's faces in ascending face index, the order in
 // which the reference's face-scatter loops reach that row, so SpMV,
 // residual and the sweeps reproduce the reference bit for bit (compiled
-// without FMA contraction).  Dot products and L1 norms are deterministic
-// tree reductions (block partials, then the last block adds them in block
-// order); they differ from the reference's sequential sums by rounding only.
+// without FMA contraction).
+//
+// Reductions (dot products, L1 norms) never synchronise inside a kernel.  A
+// producer kernel leaves one partial per block (plain stores); the next
+// kernel on the stream, where the kernel boundary makes them visible, adds
+// the partials in every block in the same fixed tree, so all blocks hold the
+// same bits and take the same decisions (alpha, beta, exit test) without a
+// device-scope fence.  Block 0 commits the scalars into a two-slot ring
+// indexed by iteration parity, so a kernel never rewrites a slot that its
+// own blocks read.  The order differs from the reference's sequential sums
+// by rounding only.
 #include <cstring>
 
 #include "ctx.h"
 
 namespace fvk {
 
+__device__ __forceinline__ bool all_done(const KS *ks, int nvc) {
+  if (!ks) return false;
+  for (int q = 0; q < nvc; ++q)
+    if (!ks[q].done) return false;
+  return true;
+}
+
 // ---------------------------------------------------------------- kernels
 
-__global__ __launch_bounds__(BS) void k_precond_jacobi(int n, const double *__restrict__ diag, double *__restrict__ c,
-                                 double *__restrict__ cr) {
+__global__ __launch_bounds__(BS) void k_precond_jacobi(int n, const double *__restrict__ diag,
+                                                       double *__restrict__ c, double *__restrict__ cr) {
   int i = blockIdx.x * BS + threadIdx.x;
   if (i >= n) return;
   double d = diag[i];
@@ -34,9 +49,9 @@ __global__ __launch_bounds__(BS) void k_precond_jacobi(int n, const double *__re
 
 // One level of the DIC (imat 1) / DILU (imat 2) factorisation.
 __global__ __launch_bounds__(BS) void k_precond_level(DMesh M, const int *__restrict__ rows, int cnt, int imat,
-                                const double *__restrict__ diag, const double *__restrict__ ngp,
-                                const double *__restrict__ ngn, double *__restrict__ c,
-                                double *__restrict__ cr) {
+                                                      const double *__restrict__ diag, const double *__restrict__ ngp,
+                                                      const double *__restrict__ ngn, double *__restrict__ c,
+                                                      double *__restrict__ cr) {
   int t = blockIdx.x * BS + threadIdx.x;
   if (t >= cnt) return;
   const int r = rows[t], n = M.n, rr = M.rank[r];
@@ -54,15 +69,9 @@ __global__ __launch_bounds__(BS) void k_precond_level(DMesh M, const int *__rest
   c[r] = 1.0 / (d + SMALL);
 }
 
-__device__ __forceinline__ bool all_done(const KS *ks, int nvc) {
-  if (!ks) return false;
-  for (int q = 0; q < nvc; ++q)
-    if (!ks[q].done) return false;
-  return true;
-}
-
-__global__ __launch_bounds__(BS) void k_jacobi_apply(int n, int nvc, const double *__restrict__ c, const double *__restrict__ b,
-                               double *__restrict__ x, const KS *ks) {
+__global__ __launch_bounds__(BS) void k_jacobi_apply(int n, int nvc, const double *__restrict__ c,
+                                                     const double *__restrict__ b, double *__restrict__ x,
+                                                     const KS *ks) {
   int i = blockIdx.x * BS + threadIdx.x;
   if (i >= n || all_done(ks, nvc)) return;
   for (int q = 0; q < nvc; ++q) x[q * n + i] = b[q * n + i] * c[i];
@@ -70,8 +79,9 @@ __global__ __launch_bounds__(BS) void k_jacobi_apply(int n, int nvc, const doubl
 
 // Forward sweep over one level: x = (b - L x) * c   (solv_direct.f90:12-24)
 __global__ __launch_bounds__(BS) void k_sweep_fwd(DMesh M, const int *__restrict__ rows, int cnt, int nvc,
-                            const double *__restrict__ spv, const double *__restrict__ c,
-                            const double *__restrict__ b, double *__restrict__ x, const KS *ks) {
+                                                  const double *__restrict__ spv, const double *__restrict__ c,
+                                                  const double *__restrict__ b, double *__restrict__ x,
+                                                  const KS *ks) {
   int t = blockIdx.x * BS + threadIdx.x;
   if (t >= cnt || all_done(ks, nvc)) return;
   const int r = rows[t], n = M.n, rr = M.rank[r];
@@ -91,8 +101,9 @@ __global__ __launch_bounds__(BS) void k_sweep_fwd(DMesh M, const int *__restrict
 
 // Backward sweep over one level: x = (x*c_r - U x) * c   (solv_direct.f90:26-38)
 __global__ __launch_bounds__(BS) void k_sweep_bwd(DMesh M, const int *__restrict__ rows, int cnt, int nvc,
-                            const double *__restrict__ spv, const double *__restrict__ c,
-                            const double *__restrict__ cr, double *__restrict__ x, const KS *ks) {
+                                                  const double *__restrict__ spv, const double *__restrict__ c,
+                                                  const double *__restrict__ cr, double *__restrict__ x,
+                                                  const KS *ks) {
   int t = blockIdx.x * BS + threadIdx.x;
   if (t >= cnt || all_done(ks, nvc)) return;
   const int r = rows[t], n = M.n, rr = M.rank[r];
@@ -125,22 +136,27 @@ __device__ __forceinline__ double row_spmv(const DMesh &M, const double *__restr
   return y;
 }
 
-__global__ __launch_bounds__(BS) void k_spmv(DMesh M, int nvc, const double *__restrict__ diag, const double *__restrict__ spv,
-                       const double *__restrict__ x, double *__restrict__ y) {
+__global__ __launch_bounds__(BS) void k_spmv(DMesh M, int nvc, const double *__restrict__ diag,
+                                             const double *__restrict__ spv, const double *__restrict__ x,
+                                             double *__restrict__ y) {
   int i = blockIdx.x * BS + threadIdx.x;
   if (i >= M.n) return;
   for (int q = 0; q < nvc; ++q) y[q * M.n + i] = row_spmv(M, diag, spv, x + q * M.n, i);
 }
 
-// r = b - A x (or r = b), L1 norm per component, Krylov scalar set-up.
-// kind: 0 = CG, 1 = BiCGStab.  (solv_res.f90, solv_cg.f90:19-33,
-// solv_bicg.f90:19-38)
-__global__ __launch_bounds__(BS) void k_residual(DMesh M, int nvc, const double *__restrict__ diag,
-                           const double *__restrict__ spv, const double *__restrict__ x,
-                           double *__restrict__ r, int not_r0, Red R, KS *ks) {
+// r = b - A x (or r = b) with partials {|r|_1[q], s2[q]} where s2 = r.(c r)
+// and z = c r (mode 1, Jacobi PCG), s2 = r.r (mode 2, BiCGStab) or 0.
+// (solv_res.f90; solv_cg.f90:19-33; solv_bicg.f90:19-38)
+__global__ __launch_bounds__(RB) void k_residual(DMesh M, int nvc, const double *__restrict__ diag,
+                                                 const double *__restrict__ spv, const double *__restrict__ x,
+                                                 double *__restrict__ r, int not_r0, int mode,
+                                                 const double *__restrict__ c, double *__restrict__ z,
+                                                 double *__restrict__ part) {
   const int n = M.n;
-  double l1[3] = {0.0, 0.0, 0.0};
-  for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+  double a[6] = {0, 0, 0, 0, 0, 0};
+  int r0, r1;
+  block_rows(n, r0, r1);
+  for (int i = r0 + threadIdx.x; i < r1; i += RB)
     for (int q = 0; q < nvc; ++q) {
       double v = r[q * n + i];
       if (!not_r0) {
@@ -153,233 +169,359 @@ __global__ __launch_bounds__(BS) void k_residual(DMesh M, int nvc, const double 
         }
         r[q * n + i] = v;
       }
-      l1[q] += fabs(v);
-    }
-  }
-  if (grid_sum<3>(l1, R)) {
-    double mx = 0.0;
-    for (int q = 0; q < nvc; ++q) {
-      ks[q].res0_n = l1[q];
-      mx = q == 0 ? l1[q] : fmax(mx, l1[q]);
-    }
-    const int stop = mx < SMALL;  // solv_cg.f90:21
-    for (int q = 0; q < nvc; ++q) ks[q].done = stop;
-  }
-}
-
-// ---- PCG ----
-// z = M^-1 r (Jacobi: computed here) and s = r.z; beta = (1/s_old)*s
-__global__ __launch_bounds__(BS) void k_cg_rz(int n, int nvc, const double *__restrict__ r, const double *__restrict__ c,
-                        int jacobi, double *__restrict__ z, Red R, KS *ks) {
-  if (all_done(ks, nvc)) return;
-  double rz[3] = {0.0, 0.0, 0.0};
-  for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS)
-    for (int q = 0; q < nvc; ++q) {
-      if (ks[q].done) continue;
-      double ri = r[q * n + i], zi;
-      if (jacobi) {
-        zi = ri * c[i];
+      a[q] += fabs(v);
+      if (mode == 1) {
+        const double zi = v * c[i];
         z[q * n + i] = zi;
-      } else {
-        zi = z[q * n + i];
+        a[3 + q] += v * zi;
+      } else if (mode == 2) {
+        a[3 + q] += v * v;
       }
-      rz[q] += ri * zi;
     }
-  if (grid_sum<3>(rz, R))
-    for (int q = 0; q < nvc; ++q) {
-      KS &k = ks[q];
-      if (k.done) continue;
-      k.beta = 1.0 / k.s;
-      k.s = rz[q];
-      k.beta = k.beta * k.s;
-    }
+  store_partials<6>(a, part);
 }
 
-// p = z + beta p ; q = A p ; alpha = s / (p.q + small).  p of the
-// neighbours is formed on the fly from (z, p_old), so p_new is written once
-// and no extra pass over p is needed.
-__global__ __launch_bounds__(BS) void k_cg_pq(DMesh M, int nvc, const double *__restrict__ diag, const double *__restrict__ spv,
-                        const double *__restrict__ z, const double *__restrict__ pold, double *__restrict__ pnew,
-                        double *__restrict__ qv, Red R, KS *ks) {
-  if (all_done(ks, nvc)) return;
+// ---- PCG (solv_cg.f90) ----
+// Scalars at the start of iteration it from the totals of the previous
+// kernel: L1 = |r|_1, rz = r.z.  it = 0: L1 is res0_n; otherwise the exit
+// test on res_n = L1/(res0_n+small) of iteration it (it iterations done).
+__device__ __forceinline__ void cg_decide(int it, int nvc, const double *L1, const double *rz, const KS *prev,
+                                          KS *cur) {
+  double mx = 0.0;
+  for (int q = 0; q < nvc; ++q) {
+    KS k = prev[q];
+    if (it == 0) {
+      k.res0_n = L1[q];
+      k.iter = 0;
+      k.res_n = 0.0;
+      mx = q == 0 ? L1[q] : fmax(mx, L1[q]);
+    } else if (!k.done) {
+      k.res_n = L1[q] / (k.res0_n + SMALL);
+      k.iter = it;
+      if (k.res_n < k.reduc || it >= k.n_iter) k.done = 1;
+    }
+    cur[q] = k;
+  }
+  if (it == 0)
+    for (int q = 0; q < nvc; ++q)
+      if (mx < SMALL || cur[q].n_iter <= 0) cur[q].done = 1;  // solv_cg.f90:21
+  for (int q = 0; q < nvc; ++q) {
+    KS &k = cur[q];
+    if (k.done) continue;
+    k.beta = 1.0 / k.s;
+    k.s = rz[q];
+    k.beta = k.beta * k.s;
+  }
+}
+
+// Decide iteration it, then p = z + beta p ; q = A p ; partial p.q.  p of the
+// neighbours is formed on the fly from (z, p_old): p_new is written once.
+template <int NVC>
+__global__ __launch_bounds__(RB) void k_cg_pq(DMesh M, int it, const double *__restrict__ diag,
+                                              const double *__restrict__ spv, const double *__restrict__ z,
+                                              const double *__restrict__ pold, double *__restrict__ pnew,
+                                              double *__restrict__ qv, const double *__restrict__ partU, int PU,
+                                              const double *__restrict__ partR, int PR, double *__restrict__ partQ,
+                                              KS *ring, KS *out) {
+  double t[6];
+  load_totals<6>(partU, PU, t);
+  double rz[3] = {t[3], t[4], t[5]};
+  if (partR) {
+    double tr[3];
+    load_totals<3>(partR, PR, tr);
+    rz[0] = tr[0]; rz[1] = tr[1]; rz[2] = tr[2];
+  }
+  KS cur[NVC];
+  cg_decide(it, NVC, t, rz, ring + ((it + 1) & 1) * 3, cur);
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    for (int q = 0; q < NVC; ++q) ring[(it & 1) * 3 + q] = out[q] = cur[q];
+  bool any = false;
+  for (int q = 0; q < NVC; ++q) any = any || !cur[q].done;
+  if (!any) return;
   const int n = M.n;
   double pq[3] = {0.0, 0.0, 0.0};
-  double beta[3];
-  for (int q = 0; q < nvc; ++q) beta[q] = ks[q].beta;
-  for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS)
-    for (int q = 0; q < nvc; ++q) {
-      if (ks[q].done) continue;
+  int r0, r1;
+  block_rows(n, r0, r1);
+  for (int i = r0 + threadIdx.x; i < r1; i += RB)
+#pragma unroll
+    for (int q = 0; q < NVC; ++q) {
+      if (cur[q].done) continue;
+      const double beta = cur[q].beta;
       const double *zq = z + q * n, *po = pold + q * n;
-      const double pi = zq[i] + beta[q] * po[i];
+      const double pi = zq[i] + beta * po[i];
       double y = diag[i] * pi;
       for (int k = 0; k < M.Ws; ++k) {
         const int j = M.spc[k * n + i];
         if (j < 0) break;
-        y = y + spv[k * n + i] * (zq[j] + beta[q] * po[j]);
+        y = y + spv[k * n + i] * (zq[j] + beta * po[j]);
       }
       pnew[q * n + i] = pi;
       qv[q * n + i] = y;
       pq[q] += pi * y;
     }
-  if (grid_sum<3>(pq, R))
-    for (int q = 0; q < nvc; ++q) {
-      KS &k = ks[q];
-      if (k.done) continue;
-      k.alpha = k.s / (pq[q] + SMALL);
-    }
+  store_partials<3>(pq, partQ);
 }
 
-// x += alpha p ; r -= alpha q ; res_n = |r|_1 / (res0_n + small) ; exit test.
-// With Jacobi the next z = c r and s = r.z are formed here too.
-__global__ __launch_bounds__(BS) void k_cg_upd(int n, int nvc, double *__restrict__ x, double *__restrict__ r,
-                         const double *__restrict__ p, const double *__restrict__ qv,
-                         const double *__restrict__ c, int jacobi, double *__restrict__ z, Red R, KS *ks) {
-  if (all_done(ks, nvc)) return;
-  double acc[6] = {0, 0, 0, 0, 0, 0};
-  double alpha[3];
-  for (int q = 0; q < nvc; ++q) alpha[q] = ks[q].alpha;
-  for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS)
-    for (int q = 0; q < nvc; ++q) {
-      if (ks[q].done) continue;
+// alpha = s/(p.q + small) ; x += alpha p ; r -= alpha q ; partials |r|_1 and
+// (Jacobi) r.z with z = c r.
+template <int NVC>
+__global__ __launch_bounds__(RB) void k_cg_upd(int n, int it, double *__restrict__ x, double *__restrict__ r,
+                                               const double *__restrict__ p, const double *__restrict__ qv,
+                                               const double *__restrict__ c, int jacobi, double *__restrict__ z,
+                                               const double *__restrict__ partQ, int PQ, double *__restrict__ partU,
+                                               KS *ring) {
+  const KS *cur = ring + (it & 1) * 3;
+  bool any = false;
+  for (int q = 0; q < NVC; ++q) any = any || !cur[q].done;
+  if (!any) return;
+  double t[3];
+  load_totals<3>(partQ, PQ, t);
+  double alpha[NVC];
+  int dn[NVC];
+  for (int q = 0; q < NVC; ++q) {
+    dn[q] = cur[q].done;
+    alpha[q] = cur[q].s / (t[q] + SMALL);
+  }
+  double a[6] = {0, 0, 0, 0, 0, 0};
+  int r0, r1;
+  block_rows(n, r0, r1);
+  for (int i = r0 + threadIdx.x; i < r1; i += RB)
+#pragma unroll
+    for (int q = 0; q < NVC; ++q) {
+      if (dn[q]) continue;
       const int o = q * n + i;
       x[o] = x[o] + alpha[q] * p[o];
       const double ri = r[o] - alpha[q] * qv[o];
       r[o] = ri;
-      acc[q] += fabs(ri);
+      a[q] += fabs(ri);
       if (jacobi) {
         const double zi = ri * c[i];
         z[o] = zi;
-        acc[3 + q] += ri * zi;
+        a[3 + q] += ri * zi;
       }
     }
-  if (grid_sum<6>(acc, R))
-    for (int q = 0; q < nvc; ++q) {
-      KS &k = ks[q];
-      if (k.done) continue;
-      k.res_n = acc[q] / (k.res0_n + SMALL);
-      k.iter += 1;
-      if (k.res_n < k.reduc || k.iter >= k.n_iter) {
-        k.done = 1;
-      } else if (jacobi) {
-        k.beta = 1.0 / k.s;
-        k.s = acc[3 + q];
-        k.beta = k.beta * k.s;
-      }
-    }
+  store_partials<6>(a, partU);
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    for (int q = 0; q < NVC; ++q) ring[(it & 1) * 3 + q].alpha = alpha[q];
 }
 
-// ---- BiCGStab (w_sc: res_n=1 alpha=2 beta0=3 beta=4 gam=5 omeg=6) ----
-__global__ __launch_bounds__(BS) void k_bi_rr0(int n, int nvc, const double *__restrict__ r, const double *__restrict__ r0, Red R,
-                         KS *ks) {
-  if (all_done(ks, nvc)) return;
-  double a[3] = {0, 0, 0};
-  for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS)
-    for (int q = 0; q < nvc; ++q)
-      if (!ks[q].done) a[q] += r[q * n + i] * r0[q * n + i];
-  if (grid_sum<3>(a, R))
-    for (int q = 0; q < nvc; ++q) {
-      KS &k = ks[q];
-      if (k.done) continue;
-      k.w4 = a[q];
-      k.w6 = k.w4 * k.w5 / (k.w2 * k.w3 + SMALL);
-      k.w3 = k.w4;
-    }
-}
-
-// p = r + w6 (p - w2 u) ; Jacobi: z = c p
-__global__ __launch_bounds__(BS) void k_bi_p(int n, int nvc, const double *__restrict__ r, double *__restrict__ p,
-                       const double *__restrict__ u, const double *__restrict__ c, int jacobi,
-                       double *__restrict__ z, const KS *ks) {
-  int i = blockIdx.x * BS + threadIdx.x;
-  if (i >= n) return;
-  for (int q = 0; q < nvc; ++q) {
-    if (ks[q].done) continue;
-    const int o = q * n + i;
-    const double pi = r[o] + ks[q].w6 * (p[o] - ks[q].w2 * u[o]);
-    p[o] = pi;
-    if (jacobi) z[o] = pi * c[i];
+// r.z after the sweeps (DIC / DILU)
+__global__ __launch_bounds__(RB) void k_dot(int n, int nvc, const double *__restrict__ a,
+                                            const double *__restrict__ b, double *__restrict__ part, const KS *gate) {
+  double s[3] = {0, 0, 0};
+  if (!all_done(gate, nvc)) {
+    int r0, r1;
+    block_rows(n, r0, r1);
+    for (int i = r0 + threadIdx.x; i < r1; i += RB)
+      for (int q = 0; q < nvc; ++q) s[q] += a[q * n + i] * b[q * n + i];
   }
+  store_partials<3>(s, part);
 }
 
-// y = A z with one dot (mode 0: y.r0 -> w5) or two (mode 1: y.r, y.y -> w2)
-__global__ __launch_bounds__(BS) void k_bi_spmv(DMesh M, int nvc, int mode, const double *__restrict__ diag,
-                          const double *__restrict__ spv, const double *__restrict__ z, double *__restrict__ y,
-                          const double *__restrict__ other, Red R, KS *ks) {
-  if (all_done(ks, nvc)) return;
-  const int n = M.n;
-  double a[6] = {0, 0, 0, 0, 0, 0};
-  for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS)
-    for (int q = 0; q < nvc; ++q) {
-      if (ks[q].done) continue;
-      const double yi = row_spmv(M, diag, spv, z + q * n, i);
-      y[q * n + i] = yi;
-      a[q] += yi * other[q * n + i];
-      if (mode) a[3 + q] += yi * yi;
-    }
-  if (grid_sum<6>(a, R))
-    for (int q = 0; q < nvc; ++q) {
-      KS &k = ks[q];
-      if (k.done) continue;
-      if (mode == 0) {
-        k.w5 = a[q];
-        k.w5 = k.w4 / (k.w5 + SMALL);
-      } else {
-        k.w1 = a[q];
-        k.w2 = a[3 + q];
-        k.w2 = k.w1 / (k.w2 + SMALL);
-      }
-    }
-}
-
-// x += w5 z ; r -= w5 u ; Jacobi: z = c r
-__global__ __launch_bounds__(BS) void k_bi_upd1(int n, int nvc, double *__restrict__ x, double *__restrict__ r,
-                          double *__restrict__ z, const double *__restrict__ u, const double *__restrict__ c,
-                          int jacobi, const KS *ks) {
-  int i = blockIdx.x * BS + threadIdx.x;
-  if (i >= n) return;
-  for (int q = 0; q < nvc; ++q) {
-    if (ks[q].done) continue;
-    const int o = q * n + i;
-    const double w5 = ks[q].w5;
-    x[o] = x[o] + w5 * z[o];
-    const double ri = r[o] - w5 * u[o];
-    r[o] = ri;
-    if (jacobi) z[o] = ri * c[i];
+// Final decision after the last queued iteration (1 block).
+__global__ __launch_bounds__(RB) void k_cg_final(int it, int nvc, const double *partU, int PU, const double *partR,
+                                                 int PR, KS *ring, KS *out) {
+  double t[6];
+  load_totals<6>(partU, PU, t);
+  double rz[3] = {t[3], t[4], t[5]};
+  if (partR) {
+    double tr[3];
+    load_totals<3>(partR, PR, tr);
+    rz[0] = tr[0]; rz[1] = tr[1]; rz[2] = tr[2];
   }
+  KS cur[3];
+  cg_decide(it, nvc, t, rz, ring + ((it + 1) & 1) * 3, cur);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < nvc; ++q) out[q] = cur[q];
 }
 
-// x += w2 z ; r -= w2 v ; w1 = |r|_1/(res0_n+small), exit test ; next w4, w6, w3
-__global__ __launch_bounds__(BS) void k_bi_upd2(int n, int nvc, double *__restrict__ x, double *__restrict__ r,
-                          const double *__restrict__ z, const double *__restrict__ v,
-                          const double *__restrict__ r0, Red R, KS *ks) {
-  if (all_done(ks, nvc)) return;
-  double a[6] = {0, 0, 0, 0, 0, 0};
-  for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS)
-    for (int q = 0; q < nvc; ++q) {
-      if (ks[q].done) continue;
-      const int o = q * n + i;
-      const double w2 = ks[q].w2;
-      x[o] = x[o] + w2 * z[o];
-      const double ri = r[o] - w2 * v[o];
-      r[o] = ri;
-      a[q] += fabs(ri);
-      a[3 + q] += ri * r0[o];
-    }
-  if (grid_sum<6>(a, R))
-    for (int q = 0; q < nvc; ++q) {
-      KS &k = ks[q];
-      if (k.done) continue;
-      k.w1 = a[q] / (k.res0_n + SMALL);
+// ---- BiCGStab (solv_bicg.f90; w_sc: res_n=1 alpha=2 beta0=3 beta=4 gam=5 omeg=6) ----
+__device__ __forceinline__ void bi_decide(int it, int nvc, const double *L1, const double *rr0, const KS *prev,
+                                          KS *cur) {
+  double mx = 0.0;
+  for (int q = 0; q < nvc; ++q) {
+    KS k = prev[q];
+    if (it == 0) {
+      k.res0_n = L1[q];
+      k.iter = 0;
+      mx = q == 0 ? L1[q] : fmax(mx, L1[q]);
+    } else if (!k.done) {
+      k.w1 = L1[q] / (k.res0_n + SMALL);
       k.res_n = k.w1;
-      k.iter += 1;
-      if (k.w1 < k.reduc || k.iter >= k.n_iter) {
-        k.done = 1;
-      } else {
-        k.w4 = a[3 + q];
-        k.w6 = k.w4 * k.w5 / (k.w2 * k.w3 + SMALL);
-        k.w3 = k.w4;
-      }
+      k.iter = it;
+      if (k.w1 < k.reduc || it >= k.n_iter) k.done = 1;
     }
+    cur[q] = k;
+  }
+  if (it == 0)
+    for (int q = 0; q < nvc; ++q)
+      if (mx < SMALL || cur[q].n_iter <= 0) cur[q].done = 1;  // solv_bicg.f90:21
+  for (int q = 0; q < nvc; ++q) {
+    KS &k = cur[q];
+    if (k.done) continue;
+    k.w4 = rr0[q];
+    k.w6 = k.w4 * k.w5 / (k.w2 * k.w3 + SMALL);
+    k.w3 = k.w4;
+  }
+}
+
+// Decide iteration it ; p = r + w6 (p - w2 u) ; Jacobi: z = c p
+__global__ __launch_bounds__(RB) void k_bi_p(int n, int it, int nvc, const double *__restrict__ r,
+                                             double *__restrict__ p, const double *__restrict__ u,
+                                             const double *__restrict__ c, int jacobi, double *__restrict__ z,
+                                             const double *__restrict__ partU, int PU, KS *ring, KS *out) {
+  double t[6];
+  load_totals<6>(partU, PU, t);
+  KS cur[3];
+  bi_decide(it, nvc, t, t + 3, ring + ((it + 1) & 1) * 3, cur);
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    for (int q = 0; q < nvc; ++q) ring[(it & 1) * 3 + q] = out[q] = cur[q];
+  int r0, r1;
+  block_rows(n, r0, r1);
+  for (int i = r0 + threadIdx.x; i < r1; i += RB)
+    for (int q = 0; q < nvc; ++q) {
+      if (cur[q].done) continue;
+      const int o = q * n + i;
+      const double pi = r[o] + cur[q].w6 * (p[o] - cur[q].w2 * u[o]);
+      p[o] = pi;
+      if (jacobi) z[o] = pi * c[i];
+    }
+}
+
+// y = A z with partials y.other (and y.y when mode = 1)
+__global__ __launch_bounds__(RB) void k_bi_spmv(DMesh M, int it, int nvc, int mode, const double *__restrict__ diag,
+                                                const double *__restrict__ spv, const double *__restrict__ z,
+                                                double *__restrict__ y, const double *__restrict__ other,
+                                                double *__restrict__ part, const KS *ring) {
+  const KS *cur = ring + (it & 1) * 3;
+  double a[6] = {0, 0, 0, 0, 0, 0};
+  if (!all_done(cur, nvc)) {
+    const int n = M.n;
+    int r0, r1;
+    block_rows(n, r0, r1);
+    for (int i = r0 + threadIdx.x; i < r1; i += RB)
+      for (int q = 0; q < nvc; ++q) {
+        if (cur[q].done) continue;
+        const double yi = row_spmv(M, diag, spv, z + q * n, i);
+        y[q * n + i] = yi;
+        a[q] += yi * other[q * n + i];
+        if (mode) a[3 + q] += yi * yi;
+      }
+  }
+  store_partials<6>(a, part);
+}
+
+// gam = beta/(u.r0 + small) ; x += gam z ; r -= gam u ; Jacobi: z = c r
+__global__ __launch_bounds__(RB) void k_bi_upd1(int n, int it, int nvc, double *__restrict__ x,
+                                                double *__restrict__ r, double *__restrict__ z,
+                                                const double *__restrict__ u, const double *__restrict__ c, int jacobi,
+                                                const double *__restrict__ partB, int PB, KS *ring) {
+  KS *cur = ring + (it & 1) * 3;
+  if (all_done(cur, nvc)) return;
+  double t[6];
+  load_totals<6>(partB, PB, t);
+  double w5[3];
+  int dn[3];
+  for (int q = 0; q < nvc; ++q) {
+    dn[q] = cur[q].done;
+    w5[q] = t[q];
+    w5[q] = cur[q].w4 / (w5[q] + SMALL);
+  }
+  // every block has read cur[] above; only w5 is written, which no block of
+  // this kernel reads
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    for (int q = 0; q < nvc; ++q)
+      if (!dn[q]) cur[q].w5 = w5[q];
+  int r0, r1;
+  block_rows(n, r0, r1);
+  for (int i = r0 + threadIdx.x; i < r1; i += RB)
+    for (int q = 0; q < nvc; ++q) {
+      if (dn[q]) continue;
+      const int o = q * n + i;
+      x[o] = x[o] + w5[q] * z[o];
+      const double ri = r[o] - w5[q] * u[o];
+      r[o] = ri;
+      if (jacobi) z[o] = ri * c[i];
+    }
+}
+
+// omega = (v.r)/(v.v + small) ; x += omega z ; r -= omega v ; partials |r|_1, r.r0
+__global__ __launch_bounds__(RB) void k_bi_upd2(int n, int it, int nvc, double *__restrict__ x,
+                                                double *__restrict__ r, const double *__restrict__ z,
+                                                const double *__restrict__ v, const double *__restrict__ r0v,
+                                                const double *__restrict__ partD, int PD, double *__restrict__ partU,
+                                                KS *ring) {
+  KS *cur = ring + (it & 1) * 3;
+  double a[6] = {0, 0, 0, 0, 0, 0};
+  if (!all_done(cur, nvc)) {
+    double t[6];
+    load_totals<6>(partD, PD, t);
+    double w1[3], w2[3];
+    int dn[3];
+    for (int q = 0; q < nvc; ++q) {
+      dn[q] = cur[q].done;
+      w1[q] = t[q];
+      w2[q] = t[3 + q];
+      w2[q] = w1[q] / (w2[q] + SMALL);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+      for (int q = 0; q < nvc; ++q)
+        if (!dn[q]) { cur[q].w1 = w1[q]; cur[q].w2 = w2[q]; }
+    int r0, r1;
+    block_rows(n, r0, r1);
+    for (int i = r0 + threadIdx.x; i < r1; i += RB)
+      for (int q = 0; q < nvc; ++q) {
+        if (dn[q]) continue;
+        const int o = q * n + i;
+        x[o] = x[o] + w2[q] * z[o];
+        const double ri = r[o] - w2[q] * v[o];
+        r[o] = ri;
+        a[q] += fabs(ri);
+        a[3 + q] += ri * r0v[o];
+      }
+  }
+  store_partials<6>(a, partU);
+}
+
+__global__ __launch_bounds__(RB) void k_bi_final(int it, int nvc, const double *partU, int PU, KS *ring, KS *out) {
+  double t[6];
+  load_totals<6>(partU, PU, t);
+  KS cur[3];
+  bi_decide(it, nvc, t, t + 3, ring + ((it + 1) & 1) * 3, cur);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < nvc; ++q) out[q] = cur[q];
+}
+
+// res0_n only (fvk_solv_res)
+__global__ __launch_bounds__(RB) void k_res_final(int nvc, const double *partU, int PU, KS *out) {
+  double t[6];
+  load_totals<6>(partU, PU, t);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < nvc; ++q) out[q].res0_n = t[q];
+}
+
+// FP64 true residual {|b-Ax|_1, |b-Ax|_2^2, |b|_1, |b|_2^2} as partials
+__global__ __launch_bounds__(RB) void k_true_res(DMesh M, const double *__restrict__ diag,
+                                                 const double *__restrict__ spv, const double *__restrict__ x,
+                                                 const double *__restrict__ b, double *__restrict__ part) {
+  double a[4] = {0, 0, 0, 0};
+  int r0, r1;
+  block_rows(M.n, r0, r1);
+  for (int i = r0 + threadIdx.x; i < r1; i += RB) {
+    const double bi = b[i], ri = bi - row_spmv(M, diag, spv, x, i);
+    a[0] += fabs(ri);
+    a[1] += ri * ri;
+    a[2] += fabs(bi);
+    a[3] += bi * bi;
+  }
+  store_partials<4>(a, part);
+}
+
+__global__ __launch_bounds__(RB) void k_totals4(const double *part, int P, double *out) {
+  double s[4];
+  load_totals<4>(part, P, s);
+  if (threadIdx.x == 0)
+    for (int j = 0; j < 4; ++j) out[j] = s[j];
 }
 
 // ------------------------------------------------------------ host side
@@ -427,33 +569,32 @@ int spmv(Ctx &c, int nvc, const double *x, double *y) {
   return 0;
 }
 
+static inline double *wv(Ctx &c, int k) { return c.wk + (size_t)k * 3 * c.n; }
+static inline double *pbuf(Ctx &c, int k) { return c.part + (size_t)k * RG * 8; }
+
 static int ks_init(Ctx &c, int nvc, double reduc, int n_iter, bool bicg) {
-  KS h[3];
+  KS h[6];
   std::memset(h, 0, sizeof h);
-  for (int q = 0; q < nvc; ++q) {
-    h[q].reduc = reduc;
-    h[q].n_iter = n_iter;
-    if (bicg) {
-      h[q].w2 = h[q].w3 = h[q].w5 = 1.0;
-    } else {
-      h[q].s = 1.0 / SMALL;  // large (precision.f90:9)
+  for (int s = 0; s < 2; ++s)
+    for (int q = 0; q < nvc; ++q) {
+      KS &k = h[s * 3 + q];
+      k.reduc = reduc;
+      k.n_iter = n_iter;
+      if (bicg) k.w2 = k.w3 = k.w5 = 1.0;
+      else k.s = 1.0 / SMALL;  // large (precision.f90:9)
     }
-    h[q].done = n_iter <= 0;
-  }
-  FVK_HIP(hipMemcpyAsync(c.ks, h, sizeof h, hipMemcpyHostToDevice, c.stream));
+  FVK_HIP(hipMemcpyAsync(c.ksr, h, sizeof h, hipMemcpyHostToDevice, c.stream));
   return 0;
 }
 
 int residual(Ctx &c, int nvc, const double *x, double *r, bool not_r0) {
-  k_residual<<<red_grid(c.n), BS, 0, c.stream>>>(c.M, nvc, c.diag, c.spv, x, r, not_r0 ? 1 : 0, c.red, c.ks);
+  const int G = red_grid(c.n);
+  k_residual<<<G, RB, 0, c.stream>>>(c.M, nvc, c.diag, c.spv, x, r, not_r0 ? 1 : 0, 0, nullptr, nullptr, pbuf(c, 0));
+  k_res_final<<<1, RB, 0, c.stream>>>(nvc, pbuf(c, 0), G, c.ks);
   FVK_HIP(hipGetLastError());
   return 0;
 }
 
-// Workspace vectors (3 components each)
-static inline double *wv(Ctx &c, int k) { return c.wk + (size_t)k * 3 * c.n; }
-
-// Host check of the done flags (after a chunk of queued iterations).
 static int all_done_host(Ctx &c, int nvc, bool &done) {
   KS h[3];
   FVK_HIP(hipMemcpyAsync(h, c.ks, sizeof(KS) * nvc, hipMemcpyDeviceToHost, c.stream));
@@ -463,68 +604,91 @@ static int all_done_host(Ctx &c, int nvc, bool &done) {
   return 0;
 }
 
-// The full PCG: r0 already holds the residual and ks[] its norms.  With
-// chunk > 0 the host stops queueing iterations once every component is done
-// (checked every chunk iterations); T (optional) times each k_cg_pq launch.
-static int cg_iterate(Ctx &c, int imat, int n_iter, int nvc, double *x, const double *r0, int chunk = 0,
-                      KTimer *T = nullptr) {
+template <int NVC>
+static void launch_cg_pq(Ctx &c, int G, int it, double *z, double *pold, double *pnew, double *q, const double *pU,
+                         const double *pR) {
+  k_cg_pq<NVC><<<G, RB, 0, c.stream>>>(c.M, it, c.diag, c.spv, z, pold, pnew, q, pU, G, pR, G, pbuf(c, 1), c.ksr,
+                                       c.ks);
+}
+template <int NVC>
+static void launch_cg_upd(Ctx &c, int G, int it, double *x, double *r, double *p, double *q, int jac, double *z) {
+  k_cg_upd<NVC><<<G, RB, 0, c.stream>>>(c.n, it, x, r, p, q, c.pc_c, jac, z, pbuf(c, 1), G, pbuf(c, 0), c.ksr);
+}
+
+// PCG: r0 holds b on entry, b - A x0 (the initial residual) on exit; x is
+// updated.  chunk > 0: stop queueing iterations once converged.
+static int cg_run(Ctx &c, int imat, int n_iter, int nvc, double *x, double *r0, bool not_r0, int chunk, KTimer *T) {
   const int n = c.n, G = red_grid(n);
   double *r = wv(c, 0), *z = wv(c, 1), *P = wv(c, 2), *q = wv(c, 4);
-  FVK_HIP(hipMemcpyAsync(r, r0, sizeof(double) * nvc * n, hipMemcpyDeviceToDevice, c.stream));
-  FVK_HIP(hipMemsetAsync(P, 0, sizeof(double) * 2 * 3 * n, c.stream));
   const int jac = imat == 0;
   if (!c.pc_valid || c.pc_imat != imat) {
     int rc = precond_build(c, imat);
     if (rc) return rc;
   }
+  FVK_HIP(hipMemsetAsync(P, 0, sizeof(double) * 2 * 3 * n, c.stream));
+  k_residual<<<G, RB, 0, c.stream>>>(c.M, nvc, c.diag, c.spv, x, r0, not_r0 ? 1 : 0, jac ? 1 : 0, c.pc_c, z,
+                                     pbuf(c, 0));
+  FVK_HIP(hipMemcpyAsync(r, r0, sizeof(double) * nvc * n, hipMemcpyDeviceToDevice, c.stream));
+  const double *pR = nullptr;
   if (!jac) {
-    int rc = precond_apply(c, imat, nvc, r, z, c.ks);
+    int rc = precond_apply(c, imat, nvc, r, z, nullptr);
     if (rc) return rc;
+    k_dot<<<G, RB, 0, c.stream>>>(n, nvc, r, z, pbuf(c, 2), nullptr);
+    pR = pbuf(c, 2);
   }
-  k_cg_rz<<<G, BS, 0, c.stream>>>(n, nvc, r, c.pc_c, jac, z, c.red, c.ks);
-  for (int it = 0; it < n_iter; ++it) {
+  int it = 0;
+  for (; it < n_iter; ++it) {
     double *pold = P + (size_t)(it & 1) * 3 * n, *pnew = P + (size_t)((it + 1) & 1) * 3 * n;
     if (T) T->begin(c.stream);
-    k_cg_pq<<<G, BS, 0, c.stream>>>(c.M, nvc, c.diag, c.spv, z, pold, pnew, q, c.red, c.ks);
+    if (nvc == 1) launch_cg_pq<1>(c, G, it, z, pold, pnew, q, pbuf(c, 0), pR);
+    else if (nvc == 2) launch_cg_pq<2>(c, G, it, z, pold, pnew, q, pbuf(c, 0), pR);
+    else launch_cg_pq<3>(c, G, it, z, pold, pnew, q, pbuf(c, 0), pR);
     if (T) T->end(c.stream);
-    k_cg_upd<<<G, BS, 0, c.stream>>>(n, nvc, x, r, pnew, q, c.pc_c, jac, z, c.red, c.ks);
-    if (!jac && it + 1 < n_iter) {
-      int rc = precond_apply(c, imat, nvc, r, z, c.ks);
+    if (nvc == 1) launch_cg_upd<1>(c, G, it, x, r, pnew, q, jac, z);
+    else if (nvc == 2) launch_cg_upd<2>(c, G, it, x, r, pnew, q, jac, z);
+    else launch_cg_upd<3>(c, G, it, x, r, pnew, q, jac, z);
+    if (!jac) {
+      int rc = precond_apply(c, imat, nvc, r, z, c.ksr + (it & 1) * 3);
       if (rc) return rc;
-      k_cg_rz<<<G, BS, 0, c.stream>>>(n, nvc, r, c.pc_c, 0, z, c.red, c.ks);
+      k_dot<<<G, RB, 0, c.stream>>>(n, nvc, r, z, pbuf(c, 2), c.ksr + (it & 1) * 3);
     }
     if (chunk > 0 && (it + 1) % chunk == 0 && it + 1 < n_iter) {
       bool done;
       if (T && T->flush()) return FVK_ERR_HIP;
       if (all_done_host(c, nvc, done)) return FVK_ERR_HIP;
-      if (done) break;
+      if (done) { ++it; break; }
     }
   }
-  if (T && T->flush()) return FVK_ERR_HIP;
+  k_cg_final<<<1, RB, 0, c.stream>>>(it, nvc, pbuf(c, 0), G, pR, G, c.ksr, c.ks);
   FVK_HIP(hipGetLastError());
+  if (T && T->flush()) return FVK_ERR_HIP;
   return 0;
 }
 
-static int bicg_iterate(Ctx &c, int imat, int n_iter, int nvc, double *x, const double *r0) {
-  const int n = c.n, G = red_grid(n), g1 = grid_for(n);
+static int bicg_run(Ctx &c, int imat, int n_iter, int nvc, double *x, double *r0, bool not_r0) {
+  const int n = c.n, G = red_grid(n);
   double *r = wv(c, 0), *p = wv(c, 1), *u = wv(c, 2), *v = wv(c, 3), *z = wv(c, 4);
-  FVK_HIP(hipMemcpyAsync(r, r0, sizeof(double) * nvc * n, hipMemcpyDeviceToDevice, c.stream));
-  FVK_HIP(hipMemsetAsync(p, 0, sizeof(double) * 4 * 3 * n, c.stream));
   const int jac = imat == 0;
   if (!c.pc_valid || c.pc_imat != imat) {
     int rc = precond_build(c, imat);
     if (rc) return rc;
   }
-  k_bi_rr0<<<G, BS, 0, c.stream>>>(n, nvc, r, r0, c.red, c.ks);
-  for (int it = 0; it < n_iter; ++it) {
-    k_bi_p<<<g1, BS, 0, c.stream>>>(n, nvc, r, p, u, c.pc_c, jac, z, c.ks);
-    if (!jac && precond_apply(c, imat, nvc, p, z, c.ks)) return FVK_ERR_HIP;
-    k_bi_spmv<<<G, BS, 0, c.stream>>>(c.M, nvc, 0, c.diag, c.spv, z, u, r0, c.red, c.ks);
-    k_bi_upd1<<<g1, BS, 0, c.stream>>>(n, nvc, x, r, z, u, c.pc_c, jac, c.ks);
-    if (!jac && precond_apply(c, imat, nvc, r, z, c.ks)) return FVK_ERR_HIP;
-    k_bi_spmv<<<G, BS, 0, c.stream>>>(c.M, nvc, 1, c.diag, c.spv, z, v, r, c.red, c.ks);
-    k_bi_upd2<<<G, BS, 0, c.stream>>>(n, nvc, x, r, z, v, r0, c.red, c.ks);
+  FVK_HIP(hipMemsetAsync(p, 0, sizeof(double) * 4 * 3 * n, c.stream));
+  k_residual<<<G, RB, 0, c.stream>>>(c.M, nvc, c.diag, c.spv, x, r0, not_r0 ? 1 : 0, 2, nullptr, nullptr,
+                                     pbuf(c, 0));
+  FVK_HIP(hipMemcpyAsync(r, r0, sizeof(double) * nvc * n, hipMemcpyDeviceToDevice, c.stream));
+  int it = 0;
+  for (; it < n_iter; ++it) {
+    const KS *gate = c.ksr + (it & 1) * 3;
+    k_bi_p<<<G, RB, 0, c.stream>>>(n, it, nvc, r, p, u, c.pc_c, jac, z, pbuf(c, 0), G, c.ksr, c.ks);
+    if (!jac && precond_apply(c, imat, nvc, p, z, gate)) return FVK_ERR_HIP;
+    k_bi_spmv<<<G, RB, 0, c.stream>>>(c.M, it, nvc, 0, c.diag, c.spv, z, u, r0, pbuf(c, 1), c.ksr);
+    k_bi_upd1<<<G, RB, 0, c.stream>>>(n, it, nvc, x, r, z, u, c.pc_c, jac, pbuf(c, 1), G, c.ksr);
+    if (!jac && precond_apply(c, imat, nvc, r, z, gate)) return FVK_ERR_HIP;
+    k_bi_spmv<<<G, RB, 0, c.stream>>>(c.M, it, nvc, 1, c.diag, c.spv, z, v, r, pbuf(c, 2), c.ksr);
+    k_bi_upd2<<<G, RB, 0, c.stream>>>(n, it, nvc, x, r, z, v, r0, pbuf(c, 2), G, pbuf(c, 0), c.ksr);
   }
+  k_bi_final<<<1, RB, 0, c.stream>>>(it, nvc, pbuf(c, 0), G, c.ksr, c.ks);
   FVK_HIP(hipGetLastError());
   return 0;
 }
@@ -533,30 +697,14 @@ int krylov(Ctx &c, int kind, int imat, double reduc, int n_iter, int nvc, double
            int chunk, KTimer *T) {
   int rc = ks_init(c, nvc, reduc, n_iter, kind == 1);
   if (rc) return rc;
-  if ((rc = residual(c, nvc, x, r0, not_r0))) return rc;
-  if (kind == 0) return cg_iterate(c, imat, n_iter, nvc, x, r0, chunk, T);
-  return bicg_iterate(c, imat, n_iter, nvc, x, r0);
-}
-
-// FP64 true residual of the current matrix: out = {|b-Ax|_1, |b-Ax|_2^2,
-// |b|_1, |b|_2^2}
-__global__ __launch_bounds__(BS) void k_true_res(DMesh M, const double *__restrict__ diag,
-                                                 const double *__restrict__ spv, const double *__restrict__ x,
-                                                 const double *__restrict__ b, Red R, double *out) {
-  double a[4] = {0, 0, 0, 0};
-  for (int i = blockIdx.x * BS + threadIdx.x; i < M.n; i += gridDim.x * BS) {
-    const double bi = b[i], ri = bi - row_spmv(M, diag, spv, x, i);
-    a[0] += fabs(ri);
-    a[1] += ri * ri;
-    a[2] += fabs(bi);
-    a[3] += bi * bi;
-  }
-  if (grid_sum<4>(a, R))
-    for (int j = 0; j < 4; ++j) out[j] = a[j];
+  if (kind == 0) return cg_run(c, imat, n_iter, nvc, x, r0, not_r0, chunk, T);
+  return bicg_run(c, imat, n_iter, nvc, x, r0, not_r0);
 }
 
 int true_residual(Ctx &c, const double *x, const double *b, double *dev_out4, double *host_out4) {
-  k_true_res<<<red_grid(c.n), BS, 0, c.stream>>>(c.M, c.diag, c.spv, x, b, c.red, dev_out4);
+  const int G = red_grid(c.n);
+  k_true_res<<<G, RB, 0, c.stream>>>(c.M, c.diag, c.spv, x, b, pbuf(c, 3));
+  k_totals4<<<1, RB, 0, c.stream>>>(pbuf(c, 3), G, dev_out4);
   FVK_HIP(hipGetLastError());
   FVK_HIP(hipMemcpyAsync(host_out4, dev_out4, 4 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   FVK_HIP(hipStreamSynchronize(c.stream));

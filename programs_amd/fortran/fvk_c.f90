@@ -25,6 +25,7 @@ module fvk_c
   type,bind(C)::fvk_outer_hist_t
     real(c_double)::reduc,redpp,redpp_sum,red_p,red_p_sum,mx_res_n
     integer(c_int)::it_u(3),it_pp,it_p
+    real(c_double)::res0_n_u(3),res0_n_pp,res0_n_p
   end type
 
   interface
@@ -57,11 +58,11 @@ module fvk_c
       real(c_double)::c(*),c_r(*)
     end function
 
-    integer(c_int) function fvk_solv_direct(ctx,imat,b,x) bind(C,name='fvk_solv_direct')
+    integer(c_int) function fvk_solv_direct(ctx,imat,c,c_r,b,x) bind(C,name='fvk_solv_direct')
       import::c_int,c_double,c_ptr
       type(c_ptr),value::ctx
       integer(c_int),value::imat
-      real(c_double),intent(in)::b(*)
+      real(c_double),intent(in)::c(*),c_r(*),b(*)
       real(c_double)::x(*)
     end function
 
@@ -104,6 +105,19 @@ module fvk_c
       type(c_ptr),value::ctx
       type(fvk_flow_cfg_t),intent(in)::cfg
       type(c_ptr)::flow
+    end function
+
+    integer(c_int) function fvk_flow_configure(flow,cfg) bind(C,name='fvk_flow_configure')
+      import::c_int,c_ptr,fvk_flow_cfg_t
+      type(c_ptr),value::flow
+      type(fvk_flow_cfg_t),intent(in)::cfg
+    end function
+
+    integer(c_int) function fvk_flow_scalars(flow,set,c_in,c_out) bind(C,name='fvk_flow_scalars')
+      import::c_int,c_double,c_ptr
+      type(c_ptr),value::flow
+      integer(c_int),value::set
+      real(c_double)::c_in,c_out
     end function
 
     integer(c_int) function fvk_flow_set_state(flow,u_cell,u_bnd,p,dpdx,spd) bind(C,name='fvk_flow_set_state')

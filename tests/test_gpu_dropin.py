@@ -1,0 +1,74 @@
+"""Fortran drop-in: the reference's own host program (grd_read / grd_conn /
+grd_geom and the golden-vector driver) with the hot-path modules solv_*_m,
+grd_solv_m and cfd_setup_m replaced by the ISO_C_BINDING modules of
+programs_amd/fortran, which call libfvk.so on the GPU.  Its dump must match
+the golden vectors the unmodified reference wrote (bars as in
+test_gpu_parity: exact where row-local, else 1e-10 or the reference's own
+summation-order sensitivity).
+
+The executable is built in the development container by
+programs_amd/fortran/build_dropin.sh (it links reference objects, so it is
+not part of the repository); the test is skipped where it is absent.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import fvo
+from tests import golden_cases as G
+from tests.golden.make_golden import write_neu
+from tests.helpers import ROOT, assert_rel
+from tests.test_gpu_parity import TOL, sens_tol, oracle_flow_fields
+
+pytestmark = pytest.mark.gpu
+
+DRIVER = os.path.join(ROOT, "oracle", "_ref", "dropin", "fvk_ref_driver")
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="drop-in executable not built here")
+@pytest.mark.parametrize("name", G.CASES)
+def test_fortran_dropin_matches_reference(name, tmp_path):
+    rec = G.load(name)
+    spec = rec["spec"]
+    src = G.mesh_inputs(rec)
+    if isinstance(src, str):
+        neu = src
+    else:
+        raw = dict(n_dx=src["n_dx"], vx=src["vx"], v_ty=src["v_ty"], v_vptr=src["v_vptr"],
+                   v_vrt=src["v_vrt"], bnd=src["bnd"], bnd_ptr=rec["mesh_bnd_ptr"])
+        neu = str(tmp_path / (name + ".neu"))
+        write_neu(neu, raw)
+    out = str(tmp_path / "dump.bin")
+    args = [DRIVER, os.path.join(G.GOLDEN, "gambit.vrt"), neu, out, str(spec["n_outer"]), repr(spec["rho"]),
+            repr(spec["mu"]), repr(spec["urf_p"])]
+    for code, u in spec["bc"]:
+        args += [str(code)] + [repr(float(v)) for v in u]
+    env = dict(os.environ)
+    if spec["variant"] == "relaxed":
+        env.update(FVK_SIMPLER="0", FVK_IMAT_P="0")
+    print("drop-in case", name, flush=True)
+    p = subprocess.run(args, capture_output=True, text=True, env=env, cwd=str(tmp_path), timeout=240)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    d = fvo.read_dump(out)
+    h = fvo.parse_history(p.stdout)
+    # mesh and gradients: exact
+    for k in ("fce_ip", "fce_in", "fce_ib", "fce_lst", "ng_lst"):
+        assert np.array_equal(d[k], rec["d_" + k]), k
+    for k in ("grad0_1", "grad0_2", "grad0_3", "grad2_1", "grad2_2", "grad2_3"):
+        assert np.array_equal(d[k], rec["d_" + k]), k
+    # flow fields and histories
+    tol = sens_tol(lambda: oracle_flow_fields(rec))
+    assert_rel(h["reduc"], rec["h_reduc"], tol, "reduc: history")
+    assert_rel(h["redpp"], rec["h_redpp"], tol, "redpp: history")
+    assert_rel(d["hist_REDUC"], rec["d_hist_REDUC"], tol, "REDUC history")
+    for k in ("u_cell", "u_bnd", "p", "dpdx", "spd", "frc_dns", "cs", "pp_pl", "pp_ngp"):
+        assert_rel(d[k], rec["d_" + k], tol, k)
+    # stand-alone solver calls made through the drop-in modules
+    for imat in (0, 1, 2):
+        assert np.array_equal(d["mom_c%d" % imat], rec["d_mom_c%d" % imat])
+        assert np.array_equal(d["mom_dir%d" % imat], rec["d_mom_dir%d" % imat])
+        assert np.array_equal(d["pp_dir%d" % imat], rec["d_pp_dir%d" % imat])
+    assert_rel(d["mom_bicgr_x0"], rec["d_mom_bicgr_x0"], TOL, "bicg reduc 0.1 jacobi")
+    assert_rel(d["pp_cgr_x0"], rec["d_pp_cgr_x0"], TOL, "cg reduc 0.1 jacobi")
