@@ -231,12 +231,21 @@ __global__ __launch_bounds__(RB) void k_cg_pq(DMesh M, int it, const double *__r
     load_totals<3>(partR, PR, tr);
     rz[0] = tr[0]; rz[1] = tr[1]; rz[2] = tr[2];
   }
-  KS cur[NVC];
-  cg_decide(it, NVC, t, rz, ring + ((it + 1) & 1) * 3, cur);
-  if (blockIdx.x == 0 && threadIdx.x == 0)
-    for (int q = 0; q < NVC; ++q) ring[(it & 1) * 3 + q] = out[q] = cur[q];
+  __shared__ KS sk[3];
+  if (threadIdx.x == 0) {
+    cg_decide(it, NVC, t, rz, ring + ((it + 1) & 1) * 3, sk);
+    if (blockIdx.x == 0)
+      for (int q = 0; q < NVC; ++q) ring[(it & 1) * 3 + q] = out[q] = sk[q];
+  }
+  __syncthreads();
+  double beta[NVC];
+  int dn[NVC];
   bool any = false;
-  for (int q = 0; q < NVC; ++q) any = any || !cur[q].done;
+  for (int q = 0; q < NVC; ++q) {
+    beta[q] = sk[q].beta;
+    dn[q] = sk[q].done;
+    any = any || !dn[q];
+  }
   if (!any) return;
   const int n = M.n;
   double pq[3] = {0.0, 0.0, 0.0};
@@ -245,15 +254,14 @@ __global__ __launch_bounds__(RB) void k_cg_pq(DMesh M, int it, const double *__r
   for (int i = r0 + threadIdx.x; i < r1; i += RB)
 #pragma unroll
     for (int q = 0; q < NVC; ++q) {
-      if (cur[q].done) continue;
-      const double beta = cur[q].beta;
+      if (dn[q]) continue;
       const double *zq = z + q * n, *po = pold + q * n;
-      const double pi = zq[i] + beta * po[i];
+      const double pi = zq[i] + beta[q] * po[i];
       double y = diag[i] * pi;
       for (int k = 0; k < M.Ws; ++k) {
         const int j = M.spc[k * n + i];
         if (j < 0) break;
-        y = y + spv[k * n + i] * (zq[j] + beta * po[j]);
+        y = y + spv[k * n + i] * (zq[j] + beta[q] * po[j]);
       }
       pnew[q * n + i] = pi;
       qv[q * n + i] = y;
@@ -329,10 +337,11 @@ __global__ __launch_bounds__(RB) void k_cg_final(int it, int nvc, const double *
     load_totals<3>(partR, PR, tr);
     rz[0] = tr[0]; rz[1] = tr[1]; rz[2] = tr[2];
   }
-  KS cur[3];
-  cg_decide(it, nvc, t, rz, ring + ((it + 1) & 1) * 3, cur);
-  if (threadIdx.x == 0)
+  if (threadIdx.x == 0) {
+    KS cur[3];
+    cg_decide(it, nvc, t, rz, ring + ((it + 1) & 1) * 3, cur);
     for (int q = 0; q < nvc; ++q) out[q] = cur[q];
+  }
 }
 
 // ---- BiCGStab (solv_bicg.f90; w_sc: res_n=1 alpha=2 beta0=3 beta=4 gam=5 omeg=6) ----
@@ -372,17 +381,27 @@ __global__ __launch_bounds__(RB) void k_bi_p(int n, int it, int nvc, const doubl
                                              const double *__restrict__ partU, int PU, KS *ring, KS *out) {
   double t[6];
   load_totals<6>(partU, PU, t);
-  KS cur[3];
-  bi_decide(it, nvc, t, t + 3, ring + ((it + 1) & 1) * 3, cur);
-  if (blockIdx.x == 0 && threadIdx.x == 0)
-    for (int q = 0; q < nvc; ++q) ring[(it & 1) * 3 + q] = out[q] = cur[q];
+  __shared__ KS sk[3];
+  if (threadIdx.x == 0) {
+    bi_decide(it, nvc, t, t + 3, ring + ((it + 1) & 1) * 3, sk);
+    if (blockIdx.x == 0)
+      for (int q = 0; q < nvc; ++q) ring[(it & 1) * 3 + q] = out[q] = sk[q];
+  }
+  __syncthreads();
+  double w6[3] = {0, 0, 0}, w2[3] = {0, 0, 0};
+  int dn[3] = {1, 1, 1};
+  for (int q = 0; q < nvc; ++q) {
+    w6[q] = sk[q].w6;
+    w2[q] = sk[q].w2;
+    dn[q] = sk[q].done;
+  }
   int r0, r1;
   block_rows(n, r0, r1);
   for (int i = r0 + threadIdx.x; i < r1; i += RB)
     for (int q = 0; q < nvc; ++q) {
-      if (cur[q].done) continue;
+      if (dn[q]) continue;
       const int o = q * n + i;
-      const double pi = r[o] + cur[q].w6 * (p[o] - cur[q].w2 * u[o]);
+      const double pi = r[o] + w6[q] * (p[o] - w2[q] * u[o]);
       p[o] = pi;
       if (jacobi) z[o] = pi * c[i];
     }
@@ -486,10 +505,11 @@ __global__ __launch_bounds__(RB) void k_bi_upd2(int n, int it, int nvc, double *
 __global__ __launch_bounds__(RB) void k_bi_final(int it, int nvc, const double *partU, int PU, KS *ring, KS *out) {
   double t[6];
   load_totals<6>(partU, PU, t);
-  KS cur[3];
-  bi_decide(it, nvc, t, t + 3, ring + ((it + 1) & 1) * 3, cur);
-  if (threadIdx.x == 0)
+  if (threadIdx.x == 0) {
+    KS cur[3];
+    bi_decide(it, nvc, t, t + 3, ring + ((it + 1) & 1) * 3, cur);
     for (int q = 0; q < nvc; ++q) out[q] = cur[q];
+  }
 }
 
 // res0_n only (fvk_solv_res)
