@@ -10,7 +10,7 @@ namespace fvk {
 constexpr int BS = 256;          // threads per block of the row kernels (4 waves)
 constexpr int WAVE = 64;
 constexpr int RB = 512;          // threads per block of the reduction kernels (8 waves)
-constexpr int RG = 512;          // at most RG blocks (partials) per reduction kernel
+constexpr int RG = 1024;         // at most RG blocks (partials) per reduction kernel
 // small = ten**(-20) as the flang build of spray-simulation/precision.f90:8
 // evaluates it (bit pattern 0x3BC79CA10C924224); large = one/small.
 constexpr double SMALL = 0x1.79ca10c924224p-67;
@@ -130,25 +130,43 @@ __device__ __forceinline__ void rblock_sum(double (&v)[NV]) {
   }
 }
 
-// This block's partials -> part[blockIdx.x*NV + j]
+// This block's partials, component-major: part[j*RG + blockIdx.x]
 template <int NV>
 __device__ __forceinline__ void store_partials(double (&v)[NV], double *part) {
   rblock_sum<NV>(v);
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int j = 0; j < NV; ++j) part[blockIdx.x * NV + j] = v[j];
+    for (int j = 0; j < NV; ++j) part[j * RG + blockIdx.x] = v[j];
 }
 
-// Totals of a previous kernel's P partials, identical in every block.
+// Totals of components comp[0..NV) of a previous kernel's P partials,
+// identical in every block (same fixed tree).
+template <int NV>
+__device__ __forceinline__ void load_sel(const double *part, int P, const int (&comp)[NV], double (&v)[NV]) {
+#pragma unroll
+  for (int j = 0; j < NV; ++j) v[j] = 0.0;
+  for (int b = threadIdx.x; b < P; b += RB)
+#pragma unroll
+    for (int j = 0; j < NV; ++j) v[j] += part[comp[j] * RG + b];
+  rblock_sum<NV>(v);
+}
+
+// Totals of components 0..NV
 template <int NV>
 __device__ __forceinline__ void load_totals(const double *part, int P, double (&v)[NV]) {
 #pragma unroll
   for (int j = 0; j < NV; ++j) v[j] = 0.0;
   for (int b = threadIdx.x; b < P; b += RB)
 #pragma unroll
-    for (int j = 0; j < NV; ++j) v[j] += part[b * NV + j];
+    for (int j = 0; j < NV; ++j) v[j] += part[j * RG + b];
   rblock_sum<NV>(v);
 }
+
+// Interior slots of a row: WS > 0 is the table width fixed at compile time
+// (loops unroll and every slot's loads issue together), 0 reads it from the
+// mesh.  Padding slots (column -1) sit at the end of a row and add nothing.
+template <int WS>
+__device__ __forceinline__ int nslots(const DMesh &M) { return WS > 0 ? WS : M.Ws; }
 
 // Grid of the reduction kernels: at most RG blocks of RB threads, a multiple
 // of 8 once there are enough rows (XCD-contiguous row chunks).

@@ -123,30 +123,34 @@ __global__ __launch_bounds__(BS) void k_sweep_bwd(DMesh M, const int *__restrict
 }
 
 // y = diag*x + sum over the row's interior faces (solv_cg.f90:56-64)
+template <int WS>
 __device__ __forceinline__ double row_spmv(const DMesh &M, const double *__restrict__ diag,
                                            const double *__restrict__ spv, const double *__restrict__ x,
                                            int i) {
   const int n = M.n;
   double y = diag[i] * x[i];
-  for (int k = 0; k < M.Ws; ++k) {
+#pragma unroll
+  for (int k = 0; k < nslots<WS>(M); ++k) {
     const int j = M.spc[k * n + i];
-    if (j < 0) break;
-    y = y + spv[k * n + i] * x[j];
+    const double a = spv[k * n + i];
+    if (j >= 0) y = y + a * x[j];
   }
   return y;
 }
 
+template <int WS>
 __global__ __launch_bounds__(BS) void k_spmv(DMesh M, int nvc, const double *__restrict__ diag,
                                              const double *__restrict__ spv, const double *__restrict__ x,
                                              double *__restrict__ y) {
   int i = blockIdx.x * BS + threadIdx.x;
   if (i >= M.n) return;
-  for (int q = 0; q < nvc; ++q) y[q * M.n + i] = row_spmv(M, diag, spv, x + q * M.n, i);
+  for (int q = 0; q < nvc; ++q) y[q * M.n + i] = row_spmv<WS>(M, diag, spv, x + q * M.n, i);
 }
 
 // r = b - A x (or r = b) with partials {|r|_1[q], s2[q]} where s2 = r.(c r)
 // and z = c r (mode 1, Jacobi PCG), s2 = r.r (mode 2, BiCGStab) or 0.
 // (solv_res.f90; solv_cg.f90:19-33; solv_bicg.f90:19-38)
+template <int WS>
 __global__ __launch_bounds__(RB) void k_residual(DMesh M, int nvc, const double *__restrict__ diag,
                                                  const double *__restrict__ spv, const double *__restrict__ x,
                                                  double *__restrict__ r, int not_r0, int mode,
@@ -162,10 +166,11 @@ __global__ __launch_bounds__(RB) void k_residual(DMesh M, int nvc, const double 
       if (!not_r0) {
         const double *xq = x + q * n;
         v = v - diag[i] * xq[i];
-        for (int k = 0; k < M.Ws; ++k) {
+#pragma unroll
+        for (int k = 0; k < nslots<WS>(M); ++k) {
           const int j = M.spc[k * n + i];
-          if (j < 0) break;
-          v = v - spv[k * n + i] * xq[j];
+          const double a = spv[k * n + i];
+          if (j >= 0) v = v - a * xq[j];
         }
         r[q * n + i] = v;
       }
@@ -216,24 +221,49 @@ __device__ __forceinline__ void cg_decide(int it, int nvc, const double *L1, con
 
 // Decide iteration it, then p = z + beta p ; q = A p ; partial p.q.  p of the
 // neighbours is formed on the fly from (z, p_old): p_new is written once.
-template <int NVC>
+// Each thread takes two rows per pass so that both rows' loads are in flight
+// together.
+template <int NVC, int WS>
+__device__ __forceinline__ void cg_pq_row(const DMesh &M, int i, const double *__restrict__ diag,
+                                          const double *__restrict__ spv, const double *__restrict__ zq,
+                                          const double *__restrict__ po, double beta, double *__restrict__ pn,
+                                          double *__restrict__ qq, double &pq) {
+  const int n = M.n;
+  const double pi = zq[i] + beta * po[i];
+  double y = diag[i] * pi;
+#pragma unroll
+  for (int k = 0; k < nslots<WS>(M); ++k) {
+    const int j = M.spc[k * n + i];
+    const double a = spv[k * n + i];
+    if (j >= 0) y = y + a * (zq[j] + beta * po[j]);
+  }
+  pn[i] = pi;
+  qq[i] = y;
+  pq += pi * y;
+}
+
+template <int NVC, int WS>
 __global__ __launch_bounds__(RB) void k_cg_pq(DMesh M, int it, const double *__restrict__ diag,
                                               const double *__restrict__ spv, const double *__restrict__ z,
                                               const double *__restrict__ pold, double *__restrict__ pnew,
                                               double *__restrict__ qv, const double *__restrict__ partU, int PU,
                                               const double *__restrict__ partR, int PR, double *__restrict__ partQ,
                                               KS *ring, KS *out) {
-  double t[6];
-  load_totals<6>(partU, PU, t);
-  double rz[3] = {t[3], t[4], t[5]};
+  // |r|_1 (components 0..NVC) and, for Jacobi, r.z (components 3..3+NVC)
+  double L1[NVC], rz[NVC];
   if (partR) {
-    double tr[3];
-    load_totals<3>(partR, PR, tr);
-    rz[0] = tr[0]; rz[1] = tr[1]; rz[2] = tr[2];
+    load_totals<NVC>(partU, PU, L1);
+    load_totals<NVC>(partR, PR, rz);
+  } else {
+    int cs[2 * NVC];
+    double t[2 * NVC];
+    for (int q = 0; q < NVC; ++q) { cs[q] = q; cs[NVC + q] = 3 + q; }
+    load_sel<2 * NVC>(partU, PU, cs, t);
+    for (int q = 0; q < NVC; ++q) { L1[q] = t[q]; rz[q] = t[NVC + q]; }
   }
   __shared__ KS sk[3];
   if (threadIdx.x == 0) {
-    cg_decide(it, NVC, t, rz, ring + ((it + 1) & 1) * 3, sk);
+    cg_decide(it, NVC, L1, rz, ring + ((it + 1) & 1) * 3, sk);
     if (blockIdx.x == 0)
       for (int q = 0; q < NVC; ++q) ring[(it & 1) * 3 + q] = out[q] = sk[q];
   }
@@ -248,26 +278,27 @@ __global__ __launch_bounds__(RB) void k_cg_pq(DMesh M, int it, const double *__r
   }
   if (!any) return;
   const int n = M.n;
-  double pq[3] = {0.0, 0.0, 0.0};
+  double pq[NVC];
+  for (int q = 0; q < NVC; ++q) pq[q] = 0.0;
   int r0, r1;
   block_rows(n, r0, r1);
-  for (int i = r0 + threadIdx.x; i < r1; i += RB)
+  int i = r0 + threadIdx.x;
+  for (; i + RB < r1; i += 2 * RB)
 #pragma unroll
     for (int q = 0; q < NVC; ++q) {
       if (dn[q]) continue;
       const double *zq = z + q * n, *po = pold + q * n;
-      const double pi = zq[i] + beta[q] * po[i];
-      double y = diag[i] * pi;
-      for (int k = 0; k < M.Ws; ++k) {
-        const int j = M.spc[k * n + i];
-        if (j < 0) break;
-        y = y + spv[k * n + i] * (zq[j] + beta[q] * po[j]);
-      }
-      pnew[q * n + i] = pi;
-      qv[q * n + i] = y;
-      pq[q] += pi * y;
+      double *pn = pnew + q * n, *qq = qv + q * n;
+      cg_pq_row<NVC, WS>(M, i, diag, spv, zq, po, beta[q], pn, qq, pq[q]);
+      cg_pq_row<NVC, WS>(M, i + RB, diag, spv, zq, po, beta[q], pn, qq, pq[q]);
     }
-  store_partials<3>(pq, partQ);
+  if (i < r1)
+#pragma unroll
+    for (int q = 0; q < NVC; ++q) {
+      if (dn[q]) continue;
+      cg_pq_row<NVC, WS>(M, i, diag, spv, z + q * n, pold + q * n, beta[q], pnew + q * n, qv + q * n, pq[q]);
+    }
+  store_partials<NVC>(pq, partQ);
 }
 
 // alpha = s/(p.q + small) ; x += alpha p ; r -= alpha q ; partials |r|_1 and
@@ -282,8 +313,8 @@ __global__ __launch_bounds__(RB) void k_cg_upd(int n, int it, double *__restrict
   bool any = false;
   for (int q = 0; q < NVC; ++q) any = any || !cur[q].done;
   if (!any) return;
-  double t[3];
-  load_totals<3>(partQ, PQ, t);
+  double t[NVC];
+  load_totals<NVC>(partQ, PQ, t);
   double alpha[NVC];
   int dn[NVC];
   for (int q = 0; q < NVC; ++q) {
@@ -408,6 +439,7 @@ __global__ __launch_bounds__(RB) void k_bi_p(int n, int it, int nvc, const doubl
 }
 
 // y = A z with partials y.other (and y.y when mode = 1)
+template <int WS>
 __global__ __launch_bounds__(RB) void k_bi_spmv(DMesh M, int it, int nvc, int mode, const double *__restrict__ diag,
                                                 const double *__restrict__ spv, const double *__restrict__ z,
                                                 double *__restrict__ y, const double *__restrict__ other,
@@ -421,7 +453,7 @@ __global__ __launch_bounds__(RB) void k_bi_spmv(DMesh M, int it, int nvc, int mo
     for (int i = r0 + threadIdx.x; i < r1; i += RB)
       for (int q = 0; q < nvc; ++q) {
         if (cur[q].done) continue;
-        const double yi = row_spmv(M, diag, spv, z + q * n, i);
+        const double yi = row_spmv<WS>(M, diag, spv, z + q * n, i);
         y[q * n + i] = yi;
         a[q] += yi * other[q * n + i];
         if (mode) a[3 + q] += yi * yi;
@@ -521,6 +553,7 @@ __global__ __launch_bounds__(RB) void k_res_final(int nvc, const double *partU, 
 }
 
 // FP64 true residual {|b-Ax|_1, |b-Ax|_2^2, |b|_1, |b|_2^2} as partials
+template <int WS>
 __global__ __launch_bounds__(RB) void k_true_res(DMesh M, const double *__restrict__ diag,
                                                  const double *__restrict__ spv, const double *__restrict__ x,
                                                  const double *__restrict__ b, double *__restrict__ part) {
@@ -528,7 +561,7 @@ __global__ __launch_bounds__(RB) void k_true_res(DMesh M, const double *__restri
   int r0, r1;
   block_rows(M.n, r0, r1);
   for (int i = r0 + threadIdx.x; i < r1; i += RB) {
-    const double bi = b[i], ri = bi - row_spmv(M, diag, spv, x, i);
+    const double bi = b[i], ri = bi - row_spmv<WS>(M, diag, spv, x, i);
     a[0] += fabs(ri);
     a[1] += ri * ri;
     a[2] += fabs(bi);
@@ -583,8 +616,20 @@ int precond_apply(Ctx &c, int imat, int nvc, const double *b, double *x, const K
   return 0;
 }
 
+// Launch F(WS) with the slot count of the mesh's table as a template
+// argument where an instantiation exists.
+#define FVK_WS(ws, F) \
+  switch (ws) {       \
+    case 3: F(3); break; \
+    case 4: F(4); break; \
+    case 6: F(6); break; \
+    default: F(0); break; \
+  }
+
 int spmv(Ctx &c, int nvc, const double *x, double *y) {
-  k_spmv<<<grid_for(c.n), BS, 0, c.stream>>>(c.M, nvc, c.diag, c.spv, x, y);
+#define L(W) k_spmv<W><<<grid_for(c.n), BS, 0, c.stream>>>(c.M, nvc, c.diag, c.spv, x, y)
+  FVK_WS(c.M.Ws, L);
+#undef L
   FVK_HIP(hipGetLastError());
   return 0;
 }
@@ -609,7 +654,10 @@ static int ks_init(Ctx &c, int nvc, double reduc, int n_iter, bool bicg) {
 
 int residual(Ctx &c, int nvc, const double *x, double *r, bool not_r0) {
   const int G = red_grid(c.n);
-  k_residual<<<G, RB, 0, c.stream>>>(c.M, nvc, c.diag, c.spv, x, r, not_r0 ? 1 : 0, 0, nullptr, nullptr, pbuf(c, 0));
+#define L(W) \
+  k_residual<W><<<G, RB, 0, c.stream>>>(c.M, nvc, c.diag, c.spv, x, r, not_r0 ? 1 : 0, 0, nullptr, nullptr, pbuf(c, 0))
+  FVK_WS(c.M.Ws, L);
+#undef L
   k_res_final<<<1, RB, 0, c.stream>>>(nvc, pbuf(c, 0), G, c.ks);
   FVK_HIP(hipGetLastError());
   return 0;
@@ -627,8 +675,11 @@ static int all_done_host(Ctx &c, int nvc, bool &done) {
 template <int NVC>
 static void launch_cg_pq(Ctx &c, int G, int it, double *z, double *pold, double *pnew, double *q, const double *pU,
                          const double *pR) {
-  k_cg_pq<NVC><<<G, RB, 0, c.stream>>>(c.M, it, c.diag, c.spv, z, pold, pnew, q, pU, G, pR, G, pbuf(c, 1), c.ksr,
-                                       c.ks);
+#define L(W)                                                                                                    \
+  k_cg_pq<NVC, W><<<G, RB, 0, c.stream>>>(c.M, it, c.diag, c.spv, z, pold, pnew, q, pU, G, pR, G, pbuf(c, 1), \
+                                          c.ksr, c.ks)
+  FVK_WS(c.M.Ws, L);
+#undef L
 }
 template <int NVC>
 static void launch_cg_upd(Ctx &c, int G, int it, double *x, double *r, double *p, double *q, int jac, double *z) {
@@ -646,8 +697,11 @@ static int cg_run(Ctx &c, int imat, int n_iter, int nvc, double *x, double *r0, 
     if (rc) return rc;
   }
   FVK_HIP(hipMemsetAsync(P, 0, sizeof(double) * 2 * 3 * n, c.stream));
-  k_residual<<<G, RB, 0, c.stream>>>(c.M, nvc, c.diag, c.spv, x, r0, not_r0 ? 1 : 0, jac ? 1 : 0, c.pc_c, z,
-                                     pbuf(c, 0));
+#define L(W)                                                                                                \
+  k_residual<W><<<G, RB, 0, c.stream>>>(c.M, nvc, c.diag, c.spv, x, r0, not_r0 ? 1 : 0, jac ? 1 : 0, c.pc_c, z, \
+                                        pbuf(c, 0))
+  FVK_WS(c.M.Ws, L);
+#undef L
   FVK_HIP(hipMemcpyAsync(r, r0, sizeof(double) * nvc * n, hipMemcpyDeviceToDevice, c.stream));
   const double *pR = nullptr;
   if (!jac) {
@@ -694,18 +748,25 @@ static int bicg_run(Ctx &c, int imat, int n_iter, int nvc, double *x, double *r0
     if (rc) return rc;
   }
   FVK_HIP(hipMemsetAsync(p, 0, sizeof(double) * 4 * 3 * n, c.stream));
-  k_residual<<<G, RB, 0, c.stream>>>(c.M, nvc, c.diag, c.spv, x, r0, not_r0 ? 1 : 0, 2, nullptr, nullptr,
-                                     pbuf(c, 0));
+#define L(W)                                                                                              \
+  k_residual<W><<<G, RB, 0, c.stream>>>(c.M, nvc, c.diag, c.spv, x, r0, not_r0 ? 1 : 0, 2, nullptr, nullptr, \
+                                        pbuf(c, 0))
+  FVK_WS(c.M.Ws, L);
+#undef L
   FVK_HIP(hipMemcpyAsync(r, r0, sizeof(double) * nvc * n, hipMemcpyDeviceToDevice, c.stream));
   int it = 0;
   for (; it < n_iter; ++it) {
     const KS *gate = c.ksr + (it & 1) * 3;
     k_bi_p<<<G, RB, 0, c.stream>>>(n, it, nvc, r, p, u, c.pc_c, jac, z, pbuf(c, 0), G, c.ksr, c.ks);
     if (!jac && precond_apply(c, imat, nvc, p, z, gate)) return FVK_ERR_HIP;
-    k_bi_spmv<<<G, RB, 0, c.stream>>>(c.M, it, nvc, 0, c.diag, c.spv, z, u, r0, pbuf(c, 1), c.ksr);
+#define L(W) k_bi_spmv<W><<<G, RB, 0, c.stream>>>(c.M, it, nvc, 0, c.diag, c.spv, z, u, r0, pbuf(c, 1), c.ksr)
+    FVK_WS(c.M.Ws, L);
+#undef L
     k_bi_upd1<<<G, RB, 0, c.stream>>>(n, it, nvc, x, r, z, u, c.pc_c, jac, pbuf(c, 1), G, c.ksr);
     if (!jac && precond_apply(c, imat, nvc, r, z, gate)) return FVK_ERR_HIP;
-    k_bi_spmv<<<G, RB, 0, c.stream>>>(c.M, it, nvc, 1, c.diag, c.spv, z, v, r, pbuf(c, 2), c.ksr);
+#define L(W) k_bi_spmv<W><<<G, RB, 0, c.stream>>>(c.M, it, nvc, 1, c.diag, c.spv, z, v, r, pbuf(c, 2), c.ksr)
+    FVK_WS(c.M.Ws, L);
+#undef L
     k_bi_upd2<<<G, RB, 0, c.stream>>>(n, it, nvc, x, r, z, v, r0, pbuf(c, 2), G, pbuf(c, 0), c.ksr);
   }
   k_bi_final<<<1, RB, 0, c.stream>>>(it, nvc, pbuf(c, 0), G, c.ksr, c.ks);
@@ -723,7 +784,9 @@ int krylov(Ctx &c, int kind, int imat, double reduc, int n_iter, int nvc, double
 
 int true_residual(Ctx &c, const double *x, const double *b, double *dev_out4, double *host_out4) {
   const int G = red_grid(c.n);
-  k_true_res<<<G, RB, 0, c.stream>>>(c.M, c.diag, c.spv, x, b, pbuf(c, 3));
+#define L(W) k_true_res<W><<<G, RB, 0, c.stream>>>(c.M, c.diag, c.spv, x, b, pbuf(c, 3))
+  FVK_WS(c.M.Ws, L);
+#undef L
   k_totals4<<<1, RB, 0, c.stream>>>(pbuf(c, 3), G, dev_out4);
   FVK_HIP(hipGetLastError());
   FVK_HIP(hipMemcpyAsync(host_out4, dev_out4, 4 * sizeof(double), hipMemcpyDeviceToHost, c.stream));

@@ -20,7 +20,7 @@ from oracle import fvo
 from tests import golden_cases as G
 from tests.golden.make_golden import write_neu
 from tests.helpers import ROOT, assert_rel
-from tests.test_gpu_parity import TOL, sens_tol, oracle_flow_fields
+from tests.test_gpu_parity import b3_for, sens_tol, oracle_flow_fields
 
 pytestmark = pytest.mark.gpu
 
@@ -66,11 +66,31 @@ def test_fortran_dropin_matches_reference(name, tmp_path):
     for k in ("u_cell", "u_bnd", "p", "dpdx", "spd", "frc_dns", "cs", "pp_pl", "pp_ngp"):
         assert_rel(d[k], rec["d_" + k], tol, k)
     # stand-alone solver calls made through the drop-in modules, on the
-    # matrices the drop-in assembled itself after n_outer iterations: they
-    # carry the fields' rounding differences, so the same bar applies
-    for imat in (0, 1, 2):
-        assert_rel(d["mom_c%d" % imat], rec["d_mom_c%d" % imat], tol, "mom_c%d" % imat)
-        assert_rel(d["mom_dir%d" % imat], rec["d_mom_dir%d" % imat], tol, "mom_dir%d" % imat)
-        assert_rel(d["pp_dir%d" % imat], rec["d_pp_dir%d" % imat], tol, "pp_dir%d" % imat)
-    assert_rel(d["mom_bicgr_x0"], rec["d_mom_bicgr_x0"], tol, "bicg reduc 0.1 jacobi")
-    assert_rel(d["pp_cgr_x0"], rec["d_pp_cgr_x0"], tol, "cg reduc 0.1 jacobi")
+    # matrices the drop-in assembled itself after n_outer iterations: those
+    # match the reference's to the fields' bar, and solv_precond /
+    # solv_direct on them (row-local) equal the oracle's on the same matrix
+    # bit for bit; the short solves match the oracle's to the bar
+    for k in ("mom_pl", "mom_ngp", "mom_ngn"):
+        assert_rel(d[k], rec["d_" + k], tol, k)
+    om = G.oracle_mesh(rec)
+    try:
+        nv, nd = om.m.n_vol, om.m.n_dx
+        b3 = b3_for(nv, nd)
+        A = fvo.OSys(om, d["mom_pl"], d["mom_ngp"], d["mom_ngn"])
+        P = fvo.OSys(om, d["pp_pl"], d["pp_ngp"], d["pp_ngp"])
+        bb = b3[:, 0] - np.cumsum(b3[:, 0])[-1] / nv
+        for imat in (0, 1, 2):
+            c, _ = A.precond(imat)
+            assert np.array_equal(d["mom_c%d" % imat], c), ("mom precond", imat)
+            assert np.array_equal(d["mom_dir%d" % imat], A.direct(imat, b3[:, 0])), ("mom direct", imat)
+            c, _ = P.precond(imat)
+            assert np.array_equal(d["pp_c%d" % imat], c), ("pp precond", imat)
+            assert np.array_equal(d["pp_dir%d" % imat], P.direct(imat, bb)), ("pp direct", imat)
+        tb = sens_tol(lambda: A.solve("bicg", 0, 0.1, 5, b3)[0])
+        xb = A.solve("bicg", 0, 0.1, 5, b3)[0]
+        assert_rel(d["mom_bicgr_x0"], xb.ravel(), tb, "bicg reduc 0.1 jacobi")
+        tc = sens_tol(lambda: P.solve("cg", 0, 0.1, 20, bb, not_r0=True)[0])
+        xc = P.solve("cg", 0, 0.1, 20, bb, not_r0=True)[0]
+        assert_rel(d["pp_cgr_x0"], xc.ravel(), tc, "cg reduc 0.1 jacobi")
+    finally:
+        om.free()
