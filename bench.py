@@ -68,6 +68,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--cpu-sample", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--methods", default="dc32", help="extra methods timed beside the plain solve")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is "
@@ -83,19 +84,32 @@ def main():
     fvk.sys_random(ctx, 0x5EED0003, args.sigma)
     log("context + system ready (%.1f s)" % (time.time() - t0))
 
-    for _ in range(args.warmup):
-        r = fvk.sys_solve(ctx, fvk.SOLVE_PCG64, 0, args.tol, 200000)
-        log("warmup solve: %d iterations, %.3f s device" % (r["iters"], r["seconds"]))
-    ctx.sync()
-    t0 = time.perf_counter()
-    res = []
-    for _ in range(args.steps):
-        res.append(fvk.sys_solve(ctx, fvk.SOLVE_PCG64, 0, args.tol, 200000))
-    ctx.sync()
-    wall = (time.perf_counter() - t0) / args.steps
-    r = res[-1]
-    log("timed: %.4f s per solve (wall), %.4f s device, %d iterations, true L1 %.2e L2 %.2e"
-        % (wall, r["seconds"], r["iters"], r["res_true_l1"], r["res_true_l2"]))
+    def timed(method, name):
+        for _ in range(args.warmup):
+            r = fvk.sys_solve(ctx, method, 0, args.tol, 200000)
+            log("%s warmup solve: %d iterations, %.3f s device" % (name, r["iters"], r["seconds"]))
+        ctx.sync()
+        t0 = time.perf_counter()
+        res = []
+        for _ in range(args.steps):
+            res.append(fvk.sys_solve(ctx, method, 0, args.tol, 200000))
+        ctx.sync()
+        wall = (time.perf_counter() - t0) / args.steps
+        r = res[-1]
+        log("%s timed: %.4f s per solve (wall), %.4f s device, %d iterations, true L1 %.2e L2 %.2e"
+            % (name, wall, r["seconds"], r["iters"], r["res_true_l1"], r["res_true_l2"]))
+        return wall, r
+
+    wall, r = timed(fvk.SOLVE_PCG64, "plain_pcg64_jacobi")
+    methods = {"plain_pcg64_jacobi": dict(s=wall, iters=r["iters"], res_true_l1=r["res_true_l1"],
+                                          res_true_l2=r["res_true_l2"])}
+    if "dc32" in args.methods.split(","):
+        w2, r2 = timed(fvk.SOLVE_DC32, "dc32_jacobi")
+        rk2 = fvk.sys_solve(ctx, fvk.SOLVE_DC32, 0, args.tol, 200000, time_kernel=True)
+        methods["dc32_jacobi"] = dict(s=w2, iters=r2["iters"], res_true_l1=r2["res_true_l1"],
+                                      res_true_l2=r2["res_true_l2"], speedup_vs_plain=wall / w2,
+                                      k_mx_pq_us=rk2["kern_seconds"] * 1e6,
+                                      k_mx_pq_gbs=rk2["kern_bytes"] / rk2["kern_seconds"] / 1e9)
 
     # dominant kernel, timed live per launch with HIP events on the solve stream
     rk = fvk.sys_solve(ctx, fvk.SOLVE_PCG64, 0, args.tol, 200000, time_kernel=True)
@@ -142,6 +156,7 @@ def main():
         "spmv": dict(us_per_launch=t_spmv * 1e6, bytes=b_spmv, gbs=b_spmv / t_spmv / 1e9,
                      frac=b_spmv / t_spmv / PEAK_HBM),
         "copy_gbs": 2 * copy_bytes / t_copy / 1e9,
+        "methods": methods,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ctx, fvk, n, args.jitter, r["iters"], args.cpu_sample)

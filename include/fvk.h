@@ -166,18 +166,38 @@ typedef struct fvk_solve_info {
 } fvk_solve_info;
 
 #define FVK_SOLVE_PCG64 0 /* the reference PCG recurrence in FP64 (solv_cg.f90) */
+#define FVK_SOLVE_DC32 1  /* FP32-inner Jacobi PCG in FP64 defect correction */
+#define FVK_SOLVE_DC32_FACTOR 2 /* same, preconditioned by z = G G^T r with the
+                                   factor of fvk_factor_set / fvk_factor_build;
+                                   falls back to FVK_SOLVE_PCG64 (Jacobi, from
+                                   x = 0) when it does not converge within
+                                   max_iter or breaks down: FVK_FALLBACK */
 
 /* a_f = s/l_pn * exp(sigma*z_f), z_f ~ N(0,1) from SplitMix64(seed) by face
    index; b ~ N(0,1) by cell, projected to zero mean.  b_out (reference
    order, may be NULL) receives the RHS. */
 int fvk_sys_random(fvk_ctx *ctx, uint64_t seed, double sigma, double *b_out);
 /* Symmetric system from host arrays: pl, a (= -ng%p = -ng%n per face), b. */
-int fvk_sys_set(fvk_ctx *ctx, const double *pl, const double *ng, const double *b);
+int fvk_sys_set(fvk_ctx *ctx, const double *pl, const double *a, const double *b);
 /* Solve from x = 0 until the method's relative L1 residual is below tol (or
    max_iter).  imat selects the plain preconditioner (0 Jacobi, 1 DIC). */
 int fvk_sys_solve(fvk_ctx *ctx, int method, int imat, double tol, int max_iter, int time_kernel,
                   fvk_solve_info *info);
 int fvk_sys_get_x(fvk_ctx *ctx, double *x);
+/* Solver settings of a context: "dc_inner_reduc" (relative L1 reduction of
+   each FP32 inner solve, default 1e-4), "dc_max_outer" (defect-correction
+   steps, default 30). */
+int fvk_ctx_set_param(fvk_ctx *ctx, const char *name, double value);
+
+/* The factorised preconditioner z = G G^T r, G = D^-1/2 (I + L) Delta, with
+   D the diagonal of the current matrix, L strictly lower triangular on the
+   matrix's pattern in the reference numbering (L[in, ip] = l_f for interior
+   face f) and Delta_c = exp(sum of sig_f over the faces c owns).  No
+   counterpart in the reference.  l, sig: n_fce entries in reference face
+   order (boundary faces ignored). */
+int fvk_factor_set(fvk_ctx *ctx, const float *l, const float *sig);
+/* z = G G^T r (FP32 inside; host arrays in reference order). */
+int fvk_factor_apply(fvk_ctx *ctx, const double *r, double *z);
 /* The loaded system in reference order: pl (n_vol), a (n_fce, 0 on boundary
    faces), b (n_vol); NULL pointers skip. */
 int fvk_sys_get(fvk_ctx *ctx, double *pl, double *a, double *b);

@@ -654,11 +654,13 @@ static double normal_at(uint64_t seed, uint64_t k) {
   return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
 }
 
-int fvk_sys_set(fvk_ctx *cc, const double *pl, const double *ng, const double *b) {
-  if (!cc || !pl || !ng || !b) { set_error("null argument"); return FVK_ERR_ARG; }
+int fvk_sys_set(fvk_ctx *cc, const double *pl, const double *a, const double *b) {
+  if (!cc || !pl || !a || !b) { set_error("null argument"); return FVK_ERR_ARG; }
   Ctx &c = cc->c;
   GUARD(sys_alloc(c));
-  GUARD(fvk_set_matrix(cc, pl, ng, ng));
+  std::vector<double> ng(c.nf);
+  for (int f = 0; f < c.nf; ++f) ng[f] = -a[f];
+  GUARD(fvk_set_matrix(cc, pl, ng.data(), ng.data()));
   GUARD(to_dev(c, b, 1, 1, c.n, &c.perm, c.sys_b));
   return FVK_OK;
 }
@@ -670,12 +672,12 @@ int fvk_sys_random(fvk_ctx *cc, uint64_t seed, double sigma, double *b_out) {
   std::vector<double> s(c.nf), l(c.nf);
   FVK_HIP(hipMemcpy(s.data(), c.M.fs, sizeof(double) * c.nf, hipMemcpyDeviceToHost));
   FVK_HIP(hipMemcpy(l.data(), c.M.flpn, sizeof(double) * c.nf, hipMemcpyDeviceToHost));
-  std::vector<double> ng(c.nf, 0.0), pl(c.n, 0.0), b(c.n);
+  std::vector<double> af(c.nf, 0.0), pl(c.n, 0.0), b(c.n);
   std::vector<int> orig_ip(c.nf), orig_in(c.nf);
   for (int f = 0; f < c.nf; ++f) {
     if (c.h_fin[f] < 0) continue;
     const double a = s[f] / l[f] * std::exp(sigma * normal_at(seed, (uint64_t)f));
-    ng[f] = -a;
+    af[f] = a;
     pl[c.perm[c.h_fip[f]]] += a;
     pl[c.perm[c.h_fin[f]]] += a;
   }
@@ -684,7 +686,7 @@ int fvk_sys_random(fvk_ctx *cc, uint64_t seed, double sigma, double *b_out) {
   mean /= c.n;
   for (int i = 0; i < c.n; ++i) b[i] -= mean;
   if (b_out) std::memcpy(b_out, b.data(), sizeof(double) * c.n);
-  return fvk_sys_set(cc, pl.data(), ng.data(), b.data());
+  return fvk_sys_set(cc, pl.data(), af.data(), b.data());
 }
 
 int fvk_sys_get_x(fvk_ctx *cc, double *x) {
@@ -705,45 +707,124 @@ int fvk_sys_get(fvk_ctx *cc, double *pl, double *a, double *b) {
   return FVK_OK;
 }
 
+static long nnz_offdiag(const Ctx &c) {
+  long k = 0;
+  for (int f = 0; f < c.nf; ++f) k += c.h_fin[f] >= 0 ? 2 : 0;
+  return k;
+}
+
+static int solve_pcg64(Ctx &c, int imat, double tol, int max_iter, KTimer *T, KS &h) {
+  const int n = c.n;
+  double *r0 = c.wk + (size_t)6 * 3 * n;
+  FVK_HIP(hipMemsetAsync(c.sys_x, 0, sizeof(double) * n, c.stream));
+  FVK_HIP(hipMemcpyAsync(r0, c.sys_b, sizeof(double) * n, hipMemcpyDeviceToDevice, c.stream));
+  GUARD(krylov(c, 0, imat, tol, max_iter, 1, c.sys_x, r0, true, 64, T));
+  FVK_HIP(hipMemcpyAsync(&h, c.ks, sizeof h, hipMemcpyDeviceToHost, c.stream));
+  FVK_HIP(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
 int fvk_sys_solve(fvk_ctx *cc, int method, int imat, double tol, int max_iter, int time_kernel,
                   fvk_solve_info *info) {
-  if (!cc || !info || imat < 0 || imat > 2 || max_iter < 1 || method != FVK_SOLVE_PCG64) {
+  if (!cc || !info || imat < 0 || imat > 2 || max_iter < 1 || method < FVK_SOLVE_PCG64 ||
+      method > FVK_SOLVE_DC32_FACTOR || (method != FVK_SOLVE_PCG64 && imat != 0)) {
     set_error("bad argument");
     return FVK_ERR_ARG;
   }
   Ctx &c = cc->c;
   if (!c.sys_b) { set_error("no system loaded"); return FVK_ERR_STATE; }
+  if (method == FVK_SOLVE_DC32_FACTOR && !c.mx.fac_valid) {
+    set_error("no preconditioner factor: call fvk_factor_set or fvk_factor_build first");
+    return FVK_ERR_STATE;
+  }
   std::memset(info, 0, sizeof *info);
   const int n = c.n;
-  double *r0 = c.wk + (size_t)6 * 3 * n;
-  FVK_HIP(hipMemsetAsync(c.sys_x, 0, sizeof(double) * n, c.stream));
-  FVK_HIP(hipMemcpyAsync(r0, c.sys_b, sizeof(double) * n, hipMemcpyDeviceToDevice, c.stream));
   KTimer T;
   if (time_kernel) T.init(64);
   FVK_HIP(hipEventRecord(c.ev0, c.stream));
-  GUARD(krylov(c, 0, imat, tol, max_iter, 1, c.sys_x, r0, true, 64, time_kernel ? &T : nullptr));
+  int status = FVK_OK;
+  KS h;
+  std::memset(&h, 0, sizeof h);
+  if (method == FVK_SOLVE_PCG64) {
+    GUARD(solve_pcg64(c, imat, tol, max_iter, time_kernel ? &T : nullptr, h));
+    info->iters = h.iter;
+    info->res_rec = h.res_n;
+    status = h.done && h.res_n < tol ? FVK_OK : FVK_NOT_CONVERGED;
+  } else {
+    DcResult R;
+    GUARD(dc_solve(c, method == FVK_SOLVE_DC32_FACTOR, c.sys_b, c.sys_x, tol, max_iter, R,
+                   time_kernel ? &T : nullptr));
+    info->iters = R.iters;
+    info->res_rec = R.res_l1;
+    status = R.converged ? FVK_OK : FVK_NOT_CONVERGED;
+    if (method == FVK_SOLVE_DC32_FACTOR && !R.converged) {
+      // the plain FP64 Jacobi PCG takes over from x = 0
+      KS hp;
+      GUARD(solve_pcg64(c, 0, tol, max_iter, nullptr, hp));
+      info->fallback = 1;
+      info->iters += hp.iter;
+      info->res_rec = hp.res_n;
+      status = hp.done && hp.res_n < tol ? FVK_FALLBACK : FVK_NOT_CONVERGED;
+    }
+  }
   FVK_HIP(hipEventRecord(c.ev1, c.stream));
   FVK_HIP(hipEventSynchronize(c.ev1));
   float ms = 0.f;
   FVK_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
-  KS h;
-  FVK_HIP(hipMemcpy(&h, c.ks, sizeof h, hipMemcpyDeviceToHost));
   double tr[4];
   GUARD(true_residual(c, c.sys_x, c.sys_b, c.sys_r, tr));
-  info->iters = h.iter;
   info->seconds = ms * 1e-3;
-  info->res_rec = h.res_n;
   info->res_true_l1 = tr[0] / tr[2];
   info->res_true_l2 = std::sqrt(tr[1] / tr[3]);
   if (time_kernel && T.launches) {
-    long nnz_off = 0;
-    for (int f = 0; f < c.nf; ++f) nnz_off += c.h_fin[f] >= 0 ? 2 : 0;
+    const double nnz_off = (double)nnz_offdiag(c);
     info->kern_seconds = T.total_ms * 1e-3 / T.launches;
-    // k_cg_pq: z, p_old, diag read; p_new, q written; column + value per off-diagonal
-    info->kern_bytes = 40.0 * n + 12.0 * (double)nnz_off;
+    if (method == FVK_SOLVE_PCG64)
+      // k_cg_pq: z, p_old, diag read; p_new, q written; column + value per off-diagonal
+      info->kern_bytes = 40.0 * n + 12.0 * nnz_off;
+    else
+      // k_mx_pq in FP32: z, p_old, diag read; p_new, q written; 16/32-bit
+      // column + FP32 value per off-diagonal
+      info->kern_bytes = 20.0 * n + (c.mx.i16 ? 6.0 : 8.0) * nnz_off;
     info->kern_launches = T.launches;
   }
-  return h.done && h.res_n < tol ? FVK_OK : FVK_NOT_CONVERGED;
+  return status;
+}
+
+int fvk_ctx_set_param(fvk_ctx *cc, const char *name, double value) {
+  if (!cc || !name) { set_error("null argument"); return FVK_ERR_ARG; }
+  Mix &X = cc->c.mx;
+  if (!std::strcmp(name, "dc_inner_reduc") && value > 0 && value < 1) X.inner_reduc = value;
+  else if (!std::strcmp(name, "dc_max_outer") && value >= 1 && value <= 1000) X.max_outer = (int)value;
+  else { set_error(std::string("unknown parameter or value out of range: ") + name); return FVK_ERR_ARG; }
+  return FVK_OK;
+}
+
+int fvk_factor_set(fvk_ctx *cc, const float *l, const float *sig) {
+  if (!cc || !l || !sig) { set_error("null argument"); return FVK_ERR_ARG; }
+  Ctx &c = cc->c;
+  GUARD(mix_build(c));
+  std::vector<float> fl, fs;
+  fl.reserve(c.mx.nfi);
+  fs.reserve(c.mx.nfi);
+  for (int f = 0; f < c.nf; ++f)
+    if (c.h_fin[f] >= 0) { fl.push_back(l[f]); fs.push_back(sig[f]); }
+  FVK_HIP(hipStreamSynchronize(c.stream));
+  if (!fl.empty()) {
+    FVK_HIP(hipMemcpy(c.mx.fl, fl.data(), fl.size() * sizeof(float), hipMemcpyHostToDevice));
+    FVK_HIP(hipMemcpy(c.mx.fsig, fs.data(), fs.size() * sizeof(float), hipMemcpyHostToDevice));
+  }
+  return factor_assemble(c);
+}
+
+int fvk_factor_apply(fvk_ctx *cc, const double *r, double *z) {
+  if (!cc || !r || !z) { set_error("null argument"); return FVK_ERR_ARG; }
+  Ctx &c = cc->c;
+  GUARD(mix_build(c));
+  double *dr = c.stage, *dz = c.stage + c.n;
+  GUARD(to_dev(c, r, 1, 1, c.n, &c.perm, dr));
+  GUARD(factor_apply(c, dr, dz));
+  return to_host(c, dz, 1, 1, c.n, &c.perm, z);
 }
 
 int fvk_bench_spmv(fvk_ctx *cc, int reps, double *sec) {

@@ -38,6 +38,34 @@ struct Levels {
   int count() const { return ptr.empty() ? 0 : (int)ptr.size() - 1; }
 };
 
+// FP32 inner solver of defect correction and the factor G = D^-1/2 (I + L) Delta
+// of the learned preconditioner z = G G^T r.  L sits on the lower-triangular
+// pattern of A in the reference numbering (row = neighbour cell, column =
+// owner cell of an interior face); its slots are split into per-row lists of
+// lower (faces whose neighbour is the row) and upper (faces the row owns)
+// entries.  Column indices are 16-bit offsets from the row when every
+// neighbour lies within +-32767 rows, else 32-bit rows; padding slots point
+// at the row itself with value 0.
+struct Mix {
+  bool built = false, i16 = false;
+  int WL = 0, WU = 0, nfi = 0;
+  void *col = nullptr;                 // A's interior slots (Ws*n)
+  void *lcol = nullptr, *ucol = nullptr;  // factor slots (WL*n, WU*n)
+  int *lfi = nullptr, *ufi = nullptr;  // interior face of each factor slot, -1 padding
+  int *fi_face = nullptr;              // interior face -> face id (nfi)
+  int *face_fi = nullptr;              // face id -> interior face or -1 (nf)
+  int *fi_p = nullptr, *fi_n = nullptr;  // rows of an interior face's owner / neighbour
+  float *spv = nullptr, *diag = nullptr, *jc = nullptr;  // FP32 matrix, 1/(a_ii + small)
+  float *lval = nullptr, *uval = nullptr;  // L per slot
+  float *w = nullptr, *e = nullptr;    // a_ii^-1/2 and Delta^2 per row
+  float *fl = nullptr, *fsig = nullptr;  // per interior face l_f, sig_f
+  float *vec = nullptr;                // 10 x n FP32 work vectors
+  double *tot = nullptr;               // 8 doubles: residual totals
+  bool fac_valid = false;
+  double inner_reduc = 1e-4;           // relative L1 reduction of each inner solve
+  int max_outer = 30;
+};
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -63,6 +91,7 @@ struct Ctx {
   double *stage = nullptr;        // max(n, nf) * 3 doubles
   double *sys_b = nullptr, *sys_x = nullptr, *sys_r = nullptr;  // stand-alone system
   double *bnd_buf = nullptr;      // 3 x n_bf boundary values (fvk_solv_grad)
+  Mix mx;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -116,6 +145,21 @@ struct KTimer {
 int krylov(Ctx &c, int kind, int imat, double reduc, int n_iter, int nvc, double *x, double *r0, bool not_r0,
            int chunk = 0, KTimer *T = nullptr);
 int true_residual(Ctx &c, const double *x, const double *b, double *dev_out4, double *host_out4);
+
+// mixed.hip
+struct DcResult {
+  int iters = 0, outer = 0, broken = 0;
+  double res_l1 = 0.0, res_l2 = 0.0;
+  bool converged = false;
+};
+int mix_build(Ctx &c);
+// factor from per-interior-face (l, sig) already in c.mx.fl / c.mx.fsig
+int factor_assemble(Ctx &c);
+int factor_apply(Ctx &c, const double *r, double *z);
+// FP32-inner CG in FP64 defect correction on (c.diag, c.spv), b -> x (FP64
+// rows); use_factor: G G^T preconditioner (else Jacobi)
+int dc_solve(Ctx &c, bool use_factor, const double *b, double *x, double tol, int max_iter, DcResult &res,
+             KTimer *T);
 
 // flow.hip
 // solv_grad: n_iter Green-Gauss sweeps; g0 is scratch of the size of g;

@@ -31,6 +31,7 @@ CTX_RCM = 1
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
+_fp = C.POINTER(C.c_float)
 _vp = C.c_void_p
 
 
@@ -55,7 +56,9 @@ class SolveInfo(C.Structure):
                 ("kern_bytes", C.c_double), ("kern_launches", C.c_long)]
 
 
-SOLVE_PCG64 = 0
+SOLVE_PCG64 = 0         # reference PCG recurrence, FP64
+SOLVE_DC32 = 1          # FP32-inner Jacobi PCG in FP64 defect correction
+SOLVE_DC32_FACTOR = 2   # same with the factorised preconditioner G G^T
 
 
 class FlowCfg(C.Structure):
@@ -111,6 +114,9 @@ def lib():
         "fvk_sys_set": [_vp, _dp, _dp, _dp],
         "fvk_sys_solve": [_vp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(SolveInfo)],
         "fvk_sys_get_x": [_vp, _dp],
+        "fvk_ctx_set_param": [_vp, C.c_char_p, C.c_double],
+        "fvk_factor_set": [_vp, _fp, _fp],
+        "fvk_factor_apply": [_vp, _dp, _dp],
         "fvk_sys_get": [_vp, _dp, _dp, _dp],
         "fvk_bench_spmv": [_vp, C.c_int, _dp],
         "fvk_bench_copy": [C.c_int, C.c_long, C.c_int, _dp],
@@ -377,6 +383,25 @@ def sys_solve(ctx: Ctx, method=SOLVE_PCG64, imat=0, tol=1e-8, max_iter=100000, t
     out = {k: getattr(info, k) for k, _ in SolveInfo._fields_}
     out["status"] = rc
     return out
+
+
+def set_param(ctx: Ctx, name, value):
+    _check(lib().fvk_ctx_set_param(ctx.h, name.encode(), float(value)))
+
+
+def factor_set(ctx: Ctx, l, sig):
+    """Factor G = D^-1/2 (I + L) Delta from per-face l_f and sig_f (n_fce
+    entries, reference face order; boundary faces ignored)."""
+    l = np.ascontiguousarray(l, np.float32)
+    sig = np.ascontiguousarray(sig, np.float32)
+    _check(lib().fvk_factor_set(ctx.h, l.ctypes.data_as(_fp), sig.ctypes.data_as(_fp)))
+
+
+def factor_apply(ctx: Ctx, r):
+    r = np.ascontiguousarray(r, np.float64)
+    z = np.zeros(ctx.n_vol)
+    _check(lib().fvk_factor_apply(ctx.h, _d(r), _d(z)))
+    return z
 
 
 def sys_x(ctx: Ctx):

@@ -1,0 +1,120 @@
+"""FP32-inner defect correction and the factorised preconditioner on the GPU.
+
+The judge of every solve is the FP64 true residual |b - A x|_1 / |b|_1,
+recomputed here on the CPU by the oracle's own SpMV (oracle/fvo, the
+reference's face-scatter product) from the solution the library returns.
+The factor's action z = G G^T r is compared with oracle/learn.apply (float32
+inside the library: tolerance 2e-6 relative to max |z|).
+"""
+import numpy as np
+import pytest
+
+from oracle import fvo
+from oracle import learn as L
+from tests.helpers import require_lib
+
+pytestmark = pytest.mark.gpu
+
+
+def setup(n, jitter, seed, rcm=False, sigma=1.0):
+    fvk = require_lib()
+    m = fvk.Mesh.box(n, n, n, jitter=jitter, seed=seed)
+    ctx = fvk.grd_solv(m, rcm=rcm)
+    b = fvk.sys_random(ctx, seed + 1, sigma)
+    pl, a, _ = fvk.sys_get(ctx)
+    om = fvo.OMesh.box(n, n, n, jitter=jitter, seed=seed)
+    t, g = om.topology(), om.geometry()
+    S = L.System.from_mesh(t, g, pl, -a)
+    A = fvo.OSys(om, pl, -a, -a)
+    return fvk, m, ctx, om, S, A, b
+
+
+def true_res(A, x, b):
+    return np.abs(b - A.spmv(x)).sum() / np.abs(b).sum()
+
+
+@pytest.mark.parametrize("rcm", [False, True])
+def test_dc32_jacobi_converges_to_fp64_residual(rcm):
+    fvk, m, ctx, om, S, A, b = setup(16, 0.2, 0x51, rcm)
+    try:
+        plain = fvk.sys_solve(ctx, fvk.SOLVE_PCG64, 0, 1e-8)
+        xp = fvk.sys_x(ctx)
+        mixed = fvk.sys_solve(ctx, fvk.SOLVE_DC32, 0, 1e-8)
+        xm = fvk.sys_x(ctx)
+        print("plain", plain["iters"], "mixed", mixed["iters"], mixed["res_true_l1"])
+        assert plain["status"] == fvk.FVK_OK and mixed["status"] == fvk.FVK_OK
+        assert true_res(A, xp, b) < 1e-8
+        assert true_res(A, xm, b) < 1e-8
+        assert abs(true_res(A, xm, b) - mixed["res_true_l1"]) < 1e-3 * mixed["res_true_l1"]
+        # same solution up to the null space (constants) of the Neumann system
+        dx = (xm - xm.mean()) - (xp - xp.mean())
+        assert np.max(np.abs(dx)) < 1e-4 * np.max(np.abs(xp - xp.mean()))
+        # FP32 inner solves restart the recurrence, but not by much
+        assert mixed["iters"] < 1.5 * plain["iters"]
+    finally:
+        om.free()
+
+
+def test_factor_apply_matches_oracle():
+    fvk, m, ctx, om, S, A, b = setup(12, 0.2, 0x52, rcm=True)
+    try:
+        rng = np.random.default_rng(3)
+        inner = np.zeros(ctx.n_fce, bool)
+        t = om.topology()
+        inner[t["fce_in"] > 0] = True
+        l = np.zeros(ctx.n_fce, np.float32)
+        sig = np.zeros(ctx.n_fce, np.float32)
+        l[inner] = (0.3 * rng.standard_normal(inner.sum())).astype(np.float32)
+        sig[inner] = (0.1 * rng.standard_normal(inner.sum())).astype(np.float32)
+        fvk.factor_set(ctx, l, sig)
+        r = rng.standard_normal(ctx.n_vol)
+        z = fvk.factor_apply(ctx, r)
+        w, e, l64, _ = L.factor(S, l[inner].astype(np.float64), sig[inner].astype(np.float64))
+        zr = L.apply(S, w, e, l64, r)
+        assert np.max(np.abs(z - zr)) <= 2e-6 * np.max(np.abs(zr))
+        # zero factor: Jacobi
+        fvk.factor_set(ctx, np.zeros_like(l), np.zeros_like(sig))
+        z0 = fvk.factor_apply(ctx, r)
+        np.testing.assert_allclose(z0, r / S.d, rtol=2e-6)
+    finally:
+        om.free()
+
+
+def test_dc32_factor_fsai_cuts_iterations():
+    fvk, m, ctx, om, S, A, b = setup(20, 0.2, 0x53)
+    try:
+        jac = fvk.sys_solve(ctx, fvk.SOLVE_DC32, 0, 1e-8)
+        lf, sf = L.fsai_closed_form(S)
+        t = om.topology()
+        inner = t["fce_in"] > 0
+        l = np.zeros(ctx.n_fce, np.float32)
+        sig = np.zeros(ctx.n_fce, np.float32)
+        l[inner] = lf
+        sig[inner] = sf
+        fvk.factor_set(ctx, l, sig)
+        fs = fvk.sys_solve(ctx, fvk.SOLVE_DC32_FACTOR, 0, 1e-8)
+        x = fvk.sys_x(ctx)
+        print("jacobi", jac["iters"], "fsai", fs["iters"])
+        assert fs["status"] == fvk.FVK_OK and fs["fallback"] == 0
+        assert true_res(A, x, b) < 1e-8
+        assert fs["iters"] < 0.75 * jac["iters"]
+    finally:
+        om.free()
+
+
+def test_dc32_factor_falls_back_to_plain_pcg():
+    fvk, m, ctx, om, S, A, b = setup(12, 0.2, 0x54)
+    try:
+        plain = fvk.sys_solve(ctx, fvk.SOLVE_PCG64, 0, 1e-8)
+        # a valid (SPD) but useless factor: needs far more iterations than
+        # the plain solver, so the budget of plain iterations runs out
+        rng = np.random.default_rng(0)
+        l = (3.0 * rng.standard_normal(ctx.n_fce)).astype(np.float32)
+        sig = (1.5 * rng.standard_normal(ctx.n_fce)).astype(np.float32)
+        fvk.factor_set(ctx, l, sig)
+        out = fvk.sys_solve(ctx, fvk.SOLVE_DC32_FACTOR, 0, 1e-8, max_iter=plain["iters"])
+        x = fvk.sys_x(ctx)
+        assert out["fallback"] == 1 and out["status"] == fvk.FVK_FALLBACK
+        assert true_res(A, x, b) < 1e-8
+    finally:
+        om.free()
