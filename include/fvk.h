@@ -184,9 +184,10 @@ int fvk_sys_set(fvk_ctx *ctx, const double *pl, const double *a, const double *b
 int fvk_sys_solve(fvk_ctx *ctx, int method, int imat, double tol, int max_iter, int time_kernel,
                   fvk_solve_info *info);
 int fvk_sys_get_x(fvk_ctx *ctx, double *x);
-/* Solver settings of a context: "dc_inner_reduc" (relative L1 reduction of
-   each FP32 inner solve, default 1e-4), "dc_max_outer" (defect-correction
-   steps, default 30). */
+/* Solver settings of a context: "dc_inner_reduc" (the deepest relative L1
+   reduction asked of an FP32 inner solve, default 1e-5; each inner solve
+   aims at max(dc_inner_reduc, 0.3 tol / current residual)), "dc_max_outer"
+   (defect-correction steps, default 30). */
 int fvk_ctx_set_param(fvk_ctx *ctx, const char *name, double value);
 
 /* The factorised preconditioner z = G G^T r, G = D^-1/2 (I + L) Delta, with
@@ -198,6 +199,44 @@ int fvk_ctx_set_param(fvk_ctx *ctx, const char *name, double value);
 int fvk_factor_set(fvk_ctx *ctx, const float *l, const float *sig);
 /* z = G G^T r (FP32 inside; host arrays in reference order). */
 int fvk_factor_apply(fvk_ctx *ctx, const double *r, double *z);
+/* Per-face l_f, sig_f of the current factor (n_fce entries, 0 on boundary
+   faces). */
+int fvk_factor_get(fvk_ctx *ctx, float *l, float *sig);
+
+/* ------------------------------------------------------------------------
+ * The learned factor: a per-face MLP (12 -> 64 -> 64 -> 2, tanh; bfloat16
+ * MFMA with float32 accumulation) maps local features of the Jacobi-scaled
+ * matrix and the face geometry to (l_f, sig_f).  Trained on the GPU on the
+ * systems the solver generates, by Adam on Kaporin's measure
+ * log(tr(K)/N) - log(det K)/N of K = Gh^T Ah Gh (Ah = D^-1/2 A D^-1/2,
+ * Gh = D^1/2 G), whose minimiser over the pattern is FSAI.  No counterpart
+ * in the reference.
+ *
+ * Parameters (float32, fvk_model_nparam() = 5122): W1[12][64], b1[64],
+ * W2[64][64], b2[64], W3[64][2], b3[2], input-major.  Features per interior
+ * face f = (p, n): a/sqrt(a_pp a_nn), a/a_pp, a/a_nn, s/h^2, normal (3),
+ * l_pn/h, w_fp, |dx_p|/h, |dx_n|/h, log(V_p/V_n), with a = -A_pn and
+ * h = ((V_p + V_n)/2)^(1/3).
+ * ---------------------------------------------------------------------- */
+typedef struct fvk_model fvk_model;
+
+int fvk_model_nparam(void);
+/* Hidden layers Xavier-uniform from SplitMix64(seed), output layer zero:
+   the untrained factor is Jacobi. */
+int fvk_model_create(int device, uint64_t seed, fvk_model **out);
+void fvk_model_destroy(fvk_model *m);
+int fvk_model_get(const fvk_model *m, float *theta);
+/* Sets the parameters and resets the optimiser state. */
+int fvk_model_set(fvk_model *m, const float *theta);
+/* Factor of the context's current matrix from the network (then
+   FVK_SOLVE_DC32_FACTOR uses it). */
+int fvk_factor_build(fvk_ctx *ctx, const fvk_model *m);
+/* Loss J of the network on the context's current matrix; grad (5122, may be
+   NULL): dJ/dtheta; dl, dsig (n_fce, may be NULL): dJ/dl_f, dJ/dsig_f. */
+int fvk_model_loss(fvk_model *m, fvk_ctx *ctx, double *J, float *grad, float *dl, float *dsig);
+/* steps Adam steps (beta 0.9, 0.999) of rate lr on the context's current
+   matrix; loss (steps entries, may be NULL) receives J before each step. */
+int fvk_model_train(fvk_model *m, fvk_ctx *ctx, int steps, double lr, double *loss);
 /* The loaded system in reference order: pl (n_vol), a (n_fce, 0 on boundary
    faces), b (n_vol); NULL pointers skip. */
 int fvk_sys_get(fvk_ctx *ctx, double *pl, double *a, double *b);

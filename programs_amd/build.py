@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libfvk.so")
 BUILD = os.path.join(HERE, "_build")
 
-HIP_SOURCES = ["krylov.hip", "mixed.hip", "flow.hip", "capi.hip"]
+HIP_SOURCES = ["krylov.hip", "mixed.hip", "learn.hip", "flow.hip", "capi.hip"]
 CPP_SOURCES = ["mesh.cpp"]
 ARCH = os.environ.get("FVK_OFFLOAD_ARCH", "gfx950")
 

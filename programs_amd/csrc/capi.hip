@@ -16,9 +16,6 @@
 struct fvk_mesh {
   fvk::Mesh m;
 };
-struct fvk_ctx {
-  fvk::Ctx c;
-};
 struct fvk_flow {
   fvk::Flow f;
 };

@@ -62,7 +62,7 @@ struct Mix {
   float *vec = nullptr;                // 10 x n FP32 work vectors
   double *tot = nullptr;               // 8 doubles: residual totals
   bool fac_valid = false;
-  double inner_reduc = 1e-4;           // relative L1 reduction of each inner solve
+  double inner_reduc = 1e-5;           // deepest relative L1 reduction of an inner solve
   int max_outer = 30;
 };
 
@@ -168,3 +168,8 @@ int grad(Ctx &c, int nvc, const double *phi, const double *phib, double *g, int 
          const double *on_flag, double *g0);
 
 }  // namespace fvk
+
+// The opaque handle of the C ABI
+struct fvk_ctx {
+  fvk::Ctx c;
+};

@@ -10,7 +10,8 @@
 //
 //   x = 0
 //   repeat: r = b - A x (FP64); stop if |r|_1 / |b|_1 < tol
-//           solve A e = r in FP32 to relative L1 reduction inner_reduc
+//           solve A e = r in FP32 to relative L1 reduction
+//             eta = max(inner_reduc, 0.3 tol / (|r|_1 / |b|_1))
 //           x += e (FP64)
 //
 // Per inner iteration three kernels (factor) or two (Jacobi):
@@ -521,9 +522,12 @@ int dc_solve(Ctx &c, bool use_factor, const double *b, double *x, double tol, in
     if (!(res.res_l1 == res.res_l1)) { res.broken = 1; break; }
     if (res.res_l1 < tol) { res.converged = true; break; }
     if (outer >= X.max_outer || res.iters >= max_iter || res.broken) break;
+    // inner target: what is left to reach tol (with a margin), but no
+    // deeper than inner_reduc, where FP32 rounding takes over
+    const double eta = std::min(0.5, std::max(X.inner_reduc, 0.3 * tol / res.res_l1));
     int its = 0, broken = 0;
-    rc = X.i16 ? inner_solve<short>(c, use_factor, max_iter - res.iters, X.inner_reduc, its, broken, T)
-               : inner_solve<int>(c, use_factor, max_iter - res.iters, X.inner_reduc, its, broken, T);
+    rc = X.i16 ? inner_solve<short>(c, use_factor, max_iter - res.iters, eta, its, broken, T)
+               : inner_solve<int>(c, use_factor, max_iter - res.iters, eta, its, broken, T);
     if (rc) return rc;
     res.iters += its;
     res.broken |= broken;

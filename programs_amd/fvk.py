@@ -117,6 +117,14 @@ def lib():
         "fvk_ctx_set_param": [_vp, C.c_char_p, C.c_double],
         "fvk_factor_set": [_vp, _fp, _fp],
         "fvk_factor_apply": [_vp, _dp, _dp],
+        "fvk_factor_get": [_vp, _fp, _fp],
+        "fvk_model_nparam": [],
+        "fvk_model_create": [C.c_int, C.c_uint64, C.POINTER(_vp)],
+        "fvk_model_get": [_vp, _fp],
+        "fvk_model_set": [_vp, _fp],
+        "fvk_factor_build": [_vp, _vp],
+        "fvk_model_loss": [_vp, _vp, _dp, _fp, _fp, _fp],
+        "fvk_model_train": [_vp, _vp, C.c_int, C.c_double, _dp],
         "fvk_sys_get": [_vp, _dp, _dp, _dp],
         "fvk_bench_spmv": [_vp, C.c_int, _dp],
         "fvk_bench_copy": [C.c_int, C.c_long, C.c_int, _dp],
@@ -134,7 +142,7 @@ def lib():
         f = getattr(L, name)
         f.argtypes = args
         f.restype = C.c_int
-    for name in ("fvk_mesh_free", "fvk_ctx_destroy", "fvk_flow_destroy"):
+    for name in ("fvk_mesh_free", "fvk_ctx_destroy", "fvk_flow_destroy", "fvk_model_destroy"):
         getattr(L, name).argtypes = [_vp]
         getattr(L, name).restype = None
     _lib = L
@@ -402,6 +410,57 @@ def factor_apply(ctx: Ctx, r):
     z = np.zeros(ctx.n_vol)
     _check(lib().fvk_factor_apply(ctx.h, _d(r), _d(z)))
     return z
+
+
+def factor_get(ctx: Ctx):
+    l = np.zeros(ctx.n_fce, np.float32)
+    sig = np.zeros(ctx.n_fce, np.float32)
+    _check(lib().fvk_factor_get(ctx.h, l.ctypes.data_as(_fp), sig.ctypes.data_as(_fp)))
+    return l, sig
+
+
+class Model:
+    """The per-face MLP of the learned factor, resident on one GPU."""
+
+    def __init__(self, seed=0, device=0):
+        self.h = _vp()
+        _check(lib().fvk_model_create(device, seed, C.byref(self.h)))
+        self.nparam = lib().fvk_model_nparam()
+
+    def get(self):
+        th = np.zeros(self.nparam, np.float32)
+        _check(lib().fvk_model_get(self.h, th.ctypes.data_as(_fp)))
+        return th
+
+    def set(self, theta):
+        th = np.ascontiguousarray(theta, np.float32)
+        assert th.size == self.nparam
+        _check(lib().fvk_model_set(self.h, th.ctypes.data_as(_fp)))
+
+    def build_factor(self, ctx: Ctx):
+        _check(lib().fvk_factor_build(ctx.h, self.h))
+
+    def loss(self, ctx: Ctx, grads=False):
+        J = C.c_double()
+        if not grads:
+            _check(lib().fvk_model_loss(self.h, ctx.h, C.byref(J), None, None, None))
+            return J.value
+        g = np.zeros(self.nparam, np.float32)
+        dl = np.zeros(ctx.n_fce, np.float32)
+        ds = np.zeros(ctx.n_fce, np.float32)
+        _check(lib().fvk_model_loss(self.h, ctx.h, C.byref(J), g.ctypes.data_as(_fp), dl.ctypes.data_as(_fp),
+                                    ds.ctypes.data_as(_fp)))
+        return J.value, g, dl, ds
+
+    def train(self, ctx: Ctx, steps, lr=1e-3):
+        losses = np.zeros(steps)
+        _check(lib().fvk_model_train(self.h, ctx.h, steps, lr, _d(losses)))
+        return losses
+
+    def destroy(self):
+        if self.h:
+            lib().fvk_model_destroy(self.h)
+            self.h = None
 
 
 def sys_x(ctx: Ctx):
