@@ -215,8 +215,9 @@ int fvk_factor_get(fvk_ctx *ctx, float *l, float *sig);
  * Parameters (float32, fvk_model_nparam() = 5122): W1[12][64], b1[64],
  * W2[64][64], b2[64], W3[64][2], b3[2], input-major.  Features per interior
  * face f = (p, n): a/sqrt(a_pp a_nn), a/a_pp, a/a_nn, s/h^2, normal (3),
- * l_pn/h, w_fp, |dx_p|/h, |dx_n|/h, log(V_p/V_n), with a = -A_pn and
- * h = ((V_p + V_n)/2)^(1/3).
+ * l_pn/h, w_fp, |dx_p|/h, |dx_n|/h, log(|V_p|/|V_n|), with a = -A_pn and
+ * h = ((|V_p| + |V_n|)/2)^(1/3) (the reference's cell volumes can be
+ * negative on warped cells).
  * ---------------------------------------------------------------------- */
 typedef struct fvk_model fvk_model;
 

@@ -119,7 +119,11 @@ def features(S):
     """Twelve features per interior face (float64)."""
     g = S.geo
     dp, dn = S.d[S.p], S.d[S.n]
-    vp, vn = g["vol"][S.p], g["vol"][S.n]
+    # |V|: the reference's volumes (x_f s_f n_x summed over faces from the
+    # origin, grd_geom.f90:151-153,172-173) turn negative on strongly
+    # warped cells
+    vp = np.maximum(np.abs(g["vol"][S.p]), 1e-300)
+    vn = np.maximum(np.abs(g["vol"][S.n]), 1e-300)
     h = np.cbrt(0.5 * (vp + vn))
     X = np.empty((len(S.a), NIN))
     X[:, 0] = S.a / np.sqrt(dp * dn)

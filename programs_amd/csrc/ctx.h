@@ -55,8 +55,8 @@ struct Mix {
   int *fi_face = nullptr;              // interior face -> face id (nfi)
   int *face_fi = nullptr;              // face id -> interior face or -1 (nf)
   int *fi_p = nullptr, *fi_n = nullptr;  // rows of an interior face's owner / neighbour
-  float *spv = nullptr, *diag = nullptr, *jc = nullptr;  // FP32 matrix, 1/(a_ii + small)
-  float *lval = nullptr, *uval = nullptr;  // L per slot
+  float *spv = nullptr;                // Jacobi-scaled off-diagonals a_ij / sqrt(a_ii a_jj), FP32
+  void *lval = nullptr, *uval = nullptr;  // L per slot, FP16
   float *w = nullptr, *e = nullptr;    // a_ii^-1/2 and Delta^2 per row
   float *fl = nullptr, *fsig = nullptr;  // per interior face l_f, sig_f
   float *vec = nullptr;                // 10 x n FP32 work vectors

@@ -107,7 +107,9 @@ __device__ v4s features4(const FaceIn &F, int q, int g) {
   v4f x = {0.f, 0.f, 0.f, 0.f};
   if (q < F.nfi && g < 3) {
     const int f = F.fi_face[q], p = F.fi_p[q], m = F.fi_n[q];
-    const double vp = F.M.v[p], vn = F.M.v[m];
+    // |V|: the reference's volumes (x_f s_f n_x summed from the origin,
+    // grd_geom.f90:151-153,172-173) turn negative on strongly warped cells
+    const double vp = fmax(fabs(F.M.v[p]), 1e-300), vn = fmax(fabs(F.M.v[m]), 1e-300);
     const double h = cbrt(0.5 * (vp + vn));
     if (g == 0) {
       const double a = -F.ngp[f], dp = F.diag[p], dn = F.diag[m];

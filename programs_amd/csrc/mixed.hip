@@ -4,23 +4,27 @@
 // No counterpart in the reference: spray-simulation solves its pressure
 // systems in FP64 with solv_cg.f90 / solv_precond.f90.  This path keeps the
 // reference's PCG recurrence (the same scalar decisions, cgcore.h) but runs
-// the inner SpMV, preconditioner and vector updates in FP32 over an FP32
-// copy of the matrix, with FP64 dot products, FP64 alpha/beta and an FP64
-// true residual b - A x on the FP64 matrix between inner solves:
+// the inner SpMV, preconditioner and vector updates in FP32, with FP64 dot
+// products, FP64 alpha/beta and an FP64 true residual b - A x on the FP64
+// matrix between inner solves:
 //
 //   x = 0
 //   repeat: r = b - A x (FP64); stop if |r|_1 / |b|_1 < tol
-//           solve A e = r in FP32 to relative L1 reduction
+//           solve Ah eh = D^-1/2 r in FP32 to relative L1 reduction
 //             eta = max(inner_reduc, 0.3 tol / (|r|_1 / |b|_1))
-//           x += e (FP64)
+//           x += D^-1/2 eh (FP64)
 //
+// The inner system is the Jacobi-scaled one, Ah = D^-1/2 A D^-1/2 (unit
+// diagonal, FP32 off-diagonals, 16-bit column offsets where they fit): its
+// SpMV reads no diagonal, Jacobi PCG on A is plain CG on Ah (z = r), and the
+// factor becomes Gh = (I + L) Delta with zh = Gh Gh^T rh (L in FP16).
 // Per inner iteration three kernels (factor) or two (Jacobi):
-//   k_mx_pq   decide beta; p = z + beta p; q = A p; partial p.q
+//   k_mx_pq   decide beta; p = z + beta p; q = Ah p; partial p.q
 //   k_mx_upd  alpha; e += alpha p; r' = r - alpha q; partial |r'|_1 and
-//             Jacobi: z = c r', partial r'.z
-//             factor: t = Delta^2 (u + L^T u), u = D^-1/2 r' formed on the
-//             fly for the higher neighbours (r is double-buffered)
-//   k_mx_gz   factor: z = D^-1/2 (t + L t), partial r'.z
+//             Jacobi: partial r'.r'
+//             factor: t = Delta^2 (r' + L^T r'), formed on the fly for the
+//             higher neighbours (r is double-buffered)
+//   k_mx_gz   factor: z = t + L t, partial r'.z
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -29,6 +33,8 @@
 #include "cgcore.h"
 
 namespace fvk {
+
+typedef _Float16 lval_t;
 
 // Column of slot k of row i: 32-bit rows, or 16-bit offsets from the row.
 __device__ __forceinline__ int colat(const int *c, size_t k, int i) { return c[k]; }
@@ -39,19 +45,23 @@ __device__ __forceinline__ int nw(int w) { return W > 0 ? W : w; }
 
 __device__ __forceinline__ bool gate_done(const KS *g) { return g && g->done; }
 
-// FP32 copy of the matrix and the Jacobi factor 1/(a_ii + small)
-__global__ __launch_bounds__(BS) void k_mx_conv(int n, int Ws, const double *__restrict__ diag,
-                                                const double *__restrict__ spv, float *__restrict__ d32,
-                                                float *__restrict__ s32, float *__restrict__ jc) {
+// Jacobi-scaled FP32 copy of the matrix, w = a_ii^-1/2
+__global__ __launch_bounds__(BS) void k_mx_conv(DMesh M, const double *__restrict__ diag,
+                                                const double *__restrict__ spv, float *__restrict__ s32,
+                                                float *__restrict__ w) {
   const int i = blockIdx.x * BS + threadIdx.x;
+  const int n = M.n;
   if (i >= n) return;
-  const double d = diag[i];
-  d32[i] = (float)d;
-  jc[i] = (float)(1.0 / (d + SMALL));
-  for (int k = 0; k < Ws; ++k) s32[(size_t)k * n + i] = (float)spv[(size_t)k * n + i];
+  const double wi = 1.0 / sqrt(diag[i]);
+  w[i] = (float)wi;
+  for (int k = 0; k < M.Ws; ++k) {
+    const size_t o = (size_t)k * n + i;
+    const int j = M.spc[o];
+    s32[o] = j >= 0 ? (float)(spv[o] * wi / sqrt(diag[j])) : 0.f;
+  }
 }
 
-// FP64 true residual r = b - A x, its FP32 copy, partials
+// FP64 true residual r = b - A x, its scaled FP32 copy D^-1/2 r, partials
 // {|r|_1, |r|_2^2, |b|_1, |b|_2^2}
 template <int WS>
 __global__ __launch_bounds__(RB) void k_mx_res(DMesh M, const double *__restrict__ diag,
@@ -63,7 +73,7 @@ __global__ __launch_bounds__(RB) void k_mx_res(DMesh M, const double *__restrict
   block_rows(M.n, r0, r1);
   for (int i = r0 + threadIdx.x; i < r1; i += RB) {
     const double bi = b[i], ri = bi - row_spmv<WS>(M, diag, spv, x, i);
-    r32[i] = (float)ri;
+    r32[i] = (float)(ri / sqrt(diag[i]));
     a[0] += fabs(ri);
     a[1] += ri * ri;
     a[2] += fabs(bi);
@@ -72,15 +82,14 @@ __global__ __launch_bounds__(RB) void k_mx_res(DMesh M, const double *__restrict
   store_partials<4>(a, part);
 }
 
-// Start of an inner solve: e = 0, p_old = 0; Jacobi: z = c r with partials
-// |r|_1 [0], r.z [3]; factor: t = Delta^2 (u + L^T u), u = w r, partial |r|_1.
+// Start of an inner solve: e = 0, p_old = 0, partials |r|_1 [0] and
+// Jacobi: r.r [3]; factor: t = Delta^2 (r + L^T r).
 template <int WU, class I>
 __global__ __launch_bounds__(RB) void k_mx_start(int n, int fac, const float *__restrict__ r,
-                                                 const float *__restrict__ jc, float *__restrict__ z,
                                                  float *__restrict__ ev, float *__restrict__ p0, int wu,
-                                                 const I *__restrict__ ucol, const float *__restrict__ uval,
-                                                 const float *__restrict__ w, const float *__restrict__ ee,
-                                                 float *__restrict__ t, double *__restrict__ part) {
+                                                 const I *__restrict__ ucol, const lval_t *__restrict__ uval,
+                                                 const float *__restrict__ ee, float *__restrict__ t,
+                                                 double *__restrict__ part) {
   double a[4] = {0, 0, 0, 0};
   int r0, r1;
   block_rows(n, r0, r1);
@@ -90,16 +99,13 @@ __global__ __launch_bounds__(RB) void k_mx_start(int n, int fac, const float *__
     p0[i] = 0.f;
     a[0] += fabs((double)ri);
     if (!fac) {
-      const float zi = jc[i] * ri;
-      z[i] = zi;
-      a[3] += (double)ri * (double)zi;
+      a[3] += (double)ri * (double)ri;
     } else {
-      float acc = w[i] * ri;
+      float acc = ri;
 #pragma unroll
       for (int k = 0; k < nw<WU>(wu); ++k) {
         const size_t o = (size_t)k * n + i;
-        const int j = colat(ucol, o, i);
-        acc += uval[o] * (w[j] * r[j]);
+        acc += (float)uval[o] * r[colat(ucol, o, i)];
       }
       t[i] = ee[i] * acc;
     }
@@ -107,14 +113,13 @@ __global__ __launch_bounds__(RB) void k_mx_start(int n, int fac, const float *__
   store_partials<4>(a, part);
 }
 
-// Decide iteration it of the inner PCG; p = z + beta p ; q = A p ; p.q
+// Decide iteration it of the inner PCG; p = z + beta p ; q = Ah p ; p.q
 template <int WS, class I>
 __device__ __forceinline__ void mx_pq_row(int n, int i, const I *__restrict__ col, const float *__restrict__ spv,
-                                          const float *__restrict__ diag, const float *__restrict__ z,
-                                          const float *__restrict__ po, float bf, float *__restrict__ pn,
-                                          float *__restrict__ q, int ws, double &pq) {
+                                          const float *__restrict__ z, const float *__restrict__ po, float bf,
+                                          float *__restrict__ pn, float *__restrict__ q, int ws, double &pq) {
   const float pi = z[i] + bf * po[i];
-  float y = diag[i] * pi;
+  float y = pi;
 #pragma unroll
   for (int k = 0; k < nw<WS>(ws); ++k) {
     const size_t o = (size_t)k * n + i;
@@ -128,10 +133,9 @@ __device__ __forceinline__ void mx_pq_row(int n, int i, const I *__restrict__ co
 
 template <int WS, class I>
 __global__ __launch_bounds__(RB) void k_mx_pq(int n, int ws, int it, const I *__restrict__ col,
-                                              const float *__restrict__ spv, const float *__restrict__ diag,
-                                              const float *__restrict__ z, const float *__restrict__ pold,
-                                              float *__restrict__ pnew, float *__restrict__ q,
-                                              const double *__restrict__ partU, int PU,
+                                              const float *__restrict__ spv, const float *__restrict__ z,
+                                              const float *__restrict__ pold, float *__restrict__ pnew,
+                                              float *__restrict__ q, const double *__restrict__ partU, int PU,
                                               const double *__restrict__ partR, int PR, double *__restrict__ partQ,
                                               KS *ring, KS *out) {
   double L1[1], rz[1];
@@ -160,35 +164,65 @@ __global__ __launch_bounds__(RB) void k_mx_pq(int n, int ws, int it, const I *__
   block_rows(n, r0, r1);
   int i = r0 + threadIdx.x;
   for (; i + RB < r1; i += 2 * RB) {
-    mx_pq_row<WS, I>(n, i, col, spv, diag, z, pold, bf, pnew, q, ws, pq);
-    mx_pq_row<WS, I>(n, i + RB, col, spv, diag, z, pold, bf, pnew, q, ws, pq);
+    mx_pq_row<WS, I>(n, i, col, spv, z, pold, bf, pnew, q, ws, pq);
+    mx_pq_row<WS, I>(n, i + RB, col, spv, z, pold, bf, pnew, q, ws, pq);
   }
-  if (i < r1) mx_pq_row<WS, I>(n, i, col, spv, diag, z, pold, bf, pnew, q, ws, pq);
+  if (i < r1) mx_pq_row<WS, I>(n, i, col, spv, z, pold, bf, pnew, q, ws, pq);
   double v[1] = {pq};
   store_partials<1>(v, partQ);
 }
 
-// alpha = s/(p.q + small); e += alpha p ; r' = r - alpha q ; then Jacobi
-// z = c r' or factor t = Delta^2 (u' + L^T u').  A non-positive or non-finite
-// p.q is a breakdown: nothing is updated and the next decision stops.
-template <int WU, class I>
-__global__ __launch_bounds__(RB) void k_mx_upd(int n, int it, int fac, float *__restrict__ ev,
-                                               const float *__restrict__ rin, float *__restrict__ rout,
-                                               const float *__restrict__ p, const float *__restrict__ q,
-                                               const float *__restrict__ jc, float *__restrict__ z, int wu,
-                                               const I *__restrict__ ucol, const float *__restrict__ uval,
-                                               const float *__restrict__ w, const float *__restrict__ ee,
-                                               float *__restrict__ t, const double *__restrict__ partQ, int PQ,
-                                               double *__restrict__ partU, KS *ring) {
-  KS *cur = ring + (it & 1) * 3;
-  if (cur->done) return;
+// alpha = s/(p.q + small) from the partials; false on breakdown (a
+// non-positive or non-finite p.q), flagged for the next decision.
+__device__ __forceinline__ bool mx_alpha(KS *cur, const double *partQ, int PQ, double &alpha) {
   double tq[1];
   load_totals<1>(partQ, PQ, tq);
-  const double alpha = cur->s / (tq[0] + SMALL);
+  alpha = cur->s / (tq[0] + SMALL);
   if (!(tq[0] > 0.0) || !isfinite(alpha)) {
     if (blockIdx.x == 0 && threadIdx.x == 0) cur->pad = 1;
-    return;
+    return false;
   }
+  return true;
+}
+
+// Jacobi: e += alpha p ; r -= alpha q (in place) ; partials |r|_1, r.r
+__global__ __launch_bounds__(RB) void k_mx_upd_jac(int n, int it, float *__restrict__ ev, float *__restrict__ r,
+                                                   const float *__restrict__ p, const float *__restrict__ q,
+                                                   const double *__restrict__ partQ, int PQ,
+                                                   double *__restrict__ partU, KS *ring) {
+  KS *cur = ring + (it & 1) * 3;
+  if (cur->done) return;
+  double alpha;
+  if (!mx_alpha(cur, partQ, PQ, alpha)) return;
+  const float af = (float)alpha;
+  double a[4] = {0, 0, 0, 0};
+  int r0, r1;
+  block_rows(n, r0, r1);
+  for (int i = r0 + threadIdx.x; i < r1; i += RB) {
+    ev[i] += af * p[i];
+    const float ri = r[i] - af * q[i];
+    r[i] = ri;
+    a[0] += fabs((double)ri);
+    a[3] += (double)ri * (double)ri;
+  }
+  store_partials<4>(a, partU);
+  if (blockIdx.x == 0 && threadIdx.x == 0) cur->alpha = alpha;
+}
+
+// Factor: e += alpha p ; r' = r - alpha q ; t = Delta^2 (r' + L^T r') with
+// r' of the higher neighbours formed from (r, q) ; partial |r'|_1
+template <int WU, class I>
+__global__ __launch_bounds__(RB) void k_mx_upd_fac(int n, int it, float *__restrict__ ev,
+                                                   const float *__restrict__ rin, float *__restrict__ rout,
+                                                   const float *__restrict__ p, const float *__restrict__ q, int wu,
+                                                   const I *__restrict__ ucol, const lval_t *__restrict__ uval,
+                                                   const float *__restrict__ ee, float *__restrict__ t,
+                                                   const double *__restrict__ partQ, int PQ,
+                                                   double *__restrict__ partU, KS *ring) {
+  KS *cur = ring + (it & 1) * 3;
+  if (cur->done) return;
+  double alpha;
+  if (!mx_alpha(cur, partQ, PQ, alpha)) return;
   const float af = (float)alpha;
   double a[4] = {0, 0, 0, 0};
   int r0, r1;
@@ -198,31 +232,25 @@ __global__ __launch_bounds__(RB) void k_mx_upd(int n, int it, int fac, float *__
     const float ri = rin[i] - af * q[i];
     rout[i] = ri;
     a[0] += fabs((double)ri);
-    if (!fac) {
-      const float zi = jc[i] * ri;
-      z[i] = zi;
-      a[3] += (double)ri * (double)zi;
-    } else {
-      float acc = w[i] * ri;
+    float acc = ri;
 #pragma unroll
-      for (int k = 0; k < nw<WU>(wu); ++k) {
-        const size_t o = (size_t)k * n + i;
-        const int j = colat(ucol, o, i);
-        acc += uval[o] * (w[j] * (rin[j] - af * q[j]));
-      }
-      t[i] = ee[i] * acc;
+    for (int k = 0; k < nw<WU>(wu); ++k) {
+      const size_t o = (size_t)k * n + i;
+      const int j = colat(ucol, o, i);
+      acc += (float)uval[o] * (rin[j] - af * q[j]);
     }
+    t[i] = ee[i] * acc;
   }
   store_partials<4>(a, partU);
   if (blockIdx.x == 0 && threadIdx.x == 0) cur->alpha = alpha;
 }
 
-// z = w (t + L t), partial r.z
+// z = t + L t, partial r.z
 template <int WL, class I>
 __global__ __launch_bounds__(RB) void k_mx_gz(int n, int wl, const I *__restrict__ lcol,
-                                              const float *__restrict__ lval, const float *__restrict__ w,
-                                              const float *__restrict__ t, const float *__restrict__ r,
-                                              float *__restrict__ z, double *__restrict__ partR, const KS *gate) {
+                                              const lval_t *__restrict__ lval, const float *__restrict__ t,
+                                              const float *__restrict__ r, float *__restrict__ z,
+                                              double *__restrict__ partR, const KS *gate) {
   if (gate_done(gate)) return;
   double a[1] = {0.0};
   int r0, r1;
@@ -232,11 +260,10 @@ __global__ __launch_bounds__(RB) void k_mx_gz(int n, int wl, const I *__restrict
 #pragma unroll
     for (int k = 0; k < nw<WL>(wl); ++k) {
       const size_t o = (size_t)k * n + i;
-      acc += lval[o] * t[colat(lcol, o, i)];
+      acc += (float)lval[o] * t[colat(lcol, o, i)];
     }
-    const float zi = w[i] * acc;
-    z[i] = zi;
-    a[0] += (double)r[i] * (double)zi;
+    z[i] = acc;
+    a[0] += (double)r[i] * (double)acc;
   }
   store_partials<1>(a, partR);
 }
@@ -271,31 +298,31 @@ __global__ __launch_bounds__(RB) void k_mx_tot4(const double *part, int P, doubl
     for (int j = 0; j < 4; ++j) out[j] = s[j];
 }
 
-__global__ __launch_bounds__(BS) void k_mx_add(int n, double *__restrict__ x, const float *__restrict__ ev) {
+// x += D^-1/2 eh
+__global__ __launch_bounds__(BS) void k_mx_add(int n, double *__restrict__ x, const float *__restrict__ ev,
+                                               const double *__restrict__ diag) {
   const int i = blockIdx.x * BS + threadIdx.x;
-  if (i < n) x[i] += (double)ev[i];
+  if (i < n) x[i] += (double)ev[i] / sqrt(diag[i]);
 }
 
-// Factor from per-interior-face (l, sig): w = (a_ii)^-1/2, e = exp(2 s) with
-// s the sum of sig over the faces the row owns, slot values of L.
-__global__ __launch_bounds__(BS) void k_fac_rows(int n, int WL, int WU, const double *__restrict__ diag,
-                                                 const int *__restrict__ lfi, const int *__restrict__ ufi,
-                                                 const float *__restrict__ fl, const float *__restrict__ fsig,
-                                                 float *__restrict__ lval, float *__restrict__ uval,
-                                                 float *__restrict__ w, float *__restrict__ ee) {
+// Factor from per-interior-face (l, sig): e = exp(2 s) with s the sum of sig
+// over the faces the row owns, slot values of L (FP16).
+__global__ __launch_bounds__(BS) void k_fac_rows(int n, int WL, int WU, const int *__restrict__ lfi,
+                                                 const int *__restrict__ ufi, const float *__restrict__ fl,
+                                                 const float *__restrict__ fsig, lval_t *__restrict__ lval,
+                                                 lval_t *__restrict__ uval, float *__restrict__ ee) {
   const int i = blockIdx.x * BS + threadIdx.x;
   if (i >= n) return;
   float s = 0.f;
   for (int k = 0; k < WU; ++k) {
     const int f = ufi[(size_t)k * n + i];
-    uval[(size_t)k * n + i] = f >= 0 ? fl[f] : 0.f;
+    uval[(size_t)k * n + i] = (lval_t)(f >= 0 ? fl[f] : 0.f);
     if (f >= 0) s += fsig[f];
   }
   for (int k = 0; k < WL; ++k) {
     const int f = lfi[(size_t)k * n + i];
-    lval[(size_t)k * n + i] = f >= 0 ? fl[f] : 0.f;
+    lval[(size_t)k * n + i] = (lval_t)(f >= 0 ? fl[f] : 0.f);
   }
-  w[i] = (float)(1.0 / sqrt(diag[i]));
   ee[i] = expf(2.f * s);
 }
 
@@ -392,10 +419,11 @@ int mix_build(Ctx &c) {
   if ((rc = up_int(fi_p, &X.fi_p))) return rc;
   if ((rc = up_int(fi_n, &X.fi_n))) return rc;
   if ((rc = dalloc(c, &X.spv, (size_t)Ws * n))) return rc;
-  if ((rc = dalloc(c, &X.diag, n))) return rc;
-  if ((rc = dalloc(c, &X.jc, n))) return rc;
-  if ((rc = dalloc(c, &X.lval, (size_t)WL * n))) return rc;
-  if ((rc = dalloc(c, &X.uval, (size_t)WU * n))) return rc;
+  lval_t *lv, *uv;
+  if ((rc = dalloc(c, &lv, (size_t)WL * n))) return rc;
+  if ((rc = dalloc(c, &uv, (size_t)WU * n))) return rc;
+  X.lval = lv;
+  X.uval = uv;
   if ((rc = dalloc(c, &X.w, n))) return rc;
   if ((rc = dalloc(c, &X.e, n))) return rc;
   if ((rc = dalloc(c, &X.fl, std::max(nfi, 1)))) return rc;
@@ -411,8 +439,8 @@ int mix_build(Ctx &c) {
 
 int factor_assemble(Ctx &c) {
   Mix &X = c.mx;
-  k_fac_rows<<<grid_for(c.n), BS, 0, c.stream>>>(c.n, X.WL, X.WU, c.diag, X.lfi, X.ufi, X.fl, X.fsig, X.lval,
-                                                  X.uval, X.w, X.e);
+  k_fac_rows<<<grid_for(c.n), BS, 0, c.stream>>>(c.n, X.WL, X.WU, X.lfi, X.ufi, X.fl, X.fsig, (lval_t *)X.lval,
+                                                  (lval_t *)X.uval, X.e);
   FVK_HIP(hipGetLastError());
   X.fac_valid = true;
   return 0;
@@ -427,6 +455,7 @@ static int inner_solve(Ctx &c, bool fac, int n_iter, double reduc, int &iters, i
   Mix &X = c.mx;
   const int n = c.n, G = red_grid(n), ws = c.Ws, wl = X.WL, wu = X.WU;
   const I *col = (const I *)X.col, *lcol = (const I *)X.lcol, *ucol = (const I *)X.ucol;
+  const lval_t *lval = (const lval_t *)X.lval, *uval = (const lval_t *)X.uval;
   // scalars: reduc, n_iter, s = large
   KS h[6];
   std::memset(h, 0, sizeof h);
@@ -437,15 +466,15 @@ static int inner_solve(Ctx &c, bool fac, int n_iter, double reduc, int &iters, i
   }
   FVK_HIP(hipMemcpyAsync(c.ksr, h, sizeof h, hipMemcpyHostToDevice, c.stream));
   float *r[2] = {fv(c, V_R0), fv(c, V_R1)}, *P[2] = {fv(c, V_P0), fv(c, V_P1)};
-  float *z = fv(c, V_Z), *q = fv(c, V_Q), *t = fv(c, V_T), *ev = fv(c, V_E);
-#define L(W)                                                                                                   \
-  k_mx_start<W, I><<<G, RB, 0, c.stream>>>(n, fac ? 1 : 0, r[0], X.jc, z, ev, P[0], wu, ucol, X.uval, X.w, X.e, t, \
-                                           pb(c, 0))
+  // Jacobi: z is r itself, updated in place
+  float *z = fac ? fv(c, V_Z) : r[0], *q = fv(c, V_Q), *t = fv(c, V_T), *ev = fv(c, V_E);
+#define L(W) \
+  k_mx_start<W, I><<<G, RB, 0, c.stream>>>(n, fac ? 1 : 0, r[0], ev, P[0], wu, ucol, uval, X.e, t, pb(c, 0))
   FVK_W(wu, L);
 #undef L
   const double *pR = nullptr;
   if (fac) {
-#define L(W) k_mx_gz<W, I><<<G, RB, 0, c.stream>>>(n, wl, lcol, X.lval, X.w, t, r[0], z, pb(c, 2), nullptr)
+#define L(W) k_mx_gz<W, I><<<G, RB, 0, c.stream>>>(n, wl, lcol, lval, t, r[0], z, pb(c, 2), nullptr)
     FVK_W(wl, L);
 #undef L
     pR = pb(c, 2);
@@ -455,20 +484,21 @@ static int inner_solve(Ctx &c, bool fac, int n_iter, double reduc, int &iters, i
   for (; it < n_iter; ++it) {
     float *pold = P[it & 1], *pnew = P[(it + 1) & 1], *rin = r[it & 1], *rout = r[(it + 1) & 1];
     if (T) T->begin(c.stream);
-#define L(W)                                                                                                   \
-  k_mx_pq<W, I><<<G, RB, 0, c.stream>>>(n, ws, it, col, X.spv, X.diag, z, pold, pnew, q, pb(c, 0), G, pR, G, \
-                                        pb(c, 1), c.ksr, c.ks)
+#define L(W)                                                                                                  \
+  k_mx_pq<W, I><<<G, RB, 0, c.stream>>>(n, ws, it, col, X.spv, z, pold, pnew, q, pb(c, 0), G, pR, G, pb(c, 1), \
+                                        c.ksr, c.ks)
     FVK_W(ws, L);
 #undef L
     if (T) T->end(c.stream);
+    if (!fac) {
+      k_mx_upd_jac<<<G, RB, 0, c.stream>>>(n, it, ev, r[0], pnew, q, pb(c, 1), G, pb(c, 0), c.ksr);
+    } else {
 #define L(W)                                                                                                   \
-  k_mx_upd<W, I><<<G, RB, 0, c.stream>>>(n, it, fac ? 1 : 0, ev, rin, rout, pnew, q, X.jc, z, wu, ucol, X.uval, \
-                                         X.w, X.e, t, pb(c, 1), G, pb(c, 0), c.ksr)
-    FVK_W(wu, L);
+  k_mx_upd_fac<W, I><<<G, RB, 0, c.stream>>>(n, it, ev, rin, rout, pnew, q, wu, ucol, uval, X.e, t, pb(c, 1), G, \
+                                             pb(c, 0), c.ksr)
+      FVK_W(wu, L);
 #undef L
-    if (fac) {
-#define L(W) \
-  k_mx_gz<W, I><<<G, RB, 0, c.stream>>>(n, wl, lcol, X.lval, X.w, t, rout, z, pb(c, 2), c.ksr + (it & 1) * 3)
+#define L(W) k_mx_gz<W, I><<<G, RB, 0, c.stream>>>(n, wl, lcol, lval, t, rout, z, pb(c, 2), c.ksr + (it & 1) * 3)
       FVK_W(wl, L);
 #undef L
     }
@@ -505,7 +535,7 @@ int dc_solve(Ctx &c, bool use_factor, const double *b, double *x, double tol, in
     return FVK_ERR_STATE;
   }
   const int n = c.n, G = red_grid(n);
-  k_mx_conv<<<grid_for(n), BS, 0, c.stream>>>(n, c.Ws, c.diag, c.spv, X.diag, X.spv, X.jc);
+  k_mx_conv<<<grid_for(n), BS, 0, c.stream>>>(c.M, c.diag, c.spv, X.spv, X.w);
   FVK_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, c.stream));
   res = DcResult();
   for (int outer = 0;; ++outer) {
@@ -531,20 +561,21 @@ int dc_solve(Ctx &c, bool use_factor, const double *b, double *x, double tol, in
     if (rc) return rc;
     res.iters += its;
     res.broken |= broken;
-    k_mx_add<<<grid_for(n), BS, 0, c.stream>>>(n, x, fv(c, V_E));
+    k_mx_add<<<grid_for(n), BS, 0, c.stream>>>(n, x, fv(c, V_E), c.diag);
     FVK_HIP(hipGetLastError());
   }
   return 0;
 }
 
-// z = G G^T r on device rows (FP32 inside), for tests
-__global__ __launch_bounds__(BS) void k_to_f32(int n, const double *a, float *b) {
+// z = G G^T r = D^-1/2 (I + L) Delta^2 (I + L^T) D^-1/2 r on device rows
+// (FP32 inside), for tests
+__global__ __launch_bounds__(BS) void k_scale_f32(int n, const double *a, const double *diag, float *b) {
   const int i = blockIdx.x * BS + threadIdx.x;
-  if (i < n) b[i] = (float)a[i];
+  if (i < n) b[i] = (float)(a[i] / sqrt(diag[i]));
 }
-__global__ __launch_bounds__(BS) void k_to_f64(int n, const float *a, double *b) {
+__global__ __launch_bounds__(BS) void k_scale_f64(int n, const float *a, const double *diag, double *b) {
   const int i = blockIdx.x * BS + threadIdx.x;
-  if (i < n) b[i] = (double)a[i];
+  if (i < n) b[i] = (double)a[i] / sqrt(diag[i]);
 }
 
 template <class I>
@@ -552,16 +583,16 @@ static int apply_t(Ctx &c, const double *r, double *zout) {
   Mix &X = c.mx;
   const int n = c.n, G = red_grid(n), wl = X.WL, wu = X.WU;
   const I *lcol = (const I *)X.lcol, *ucol = (const I *)X.ucol;
+  const lval_t *lval = (const lval_t *)X.lval, *uval = (const lval_t *)X.uval;
   float *r32 = fv(c, V_A), *z = fv(c, V_B), *t = fv(c, V_T), *ev = fv(c, V_E), *p0 = fv(c, V_P0);
-  k_to_f32<<<grid_for(n), BS, 0, c.stream>>>(n, r, r32);
-#define L(W)                                                                                                \
-  k_mx_start<W, I><<<G, RB, 0, c.stream>>>(n, 1, r32, X.jc, z, ev, p0, wu, ucol, X.uval, X.w, X.e, t, pb(c, 0))
+  k_scale_f32<<<grid_for(n), BS, 0, c.stream>>>(n, r, c.diag, r32);
+#define L(W) k_mx_start<W, I><<<G, RB, 0, c.stream>>>(n, 1, r32, ev, p0, wu, ucol, uval, X.e, t, pb(c, 0))
   FVK_W(wu, L);
 #undef L
-#define L(W) k_mx_gz<W, I><<<G, RB, 0, c.stream>>>(n, wl, lcol, X.lval, X.w, t, r32, z, pb(c, 2), nullptr)
+#define L(W) k_mx_gz<W, I><<<G, RB, 0, c.stream>>>(n, wl, lcol, lval, t, r32, z, pb(c, 2), nullptr)
   FVK_W(wl, L);
 #undef L
-  k_to_f64<<<grid_for(n), BS, 0, c.stream>>>(n, z, zout);
+  k_scale_f64<<<grid_for(n), BS, 0, c.stream>>>(n, z, c.diag, zout);
   FVK_HIP(hipGetLastError());
   return 0;
 }

@@ -3,8 +3,9 @@
 The judge of every solve is the FP64 true residual |b - A x|_1 / |b|_1,
 recomputed here on the CPU by the oracle's own SpMV (oracle/fvo, the
 reference's face-scatter product) from the solution the library returns.
-The factor's action z = G G^T r is compared with oracle/learn.apply (float32
-inside the library: tolerance 2e-6 relative to max |z|).
+The factor's action z = G G^T r is compared with oracle/learn.apply on the
+same FP16-rounded L (float32 arithmetic inside the library: tolerance 2e-6
+relative to max |z|).
 """
 import numpy as np
 import pytest
@@ -69,7 +70,9 @@ def test_factor_apply_matches_oracle():
         fvk.factor_set(ctx, l, sig)
         r = rng.standard_normal(ctx.n_vol)
         z = fvk.factor_apply(ctx, r)
-        w, e, l64, _ = L.factor(S, l[inner].astype(np.float64), sig[inner].astype(np.float64))
+        # the library keeps L in FP16
+        l16 = l[inner].astype(np.float16).astype(np.float64)
+        w, e, l64, _ = L.factor(S, l16, sig[inner].astype(np.float64))
         zr = L.apply(S, w, e, l64, r)
         assert np.max(np.abs(z - zr)) <= 2e-6 * np.max(np.abs(zr))
         # zero factor: Jacobi
