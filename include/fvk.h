@@ -265,9 +265,14 @@ typedef struct fvk_flow_cfg {
   int c_sch;        /* 0 CDS, 1 minmod, 2 van Leer, 3 superbee, >3 HRIC */
   double urf_u, urf_p, reduc_u, reduc_p, c_bl;
   double rho, mu;
-  int p_solver;     /* 0: reference PCG recurrence (parity); 1: FP32-inner
-                       defect correction; 2: learned preconditioner */
-  double p_tol;     /* p_solver > 0: FP64 true relative residual target */
+  int p_solver;     /* pressure solves: 0 the reference PCG recurrence
+                       (imat_p, reduc_p, n_cg; parity); to the FP64 relative
+                       L1 residual p_tol from x = 0: 1 FP32-inner defect
+                       correction (Jacobi), 2 the same with the learned
+                       factor of the model set by fvk_flow_set_model (plain
+                       FP64 PCG when it does not converge), 3 plain FP64
+                       Jacobi PCG */
+  double p_tol;
 } fvk_flow_cfg;
 
 typedef struct fvk_outer_hist {
@@ -307,6 +312,15 @@ int fvk_flow_outer(fvk_flow *f, int it, fvk_outer_hist *h);
 /* n outer iterations it0 .. it0+n-1 with the state kept in HBM; h has n
    entries (may be NULL). */
 int fvk_flow_run(fvk_flow *f, int it0, int n, fvk_outer_hist *h);
+/* The same with iteration it0 run eagerly and the others as replays of HIP
+   graphs of one outer iteration (one graph, or one per segment between
+   pressure solves to tolerance, which check convergence from the host).
+   Same results as fvk_flow_run. */
+int fvk_flow_run_graph(fvk_flow *f, int it0, int n, fvk_outer_hist *h);
+/* Model of the learned factor for p_solver 2 (not owned; NULL detaches);
+   train_steps > 0: that many Adam steps of rate lr on every
+   pressure-correction matrix before its factor is built. */
+int fvk_flow_set_model(fvk_flow *f, fvk_model *m, int train_steps, double lr);
 
 #ifdef __cplusplus
 }

@@ -25,9 +25,6 @@ namespace fvk {
 static thread_local std::string g_err;
 void set_error(const std::string &msg) { g_err = msg; }
 
-int mom_setup(Flow &F, int it, bool &skipped);
-int pp_setup(Flow &F, int it);
-int outer_begin(Flow &F);
 
 // ---------------------------------------------------------------- helpers
 struct Topo {  // host topology + geometry, 0-based, vectors stride 3
@@ -876,7 +873,8 @@ int fvk_bench_copy(int device, long nbytes, int reps, double *sec) {
 int fvk_flow_create(fvk_ctx *cc, const fvk_flow_cfg *cfg, fvk_flow **out) {
   if (!cc || !cfg || !out) { set_error("null argument"); return FVK_ERR_ARG; }
   if (cfg->imat_u < 0 || cfg->imat_u > 2 || cfg->imat_p < 0 || cfg->imat_p > 2 || cfg->n_grad < 1 ||
-      cfg->urf_u <= 0.0 || cfg->n_bicg < 0 || cfg->n_cg < 0) {
+      cfg->urf_u <= 0.0 || cfg->n_bicg < 0 || cfg->n_cg < 0 || cfg->p_solver < 0 || cfg->p_solver > 3 ||
+      (cfg->p_solver > 0 && !(cfg->p_tol > 0.0 && cfg->p_tol < 1.0))) {
     set_error("bad flow configuration");
     return FVK_ERR_ARG;
   }
@@ -940,12 +938,17 @@ int fvk_flow_create(fvk_ctx *cc, const fvk_flow_cfg *cfg, fvk_flow **out) {
   return FVK_OK;
 }
 
-void fvk_flow_destroy(fvk_flow *f) { delete f; }  // device memory is owned by the context
+void fvk_flow_destroy(fvk_flow *f) {
+  if (!f) return;
+  for (auto g : f->f.graphs) hipGraphExecDestroy(g);
+  delete f;  // device memory is owned by the context
+}
 
 int fvk_flow_configure(fvk_flow *ff, const fvk_flow_cfg *cfg) {
   if (!ff || !cfg) { set_error("null argument"); return FVK_ERR_ARG; }
   if (cfg->imat_u < 0 || cfg->imat_u > 2 || cfg->imat_p < 0 || cfg->imat_p > 2 || cfg->n_grad < 1 ||
-      cfg->urf_u <= 0.0 || cfg->n_bicg < 0 || cfg->n_cg < 0) {
+      cfg->urf_u <= 0.0 || cfg->n_bicg < 0 || cfg->n_cg < 0 || cfg->p_solver < 0 || cfg->p_solver > 3 ||
+      (cfg->p_solver > 0 && !(cfg->p_tol > 0.0 && cfg->p_tol < 1.0))) {
     set_error("bad flow configuration");
     return FVK_ERR_ARG;
   }
@@ -1064,6 +1067,20 @@ int fvk_flow_outer(fvk_flow *ff, int it, fvk_outer_hist *h) {
   GUARD(mom_setup(F, it, skipped));
   GUARD(pp_setup(F, it));
   return read_hist(F, h);
+}
+
+int fvk_flow_set_model(fvk_flow *ff, fvk_model *m, int train_steps, double lr) {
+  if (!ff || train_steps < 0 || (train_steps > 0 && !(lr > 0))) { set_error("bad argument"); return FVK_ERR_ARG; }
+  ff->f.model = m;
+  ff->f.train_steps = m ? train_steps : 0;
+  ff->f.train_lr = lr;
+  return FVK_OK;
+}
+
+int fvk_flow_run_graph(fvk_flow *ff, int it0, int n, fvk_outer_hist *h) {
+  if (!ff || it0 < 1 || n < 0) { set_error("bad argument"); return FVK_ERR_ARG; }
+  int replayed = 0;
+  return flow_run_graph(ff->f, it0, n, h, replayed);
 }
 
 int fvk_flow_run(fvk_flow *ff, int it0, int n, fvk_outer_hist *h) {

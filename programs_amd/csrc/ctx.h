@@ -110,6 +110,9 @@ int dalloc(Ctx &c, T **p, size_t n) {
 int precond_build(Ctx &c, int imat);
 int precond_apply(Ctx &c, int imat, int nvc, const double *b, double *x, const KS *gate);
 int spmv(Ctx &c, int nvc, const double *x, double *y);
+// a[0..n) = v and b = a as kernels (capturable in a HIP graph)
+int fill(Ctx &c, double *a, size_t n, double v);
+int copy_d(Ctx &c, const double *a, double *b, size_t n);
 int residual(Ctx &c, int nvc, const double *x, double *r, bool not_r0);
 // Times one kernel per iteration with event pairs on the launch stream.
 struct KTimer {
@@ -149,7 +152,7 @@ int true_residual(Ctx &c, const double *x, const double *b, double *dev_out4, do
 // mixed.hip
 struct DcResult {
   int iters = 0, outer = 0, broken = 0;
-  double res_l1 = 0.0, res_l2 = 0.0;
+  double res_l1 = 0.0, res_l2 = 0.0, b_l1 = 0.0;
   bool converged = false;
 };
 int mix_build(Ctx &c);
@@ -160,6 +163,11 @@ int factor_apply(Ctx &c, const double *r, double *z);
 // rows); use_factor: G G^T preconditioner (else Jacobi)
 int dc_solve(Ctx &c, bool use_factor, const double *b, double *x, double tol, int max_iter, DcResult &res,
              KTimer *T);
+
+// learn.hip: factor of the current matrix from the network, and Adam steps
+// on the current matrix
+int learned_factor(Ctx &c, fvk_model *m);
+int learned_train(Ctx &c, fvk_model *m, int steps, double lr, double *last_loss);
 
 // flow.hip
 // solv_grad: n_iter Green-Gauss sweeps; g0 is scratch of the size of g;

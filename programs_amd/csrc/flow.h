@@ -1,5 +1,7 @@
 // Device state of a SIMPLE flow on an fvk context.
 #pragma once
+#include <vector>
+
 #include "ctx.h"
 
 namespace fvk {
@@ -13,6 +15,8 @@ struct FlowSc {
   double h_reduc, h_redpp, h_redpp_sum, h_red_p, h_red_p_sum;
   double res0_n_u[3], res0_n_pp, res0_n_p;
   int it_u[3], it_pp, it_p, diverged;
+  int hist_k;     // next slot of the device history (graph replays)
+  int fallbacks;  // pressure solves that fell back to plain PCG
 };
 
 struct Flow {
@@ -36,6 +40,24 @@ struct Flow {
   int *oc_cell = nullptr, *oc_face = nullptr, *cell_oc = nullptr;
   double *oc_u = nullptr;
   int row_of_cell1 = 0;
+  // launches are being captured into a HIP graph: no host round trips
+  bool capture = false;
+  // learned pressure preconditioner (p_solver 2) and optional training on
+  // every pressure-correction matrix the flow assembles
+  fvk_model *model = nullptr;
+  int train_steps = 0;
+  double train_lr = 1e-3;
+  // graphs of one outer iteration (segments between host-driven solves)
+  std::vector<hipGraphExec_t> graphs;
+  FlowSc *hist = nullptr;
+  int hist_cap = 0;
 };
+
+int mom_setup(Flow &F, int it, bool &skipped);
+int pp_setup(Flow &F, int it);
+int outer_begin(Flow &F);
+// n outer iterations from it0: it0 eagerly, the rest as replays of HIP
+// graphs of one outer iteration; history into h (n entries, may be null)
+int flow_run_graph(Flow &F, int it0, int n, fvk_outer_hist *h, int &replayed);
 
 }  // namespace fvk

@@ -645,23 +645,47 @@ __global__ __launch_bounds__(BS) void k_outer_begin(FlowSc *sc) {
 // ------------------------------------------------------------- host driver
 
 
+// The flow's own stores of the Krylov scalars a pressure solve to tolerance
+// leaves for k_pp_norm: res0_n = |b|_1 (x0 = 0) and the iteration count.
+__global__ void k_ks_store(KS *ks, double res0_n, int iters, double res_n) {
+  if (threadIdx.x != 0) return;
+  ks[0].res0_n = res0_n;
+  ks[0].iter = iters;
+  ks[0].res_n = res_n;
+  ks[0].done = 1;
+}
+
+__global__ void k_hist_push(FlowSc *sc, FlowSc *hist, int cap) {
+  if (threadIdx.x != 0) return;
+  const int k = sc->hist_k;
+  if (k < cap) hist[k] = *sc;
+  sc->hist_k = k + 1;
+}
+
+__global__ void k_hist_reset(FlowSc *sc) {
+  if (threadIdx.x == 0) sc->hist_k = 0;
+}
+
 int mom_setup(Flow &F, int it, bool &skipped) {
   Ctx &c = *F.c;
   const int n = c.n, nf = c.nf, nvc = F.nvc;
   MomPar P{F.cfg.rho, F.cfg.mu, F.cfg.c_bl, F.cfg.urf_u, F.cfg.c_sch};
   skipped = false;
-  FVK_HIP(hipMemsetAsync(&F.sc->phi_mx_cell, 0, 2 * sizeof(double), c.stream));
+  if (fill(c, &F.sc->phi_mx_cell, 2, 0.0)) return FVK_ERR_HIP;
   k_absmax<<<grid_for(n > c.nbf ? n : c.nbf) < 2048 ? grid_for(n > c.nbf ? n : c.nbf) : 2048, BS, 0, c.stream>>>(n, nvc, F.u, c.nbf, F.ub, &F.sc->phi_mx_cell,
                                                                  &F.sc->phi_mx_all);
-  // the early return when every velocity is zero (cfd_setup.f90:170-171)
-  double mx[2];
-  FVK_HIP(hipMemcpyAsync(mx, &F.sc->phi_mx_cell, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-  FVK_HIP(hipStreamSynchronize(c.stream));
-  if (mx[1] < SMALL) {
-    FVK_HIP(hipMemsetAsync(c.diag, 0, sizeof(double) * n, c.stream));
-    FVK_HIP(hipMemsetAsync(F.cs, 0, sizeof(double) * n, c.stream));
-    skipped = true;
-    return 0;
+  // the early return when every velocity is zero (cfd_setup.f90:170-171);
+  // a captured iteration follows one that did not take it, and with fixed
+  // non-zero boundary values it is never taken again
+  if (!F.capture) {
+    double mx[2];
+    FVK_HIP(hipMemcpyAsync(mx, &F.sc->phi_mx_cell, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    FVK_HIP(hipStreamSynchronize(c.stream));
+    if (mx[1] < SMALL) {
+      if (fill(c, c.diag, n, 0.0) || fill(c, F.cs, n, 0.0)) return FVK_ERR_HIP;
+      skipped = true;
+      return 0;
+    }
   }
   int rc = grad(c, nvc, F.u, F.ub, F.ddx, F.cfg.n_grad, 0, &F.sc->phi_mx_cell, F.g0);
   if (rc) return rc;
@@ -688,9 +712,11 @@ int mom_setup(Flow &F, int it, bool &skipped) {
   return 0;
 }
 
-int pp_setup(Flow &F, int it) {
+// ---- pressure correction (cfd_setup_pp), in phases split at its solves
+
+static int pp_assemble(Flow &F) {
   Ctx &c = *F.c;
-  const int n = c.n, nf = c.nf, nvc = F.nvc;
+  const int n = c.n, nf = c.nf;
   k_plr<<<grid_for(n), BS, 0, c.stream>>>(n, c.diag, F.pl_r);
   k_crat<<<1, 1, 0, c.stream>>>(F.sc);
   if (c.ndx == 3)
@@ -698,11 +724,67 @@ int pp_setup(Flow &F, int it) {
   else
     k_pp_face<2><<<grid_for(nf), BS, 0, c.stream>>>(c.M, F.p, F.dpdx, F.pl_r, F.frc_dns, F.spd, c.ngp, c.ngn, F.cpp, F.sc);
   k_pp_cell<<<grid_for(n), BS, 0, c.stream>>>(c.M, c.ngp, c.ngn, F.cpp, F.cs, c.diag, c.spv, F.rhs);
-  FVK_HIP(hipMemsetAsync(F.pc, 0, sizeof(double) * n, c.stream));
+  if (fill(c, F.pc, n, 0.0)) return FVK_ERR_HIP;
+  FVK_HIP(hipGetLastError());
   c.pc_valid = false;
-  const int chunk = F.cfg.n_cg > 64 ? 64 : 0;  // long solves stop queueing once converged
-  int rc = krylov(c, 0, F.cfg.imat_p, F.cfg.reduc_p, F.cfg.n_cg, 1, F.pc, F.rhs, true, chunk);
-  if (rc) return rc;
+  return 0;
+}
+
+// The pressure-correction (or SIMPLER pressure) solve: the reference's PCG
+// (cfd_setup.f90:662,742) with p_solver 0, else a solve from x = 0 to an
+// FP64 true relative L1 residual p_tol: 1 FP32-inner defect correction,
+// 2 the same with the learned factor (plain FP64 PCG when it fails),
+// 3 plain FP64 Jacobi PCG.
+// solves to tolerance check convergence from the host: graphs stop there
+static bool pp_host_driven(const Flow &F) { return F.cfg.p_solver != 0; }
+
+static int pp_solve(Flow &F, double *x, double *b) {
+  Ctx &c = *F.c;
+  const int chunk = F.capture ? 0 : 64;
+  const int cap = 200000;
+  switch (F.cfg.p_solver) {
+    case 0:
+      return krylov(c, 0, F.cfg.imat_p, F.cfg.reduc_p, F.cfg.n_cg, 1, x, b, true,
+                    F.cfg.n_cg > 64 ? chunk : 0);
+    case 3:
+      return krylov(c, 0, 0, F.cfg.p_tol, cap, 1, x, b, true, chunk);
+    default: {
+      int rc;
+      const bool fac = F.cfg.p_solver == 2;
+      if (fac) {
+        if (!F.model) { set_error("p_solver 2 needs a model (fvk_flow_set_model)"); return FVK_ERR_STATE; }
+        if (F.train_steps > 0 && (rc = learned_train(c, F.model, F.train_steps, F.train_lr, nullptr))) return rc;
+        if ((rc = learned_factor(c, F.model))) return rc;
+      }
+      DcResult R;
+      if ((rc = dc_solve(c, fac, b, x, F.cfg.p_tol, cap, R, nullptr))) return rc;
+      if (!R.converged) {
+        // the plain FP64 Jacobi PCG from x = 0 takes over
+        if (fill(c, x, c.n, 0.0)) return FVK_ERR_HIP;
+        if ((rc = krylov(c, 0, 0, F.cfg.p_tol, cap, 1, x, b, true, chunk))) return rc;
+        KS h;
+        FVK_HIP(hipMemcpyAsync(&h, c.ks, sizeof h, hipMemcpyDeviceToHost, c.stream));
+        FVK_HIP(hipStreamSynchronize(c.stream));
+        k_ks_store<<<1, 1, 0, c.stream>>>(c.ks, R.b_l1, R.iters + h.iter, h.res_n);
+        FlowSc s;
+        FVK_HIP(hipMemcpy(&s, F.sc, sizeof s, hipMemcpyDeviceToHost));
+        s.fallbacks += 1;
+        FVK_HIP(hipMemcpy(F.sc, &s, sizeof s, hipMemcpyHostToDevice));
+      } else {
+        k_ks_store<<<1, 1, 0, c.stream>>>(c.ks, R.b_l1, R.iters, R.res_l1);
+      }
+      FVK_HIP(hipGetLastError());
+      return 0;
+    }
+  }
+}
+
+// after the p' solve: norms, relaxed update, p' gradient, velocity and flux
+// corrections (and the SIMPLER pressure right-hand side)
+static int pp_post1(Flow &F, int it) {
+  Ctx &c = *F.c;
+  const int n = c.n, nf = c.nf, nvc = F.nvc;
+  int rc;
   if ((rc = sum_to(c, F.rhs, F.red_scratch))) return rc;
   k_pp_norm<<<1, 1, 0, c.stream>>>(it, 0, c.ks, F.red_scratch, F.sc);
   if (!F.cfg.simpler) k_p_relax<<<grid_for(n), BS, 0, c.stream>>>(n, F.cfg.urf_p, F.pc, F.row_of_cell1, F.p);
@@ -713,22 +795,165 @@ int pp_setup(Flow &F, int it) {
     k_flux_face<3><<<grid_for(nf), BS, 0, c.stream>>>(c.M, F.pc, F.ddxp, c.ngp, F.frc_dns, F.spd, F.cpp);
   else
     k_flux_face<2><<<grid_for(nf), BS, 0, c.stream>>>(c.M, F.pc, F.ddxp, c.ngp, F.frc_dns, F.spd, F.cpp);
-  if (F.cfg.simpler) {
-    k_flux_cell<<<grid_for(n), BS, 0, c.stream>>>(c.M, F.cpp, F.rhs, F.rhs_p);
-    rc = krylov(c, 0, F.cfg.imat_p, F.cfg.reduc_p, F.cfg.n_cg, 1, F.p, F.rhs_p, true, chunk);
-    if (rc) return rc;
-    if ((rc = sum_to(c, F.rhs_p, F.red_scratch))) return rc;
-    k_pp_norm<<<1, 1, 0, c.stream>>>(it, 1, c.ks, F.red_scratch, F.sc);
-  }
-  rc = grad(c, 1, F.p, nullptr, F.dpdx, F.cfg.n_grad, 2, nullptr, F.g0);
-  if (rc) return rc;
+  if (F.cfg.simpler) k_flux_cell<<<grid_for(n), BS, 0, c.stream>>>(c.M, F.cpp, F.rhs, F.rhs_p);
   FVK_HIP(hipGetLastError());
   return 0;
+}
+
+// after the SIMPLER pressure solve: its norms
+static int pp_post2(Flow &F, int it) {
+  Ctx &c = *F.c;
+  int rc;
+  if ((rc = sum_to(c, F.rhs_p, F.red_scratch))) return rc;
+  k_pp_norm<<<1, 1, 0, c.stream>>>(it, 1, c.ks, F.red_scratch, F.sc);
+  FVK_HIP(hipGetLastError());
+  return 0;
+}
+
+static int pp_final(Flow &F) {
+  Ctx &c = *F.c;
+  return grad(c, 1, F.p, nullptr, F.dpdx, F.cfg.n_grad, 2, nullptr, F.g0);
+}
+
+int pp_setup(Flow &F, int it) {
+  int rc;
+  if ((rc = pp_assemble(F))) return rc;
+  if ((rc = pp_solve(F, F.pc, F.rhs))) return rc;
+  if ((rc = pp_post1(F, it))) return rc;
+  if (F.cfg.simpler) {
+    if ((rc = pp_solve(F, F.p, F.rhs_p))) return rc;
+    if ((rc = pp_post2(F, it))) return rc;
+  }
+  return pp_final(F);
 }
 
 int outer_begin(Flow &F) {
   k_outer_begin<<<1, 1, 0, F.c->stream>>>(F.sc);
   FVK_HIP(hipGetLastError());
+  return 0;
+}
+
+// ---- HIP graphs of one outer iteration
+
+// Segments of an outer iteration between host-driven solves (p_solver > 0):
+// s = 0: outer_begin, momentum, p' assembly [, p' solve, post1[, p solve,
+// post2], final, history] as far as no host-driven solve intervenes.
+static int record_segment(Flow &F, int it, int seg) {
+  int rc;
+  bool skipped;
+  const bool host = pp_host_driven(F);
+  if (seg == 0) {
+    if ((rc = outer_begin(F))) return rc;
+    if ((rc = mom_setup(F, it, skipped))) return rc;
+    if ((rc = pp_assemble(F))) return rc;
+    if (host) return 0;
+    if ((rc = pp_solve(F, F.pc, F.rhs))) return rc;
+  }
+  if (seg <= 1) {
+    if ((rc = pp_post1(F, it))) return rc;
+    if (F.cfg.simpler) {
+      if (host) return 0;
+      if ((rc = pp_solve(F, F.p, F.rhs_p))) return rc;
+    }
+  }
+  if (F.cfg.simpler && (seg <= 1 || seg == 2)) {
+    if ((rc = pp_post2(F, it))) return rc;
+  }
+  if ((rc = pp_final(F))) return rc;
+  k_hist_push<<<1, 1, 0, F.c->stream>>>(F.sc, F.hist, F.hist_cap);
+  FVK_HIP(hipGetLastError());
+  return 0;
+}
+
+static int capture(Flow &F, int it, int seg, hipGraphExec_t &exec) {
+  Ctx &c = *F.c;
+  hipGraph_t g = nullptr;
+  F.capture = true;
+  FVK_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+  int rc = record_segment(F, it, seg);
+  hipError_t e = hipStreamEndCapture(c.stream, &g);
+  F.capture = false;
+  if (rc) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) { set_error(std::string("graph capture: ") + hipGetErrorString(e)); return FVK_ERR_HIP; }
+  e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) { set_error(std::string("graph instantiate: ") + hipGetErrorString(e)); return FVK_ERR_HIP; }
+  return 0;
+}
+
+int flow_run_graph(Flow &F, int it0, int n, fvk_outer_hist *h, int &replayed) {
+  Ctx &c = *F.c;
+  replayed = 0;
+  if (n <= 0) return 0;
+  // history slots on the device
+  if (F.hist_cap < n) {
+    FlowSc *hp = nullptr;
+    int rc = dalloc(c, &hp, (size_t)n);
+    if (rc) return rc;
+    F.hist = hp;
+    F.hist_cap = n;
+  }
+  for (auto g : F.graphs) hipGraphExecDestroy(g);
+  F.graphs.clear();
+  k_hist_reset<<<1, 1, 0, c.stream>>>(F.sc);
+  // iteration it0 eagerly: it can take the all-zero early return, and it
+  // sets the first-iteration normalisations
+  int rc;
+  bool skipped = false;
+  if ((rc = outer_begin(F))) return rc;
+  if ((rc = mom_setup(F, it0, skipped))) return rc;
+  if ((rc = pp_setup(F, it0))) return rc;
+  k_hist_push<<<1, 1, 0, c.stream>>>(F.sc, F.hist, F.hist_cap);
+  FVK_HIP(hipGetLastError());
+  int it = it0 + 1;
+  if (skipped) {
+    // keep iterating eagerly until the flow moves
+    for (; it < it0 + n && skipped; ++it) {
+      if ((rc = outer_begin(F)) || (rc = mom_setup(F, it, skipped)) || (rc = pp_setup(F, it))) return rc;
+      k_hist_push<<<1, 1, 0, c.stream>>>(F.sc, F.hist, F.hist_cap);
+    }
+  }
+  if (it < it0 + n) {
+    // captured once with an iteration number > 1 (only it == 1 changes what
+    // the kernels do), replayed for the remaining iterations
+    const bool host = pp_host_driven(F);
+    const int nseg = host ? (F.cfg.simpler ? 3 : 2) : 1;
+    F.graphs.resize(nseg, nullptr);
+    for (int s = 0; s < nseg; ++s)
+      if ((rc = capture(F, it, s, F.graphs[s]))) return rc;
+    for (; it < it0 + n; ++it) {
+      FVK_HIP(hipGraphLaunch(F.graphs[0], c.stream));
+      if (host) {
+        if ((rc = pp_solve(F, F.pc, F.rhs))) return rc;
+        FVK_HIP(hipGraphLaunch(F.graphs[1], c.stream));
+        if (F.cfg.simpler) {
+          if ((rc = pp_solve(F, F.p, F.rhs_p))) return rc;
+          FVK_HIP(hipGraphLaunch(F.graphs[2], c.stream));
+        }
+      }
+      ++replayed;
+    }
+  }
+  // history
+  std::vector<FlowSc> hs(n);
+  FVK_HIP(hipStreamSynchronize(c.stream));
+  FVK_HIP(hipMemcpy(hs.data(), F.hist, sizeof(FlowSc) * n, hipMemcpyDeviceToHost));
+  for (int k = 0; k < n; ++k) {
+    const FlowSc &s = hs[k];
+    if (h) {
+      fvk_outer_hist &o = h[k];
+      o.reduc = s.h_reduc; o.redpp = s.h_redpp; o.redpp_sum = s.h_redpp_sum;
+      o.red_p = s.h_red_p; o.red_p_sum = s.h_red_p_sum; o.mx_res_n = s.mx_res_n;
+      for (int q = 0; q < 3; ++q) { o.it_u[q] = s.it_u[q]; o.res0_n_u[q] = s.res0_n_u[q]; }
+      o.it_pp = s.it_pp; o.it_p = s.it_p;
+      o.res0_n_pp = s.res0_n_pp;
+      o.res0_n_p = s.res0_n_p;
+    }
+    if (s.diverged) { set_error("momentum residual exceeded large"); return FVK_ERR_DIVERGED; }
+  }
   return 0;
 }
 

@@ -23,6 +23,7 @@
 // indexed by iteration parity, so a kernel never rewrites a slot that its
 // own blocks read.  The order differs from the reference's sequential sums
 // by rounding only.
+#include <algorithm>
 #include <cstring>
 
 #include "cgcore.h"
@@ -521,6 +522,48 @@ __global__ __launch_bounds__(RB) void k_totals4(const double *part, int P, doubl
     for (int j = 0; j < 4; ++j) out[j] = s[j];
 }
 
+// Fills and copies as kernels, so that a launch sequence can be captured
+// into a HIP graph (no memset or host-to-device copy nodes).
+__global__ __launch_bounds__(BS) void k_fill(size_t n, double *__restrict__ a, double v) {
+  for (size_t i = blockIdx.x * (size_t)BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) a[i] = v;
+}
+__global__ __launch_bounds__(BS) void k_copy_d(size_t n, const double *__restrict__ a, double *__restrict__ b) {
+  for (size_t i = blockIdx.x * (size_t)BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) b[i] = a[i];
+}
+// Both ring slots of the Krylov scalars: reduc, n_iter and the recurrences'
+// initial values (solv_cg.f90:28 s = large; solv_bicg.f90:30 w_sc = 1)
+__global__ void k_ks_init(KS *ring, int nvc, double reduc, int n_iter, int bicg) {
+  const int t = threadIdx.x;
+  if (t >= 6) return;
+  KS k;
+  k.res0_n = k.res_n = k.s = k.alpha = k.beta = 0.0;
+  k.w1 = k.w2 = k.w3 = k.w4 = k.w5 = k.w6 = 0.0;
+  k.reduc = 0.0;
+  k.iter = k.n_iter = k.done = k.pad = 0;
+  if (t % 3 < nvc) {
+    k.reduc = reduc;
+    k.n_iter = n_iter;
+    if (bicg) k.w2 = k.w3 = k.w5 = 1.0;
+    else k.s = 1.0 / SMALL;  // large (precision.f90:9)
+  }
+  ring[t] = k;
+}
+
+int fill(Ctx &c, double *a, size_t n, double v) {
+  if (n == 0) return 0;
+  const size_t g = std::min<size_t>((n + BS - 1) / BS, 4096);
+  k_fill<<<(int)g, BS, 0, c.stream>>>(n, a, v);
+  FVK_HIP(hipGetLastError());
+  return 0;
+}
+int copy_d(Ctx &c, const double *a, double *b, size_t n) {
+  if (n == 0) return 0;
+  const size_t g = std::min<size_t>((n + BS - 1) / BS, 4096);
+  k_copy_d<<<(int)g, BS, 0, c.stream>>>(n, a, b);
+  FVK_HIP(hipGetLastError());
+  return 0;
+}
+
 // ------------------------------------------------------------ host side
 
 int precond_build(Ctx &c, int imat) {
@@ -582,17 +625,8 @@ static inline double *wv(Ctx &c, int k) { return c.wk + (size_t)k * 3 * c.n; }
 static inline double *pbuf(Ctx &c, int k) { return c.part + (size_t)k * RG * 8; }
 
 static int ks_init(Ctx &c, int nvc, double reduc, int n_iter, bool bicg) {
-  KS h[6];
-  std::memset(h, 0, sizeof h);
-  for (int s = 0; s < 2; ++s)
-    for (int q = 0; q < nvc; ++q) {
-      KS &k = h[s * 3 + q];
-      k.reduc = reduc;
-      k.n_iter = n_iter;
-      if (bicg) k.w2 = k.w3 = k.w5 = 1.0;
-      else k.s = 1.0 / SMALL;  // large (precision.f90:9)
-    }
-  FVK_HIP(hipMemcpyAsync(c.ksr, h, sizeof h, hipMemcpyHostToDevice, c.stream));
+  k_ks_init<<<1, 64, 0, c.stream>>>(c.ksr, nvc, reduc, n_iter, bicg ? 1 : 0);
+  FVK_HIP(hipGetLastError());
   return 0;
 }
 
@@ -640,13 +674,13 @@ static int cg_run(Ctx &c, int imat, int n_iter, int nvc, double *x, double *r0, 
     int rc = precond_build(c, imat);
     if (rc) return rc;
   }
-  FVK_HIP(hipMemsetAsync(P, 0, sizeof(double) * 2 * 3 * n, c.stream));
+  if (fill(c, P, (size_t)2 * 3 * n, 0.0)) return FVK_ERR_HIP;
 #define L(W)                                                                                                \
   k_residual<W><<<G, RB, 0, c.stream>>>(c.M, nvc, c.diag, c.spv, x, r0, not_r0 ? 1 : 0, jac ? 1 : 0, c.pc_c, z, \
                                         pbuf(c, 0))
   FVK_WS(c.M.Ws, L);
 #undef L
-  FVK_HIP(hipMemcpyAsync(r, r0, sizeof(double) * nvc * n, hipMemcpyDeviceToDevice, c.stream));
+  if (copy_d(c, r0, r, (size_t)nvc * n)) return FVK_ERR_HIP;
   const double *pR = nullptr;
   if (!jac) {
     int rc = precond_apply(c, imat, nvc, r, z, nullptr);
@@ -691,13 +725,13 @@ static int bicg_run(Ctx &c, int imat, int n_iter, int nvc, double *x, double *r0
     int rc = precond_build(c, imat);
     if (rc) return rc;
   }
-  FVK_HIP(hipMemsetAsync(p, 0, sizeof(double) * 4 * 3 * n, c.stream));
+  if (fill(c, p, (size_t)4 * 3 * n, 0.0)) return FVK_ERR_HIP;
 #define L(W)                                                                                              \
   k_residual<W><<<G, RB, 0, c.stream>>>(c.M, nvc, c.diag, c.spv, x, r0, not_r0 ? 1 : 0, 2, nullptr, nullptr, \
                                         pbuf(c, 0))
   FVK_WS(c.M.Ws, L);
 #undef L
-  FVK_HIP(hipMemcpyAsync(r, r0, sizeof(double) * nvc * n, hipMemcpyDeviceToDevice, c.stream));
+  if (copy_d(c, r0, r, (size_t)nvc * n)) return FVK_ERR_HIP;
   int it = 0;
   for (; it < n_iter; ++it) {
     const KS *gate = c.ksr + (it & 1) * 3;

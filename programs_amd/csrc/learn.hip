@@ -548,6 +548,40 @@ uint64_t smix(uint64_t &s) {
 
 }  // namespace
 
+namespace fvk {
+
+int learned_factor(Ctx &c, fvk_model *m) {
+  int rc = check_model(m, c);
+  if (rc) return rc;
+  if ((rc = run_forward(m, c))) return rc;
+  return factor_assemble(c);
+}
+
+int learned_train(Ctx &c, fvk_model *m, int steps, double lr, double *last_loss) {
+  int rc = check_model(m, c);
+  if (rc) return rc;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  LossBufs B;
+  for (int k = 0; k < steps; ++k) {
+    int nblk = 0;
+    if ((rc = run_loss_grad(m, c, B, nblk))) return rc;
+    m->step += 1;
+    const float c1 = 1.f - std::pow(b1, (float)m->step), c2 = 1.f - std::pow(b2, (float)m->step);
+    k_adam<<<(NPARAM + BS - 1) / BS, BS, 0, c.stream>>>(m->theta, m->m1, m->m2, m->gpart, nblk, nullptr, 1,
+                                                        (float)lr, b1, b2, eps, c1, c2);
+    FVK_HIP(hipGetLastError());
+  }
+  if (last_loss && steps > 0) {
+    double jt[2];
+    FVK_HIP(hipMemcpyAsync(jt, B.jout, sizeof jt, hipMemcpyDeviceToHost, c.stream));
+    FVK_HIP(hipStreamSynchronize(c.stream));
+    *last_loss = jt[0];
+  }
+  return 0;
+}
+
+}  // namespace fvk
+
 #define GUARD_L(x)   \
   do {               \
     int rc_ = (x);   \
@@ -629,11 +663,7 @@ int fvk_model_set(fvk_model *m, const float *theta) {
 
 int fvk_factor_build(fvk_ctx *cc, const fvk_model *mc) {
   if (!cc || !mc) { set_error("null argument"); return FVK_ERR_ARG; }
-  Ctx &c = cc->c;
-  fvk_model *m = const_cast<fvk_model *>(mc);
-  GUARD_L(check_model(m, c));
-  GUARD_L(run_forward(m, c));
-  return factor_assemble(c);
+  return learned_factor(cc->c, const_cast<fvk_model *>(mc));
 }
 
 int fvk_factor_get(fvk_ctx *cc, float *l, float *sig) {

@@ -546,6 +546,7 @@ int dc_solve(Ctx &c, bool use_factor, const double *b, double *x, double tol, in
     double h[4];
     FVK_HIP(hipMemcpyAsync(h, X.tot, sizeof h, hipMemcpyDeviceToHost, c.stream));
     FVK_HIP(hipStreamSynchronize(c.stream));
+    res.b_l1 = h[2];
     res.res_l1 = h[2] > 0 ? h[0] / h[2] : h[0];
     res.res_l2 = h[3] > 0 ? std::sqrt(h[1] / h[3]) : std::sqrt(h[1]);
     res.outer = outer;

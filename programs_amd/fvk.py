@@ -137,6 +137,8 @@ def lib():
         "fvk_flow_setup_pp": [_vp, C.c_int, C.POINTER(OuterHist)],
         "fvk_flow_outer": [_vp, C.c_int, C.POINTER(OuterHist)],
         "fvk_flow_run": [_vp, C.c_int, C.c_int, C.POINTER(OuterHist)],
+        "fvk_flow_run_graph": [_vp, C.c_int, C.c_int, C.POINTER(OuterHist)],
+        "fvk_flow_set_model": [_vp, _vp, C.c_int, C.c_double],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -532,6 +534,15 @@ class Flow:
         hs = (OuterHist * max(n, 1))()
         _check(lib().fvk_flow_run(self.h, it0, n, hs))
         return [_hist(hs[k]) for k in range(n)]
+
+    def run_graph(self, it0, n):
+        """n outer iterations, it0 eagerly and the rest as HIP-graph replays."""
+        H = (OuterHist * max(n, 1))()
+        _check(lib().fvk_flow_run_graph(self.h, it0, n, H))
+        return [_hist(H[k]) for k in range(n)]
+
+    def set_model(self, model, train_steps=0, lr=1e-3):
+        _check(lib().fvk_flow_set_model(self.h, model.h if model is not None else None, train_steps, lr))
 
     def state(self):
         c = self.ctx
