@@ -305,6 +305,11 @@ int fvk_flow_get_state(fvk_flow *f, double *u_cell, double *u_bnd, double *p, do
                        double *spd, double *cs, double *frc_dns);
 /* Last assembled matrix (pl, ng%p, ng%n in reference order). */
 int fvk_flow_get_matrix(fvk_flow *f, double *pl, double *ng_p, double *ng_n);
+/* Right-hand side of the last solve of the first component (after
+   cfd_setup_pp: the pressure-correction residual, the res0 the reference
+   passes to solv_cg with not_r0), reference order.  With fvk_flow_get_matrix
+   it harvests the pressure systems the flow generates. */
+int fvk_flow_get_rhs(fvk_flow *f, double *rhs);
 int fvk_flow_setup_eqn(fvk_flow *f, int it, fvk_outer_hist *h);
 int fvk_flow_setup_pp(fvk_flow *f, int it, fvk_outer_hist *h);
 /* One outer iteration (cfd_solve.f90:134-178, plain branch). */

@@ -777,9 +777,9 @@ int fvk_sys_solve(fvk_ctx *cc, int method, int imat, double tol, int max_iter, i
       // k_cg_pq: z, p_old, diag read; p_new, q written; column + value per off-diagonal
       info->kern_bytes = 40.0 * n + 12.0 * nnz_off;
     else
-      // k_mx_pq in FP32: z, p_old, diag read; p_new, q written; 16/32-bit
-      // column + FP32 value per off-diagonal
-      info->kern_bytes = 20.0 * n + (c.mx.i16 ? 6.0 : 8.0) * nnz_off;
+      // k_mx_pq on the Jacobi-scaled FP32 system (unit diagonal): z, p_old
+      // read; p_new, q written; 16/32-bit column + FP32 value per off-diagonal
+      info->kern_bytes = 16.0 * n + (c.mx.i16 ? 6.0 : 8.0) * nnz_off;
     info->kern_launches = T.launches;
   }
   return status;
@@ -1027,6 +1027,12 @@ int fvk_flow_get_matrix(fvk_flow *ff, double *pl, double *ng_p, double *ng_n) {
   if (ng_p) FVK_HIP(hipMemcpy(ng_p, c.ngp, sizeof(double) * c.nf, hipMemcpyDeviceToHost));
   if (ng_n) FVK_HIP(hipMemcpy(ng_n, c.ngn, sizeof(double) * c.nf, hipMemcpyDeviceToHost));
   return FVK_OK;
+}
+
+int fvk_flow_get_rhs(fvk_flow *ff, double *rhs) {
+  if (!ff || !rhs) { set_error("null argument"); return FVK_ERR_ARG; }
+  Flow &F = ff->f;
+  return to_host(*F.c, F.rhs, 1, 1, F.c->n, &F.c->perm, rhs);
 }
 
 static int read_hist(Flow &F, fvk_outer_hist *h) {

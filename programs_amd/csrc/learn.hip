@@ -53,9 +53,16 @@ __device__ __forceinline__ v4s pack4(v4f v) {
   r[0] = bf16(v[0]); r[1] = bf16(v[1]); r[2] = bf16(v[2]); r[3] = bf16(v[3]);
   return r;
 }
+// tanh(x) = 1 - 2 / (exp(2x) + 1) with the hardware exp2 and reciprocal
+// (absolute error ~1e-7, below the bfloat16 rounding that follows);
+// saturates to +-1
+__device__ __forceinline__ float tanh_fast(float x) {
+  const float e = __builtin_amdgcn_exp2f(2.8853900817779268f * x);  // exp(2x)
+  return 1.f - 2.f * __builtin_amdgcn_rcpf(e + 1.f);
+}
 __device__ __forceinline__ v4f tanh4(v4f v) {
   v4f r;
-  r[0] = tanhf(v[0]); r[1] = tanhf(v[1]); r[2] = tanhf(v[2]); r[3] = tanhf(v[3]);
+  r[0] = tanh_fast(v[0]); r[1] = tanh_fast(v[1]); r[2] = tanh_fast(v[2]); r[3] = tanh_fast(v[3]);
   return r;
 }
 

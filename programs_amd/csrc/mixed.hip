@@ -198,13 +198,19 @@ __global__ __launch_bounds__(RB) void k_mx_upd_jac(int n, int it, float *__restr
   double a[4] = {0, 0, 0, 0};
   int r0, r1;
   block_rows(n, r0, r1);
-  for (int i = r0 + threadIdx.x; i < r1; i += RB) {
+  auto row = [&](int i) {
     ev[i] += af * p[i];
     const float ri = r[i] - af * q[i];
     r[i] = ri;
     a[0] += fabs((double)ri);
     a[3] += (double)ri * (double)ri;
+  };
+  int i = r0 + threadIdx.x;
+  for (; i + RB < r1; i += 2 * RB) {
+    row(i);
+    row(i + RB);
   }
+  if (i < r1) row(i);
   store_partials<4>(a, partU);
   if (blockIdx.x == 0 && threadIdx.x == 0) cur->alpha = alpha;
 }
@@ -227,7 +233,7 @@ __global__ __launch_bounds__(RB) void k_mx_upd_fac(int n, int it, float *__restr
   double a[4] = {0, 0, 0, 0};
   int r0, r1;
   block_rows(n, r0, r1);
-  for (int i = r0 + threadIdx.x; i < r1; i += RB) {
+  auto row = [&](int i) {
     ev[i] += af * p[i];
     const float ri = rin[i] - af * q[i];
     rout[i] = ri;
@@ -240,7 +246,13 @@ __global__ __launch_bounds__(RB) void k_mx_upd_fac(int n, int it, float *__restr
       acc += (float)uval[o] * (rin[j] - af * q[j]);
     }
     t[i] = ee[i] * acc;
+  };
+  int i = r0 + threadIdx.x;
+  for (; i + RB < r1; i += 2 * RB) {
+    row(i);
+    row(i + RB);
   }
+  if (i < r1) row(i);
   store_partials<4>(a, partU);
   if (blockIdx.x == 0 && threadIdx.x == 0) cur->alpha = alpha;
 }
@@ -255,7 +267,7 @@ __global__ __launch_bounds__(RB) void k_mx_gz(int n, int wl, const I *__restrict
   double a[1] = {0.0};
   int r0, r1;
   block_rows(n, r0, r1);
-  for (int i = r0 + threadIdx.x; i < r1; i += RB) {
+  auto row = [&](int i) {
     float acc = t[i];
 #pragma unroll
     for (int k = 0; k < nw<WL>(wl); ++k) {
@@ -264,7 +276,13 @@ __global__ __launch_bounds__(RB) void k_mx_gz(int n, int wl, const I *__restrict
     }
     z[i] = acc;
     a[0] += (double)r[i] * (double)acc;
+  };
+  int i = r0 + threadIdx.x;
+  for (; i + RB < r1; i += 2 * RB) {
+    row(i);
+    row(i + RB);
   }
+  if (i < r1) row(i);
   store_partials<1>(a, partR);
 }
 

@@ -63,8 +63,8 @@ contains
     ! uniform properties: density prp(2), viscosity prp(3) (cfd_solve.f90:64-66)
     cfg%rho=grd(igrd)%vol(1)%phs(iph)%prp(2)%rho(1)
     cfg%mu=grd(igrd)%vol(1)%phs(iph)%prp(3)%rho(1)
-    cfg%p_solver=0
-    cfg%p_tol=1.e-8_c_double
+    cfg%p_solver=fvk_env_int('FVK_P_SOLVER',0)
+    cfg%p_tol=fvk_env_real('FVK_P_TOL',1.e-8_c_double)
   end subroutine
 
   subroutine fvk_flow_ensure(igrd,iph,ieq,urf_p)

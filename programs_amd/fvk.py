@@ -139,6 +139,7 @@ def lib():
         "fvk_flow_run": [_vp, C.c_int, C.c_int, C.POINTER(OuterHist)],
         "fvk_flow_run_graph": [_vp, C.c_int, C.c_int, C.POINTER(OuterHist)],
         "fvk_flow_set_model": [_vp, _vp, C.c_int, C.c_double],
+        "fvk_flow_get_rhs": [_vp, _dp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -540,6 +541,11 @@ class Flow:
         H = (OuterHist * max(n, 1))()
         _check(lib().fvk_flow_run_graph(self.h, it0, n, H))
         return [_hist(H[k]) for k in range(n)]
+
+    def rhs(self):
+        r = np.zeros(self.ctx.n_vol)
+        _check(lib().fvk_flow_get_rhs(self.h, _d(r)))
+        return r
 
     def set_model(self, model, train_steps=0, lr=1e-3):
         _check(lib().fvk_flow_set_model(self.h, model.h if model is not None else None, train_steps, lr))

@@ -5,10 +5,12 @@ tolerance inside the flow.
   arguments as the eager path: fields and histories must be identical bit
   for bit (relaxed/Jacobi and shipped SIMPLER/DIC/DILU configurations).
 * With the pressure-correction systems solved to an FP64 relative residual
-  of 1e-9 (p_solver 3: plain FP64 PCG; 1: FP32-inner defect correction;
+  of 1e-12 (p_solver 3: plain FP64 PCG; 1: FP32-inner defect correction;
   2: the same with the learned factor, trained online on the flow's own
   pressure systems), the fields after a fixed number of outer iterations
-  agree to 1e-7 relative: the solvers differ only within the tolerance.
+  agree to 1e-8 relative: the solvers differ only within the tolerance
+  (a solve to 1e-9 leaves p' errors of about condition number x 1e-9,
+  which moved the fields by 4e-7 between preconditioners).
 """
 import numpy as np
 import pytest
@@ -18,7 +20,7 @@ from tests.helpers import require_lib
 pytestmark = pytest.mark.gpu
 
 
-def make_flow(n, jitter, simpler, p_solver=0, p_tol=1e-9, imat_p=0, imat_u=0):
+def make_flow(n, jitter, simpler, p_solver=0, p_tol=1e-12, imat_p=0, imat_u=0):
     fvk = require_lib()
     m = fvk.Mesh.box(n, n, n, jitter=jitter, seed=0x5EED0002)
     ctx = fvk.grd_solv(m)
@@ -68,4 +70,4 @@ def test_pressure_to_tolerance_modes_agree():
         for k in ("u_cell", "p"):
             e = np.max(np.abs(out[ps][0][k] - ref[k])) / np.max(np.abs(ref[k]))
             print(ps, k, e)
-            assert e < 1e-7, (ps, k, e)
+            assert e < 1e-8, (ps, k, e)
