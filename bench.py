@@ -69,6 +69,28 @@ def cpu_baseline(ctx, fvk, n, jitter, gpu_iters, sample_iters):
                 per_iter_ms=per_it * 1e3)
 
 
+def cpu_simple(args):
+    """The oracle's SIMPLE outer iteration (plain C, one core) on the same
+    cavity as the SIMPLE leg: iterations 1-2 untimed, iteration 3 timed."""
+    from oracle import fvo
+    n = args.simple_n
+    t0 = time.time()
+    om = fvo.OMesh.box(n, n, n, jitter=0.02, seed=0x5EED0002)
+    of = fvo.OFlow(om, fvo.default_cfg(simpler=0, imat_u=0, imat_p=0, urf_p=0.3))
+    of.set_bnd_u(31, [1.0, 0.0, 0.0])
+    of.outer(1)
+    of.outer(2)
+    t_setup = time.time() - t0
+    t0 = time.time()
+    of.outer(3)
+    dt = time.time() - t0
+    of.free()
+    om.free()
+    return dict(value=dt, unit="s per outer iteration", cores=1, kind="port",
+                sample="outer iteration 3 of the oracle's SIMPLE on the %d^3 cavity (set-up and iterations "
+                       "1-2, %.1f s, excluded)" % (n, t_setup))
+
+
 def train_model(fvk, args):
     """The network of the learned factor, trained on the GPU (untimed): one
     Adam step on each of args.train_lognormal lognormal pressure systems
@@ -85,6 +107,8 @@ def train_model(fvk, args):
         j = model.train(c, 1, lr=args.lr)[0]
         first = j if first is None else first
         last = j
+        if (k + 1) % 200 == 0:
+            log("  trained on %d lognormal systems, loss %.5f (%.1f s)" % (k + 1, j, time.time() - t0))
     c.destroy()
     m.free()
     n_simple = 0
@@ -95,7 +119,10 @@ def train_model(fvk, args):
         fl = fvk.Flow(c, cfg)
         fl.set_bnd_u(31, [1.0, 0.0, 0.0])
         fl.set_model(model, train_steps=1, lr=args.lr)
-        fl.run(1, args.train_simple)
+        for k0 in range(0, args.train_simple, 100):
+            fl.run(1 + k0, min(100, args.train_simple - k0))
+            log("  trained on %d SIMPLE pressure systems (%.1f s)" % (min(k0 + 100, args.train_simple),
+                                                                    time.time() - t0))
         n_simple = args.train_simple
         fl.destroy()
         c.destroy()
@@ -105,6 +132,62 @@ def train_model(fvk, args):
         "lognormal systems)" % (args.train_lognormal, n_simple, ctx_s, first, last))
     return model, dict(systems=args.train_lognormal + n_simple, lognormal=args.train_lognormal,
                        simple=n_simple, seconds=ctx_s, loss_first=first, loss_last=last)
+
+
+def simple_leg(fvk, args, model):
+    """SIMPLE outer iterations on the 128^3 lid-driven cavity (Re=100, lid u=1,
+    +-2% jitter: the reference's cell volumes stay positive there), momentum
+    BiCGStab with u, v, w batched (5 iterations, Jacobi), relaxed pressure
+    update (urf 0.3):
+      * reference pressure solves (PCG recurrence, 20 iterations, Jacobi),
+        eager and replayed from HIP graphs: outer iterations per second;
+      * pressure-correction solves to an FP64 relative residual of tol, plain
+        FP64 PCG against the learned FP32-inner solve: seconds per outer
+        iteration."""
+    n = args.simple_n
+    out = {}
+    m = fvk.Mesh.box(n, n, n, jitter=0.02, seed=0x5EED0002)
+    c = fvk.grd_solv(m)
+
+    def flow(**kw):
+        cfg = fvk.flow_cfg(simpler=0, imat_u=0, imat_p=0, urf_p=0.3, **kw)
+        fl = fvk.Flow(c, cfg)
+        fl.set_bnd_u(31, [1.0, 0.0, 0.0])
+        return fl
+
+    for mode in ("eager", "graph"):
+        fl = flow()
+        run = fl.run if mode == "eager" else fl.run_graph
+        run(1, 2)
+        c.sync()
+        t0 = time.perf_counter()
+        h = run(3, args.simple_its)
+        c.sync()
+        dt = (time.perf_counter() - t0) / args.simple_its
+        out["reference_pressure_" + mode] = dict(outer_its_per_s=1.0 / dt, ms_per_outer=dt * 1e3,
+                                                 mx_res_n_last=h[-1]["mx_res_n"])
+        fl.destroy()
+    legs = [("plain_pcg64", 3, None)]
+    if model is not None:
+        legs.append(("dc32_learned", 2, model))
+    for name, ps, mdl in legs:
+        fl = flow(p_solver=ps, p_tol=args.tol)
+        if mdl is not None:
+            fl.set_model(mdl, train_steps=0)
+        fl.run(1, 2)
+        c.sync()
+        t0 = time.perf_counter()
+        h = fl.run(3, args.simple_tol_its)
+        c.sync()
+        dt = (time.perf_counter() - t0) / args.simple_tol_its
+        out["pressure_to_tol_" + name] = dict(ms_per_outer=dt * 1e3, outer_its_per_s=1.0 / dt,
+                                              p_iters=[x["it_pp"] for x in h])
+        fl.destroy()
+    c.destroy()
+    m.free()
+    log("SIMPLE %d^3: %s" % (n, json.dumps(out)))
+    return dict(workload="SIMPLE outer iterations, %d^3 lid-driven cavity (+-2%% jitter), Re=100, urf_u 0.7, "
+                         "urf_p 0.3, momentum BiCGStab (3 RHS batched, Jacobi, 5 its)" % n, **out)
 
 
 def main():
@@ -122,6 +205,10 @@ def main():
     ap.add_argument("--train-lognormal", type=int, default=1600)
     ap.add_argument("--train-simple", type=int, default=400)
     ap.add_argument("--lr", type=float, default=3e-3)
+    ap.add_argument("--simple-n", type=int, default=128)
+    ap.add_argument("--simple-its", type=int, default=20)
+    ap.add_argument("--simple-tol-its", type=int, default=3)
+    ap.add_argument("--no-simple", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is "
@@ -161,6 +248,7 @@ def main():
     wall, r = timed(fvk.SOLVE_PCG64, "plain_pcg64_jacobi")
     methods = {"plain_pcg64_jacobi": dict(s=wall, iters=r["iters"], res_true_l1=r["res_true_l1"],
                                           res_true_l2=r["res_true_l2"])}
+    model = None
     if "learned" in args.methods.split(","):
         model, tinfo = train_model(fvk, args)
         w3, r3 = timed(fvk.SOLVE_DC32_FACTOR, "dc32_learned", model)
@@ -237,8 +325,12 @@ def main():
         "copy_gbs": 2 * copy_bytes / t_copy / 1e9,
         "methods": methods,
     }
+    if not args.no_simple:
+        out["simple"] = simple_leg(fvk, args, model)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ctx, fvk, n, args.jitter, r["iters"], args.cpu_sample)
+        if not args.no_simple:
+            out["cpu_baseline_simple"] = cpu_simple(args)
     print(json.dumps(out), flush=True)
     ctx.destroy()
 

@@ -708,14 +708,7 @@ static long nnz_offdiag(const Ctx &c) {
 }
 
 static int solve_pcg64(Ctx &c, int imat, double tol, int max_iter, KTimer *T, KS &h) {
-  const int n = c.n;
-  double *r0 = c.wk + (size_t)6 * 3 * n;
-  FVK_HIP(hipMemsetAsync(c.sys_x, 0, sizeof(double) * n, c.stream));
-  FVK_HIP(hipMemcpyAsync(r0, c.sys_b, sizeof(double) * n, hipMemcpyDeviceToDevice, c.stream));
-  GUARD(krylov(c, 0, imat, tol, max_iter, 1, c.sys_x, r0, true, 64, T));
-  FVK_HIP(hipMemcpyAsync(&h, c.ks, sizeof h, hipMemcpyDeviceToHost, c.stream));
-  FVK_HIP(hipStreamSynchronize(c.stream));
-  return 0;
+  return pcg_tol(c, imat, c.sys_b, c.sys_x, tol, max_iter, 64, T, h);
 }
 
 int fvk_sys_solve(fvk_ctx *cc, int method, int imat, double tol, int max_iter, int time_kernel,

@@ -148,6 +148,13 @@ struct KTimer {
 int krylov(Ctx &c, int kind, int imat, double reduc, int n_iter, int nvc, double *x, double *r0, bool not_r0,
            int chunk = 0, KTimer *T = nullptr);
 int true_residual(Ctx &c, const double *x, const double *b, double *dev_out4, double *host_out4);
+// Plain FP64 PCG (the reference recurrence with preconditioner imat) from
+// x = 0 to a relative L1 residual tol.  The right-hand side is normalised to
+// a mean |b| of 1 first: the recurrence's alpha = s/(p.q + small)
+// (solv_cg.f90:69-70) has an absolute small = 1e-20 that stalls solves whose
+// p.q gets that small.  out: the final scalars, res0_n = |b|_1.
+int pcg_tol(Ctx &c, int imat, const double *b, double *x, double tol, int max_iter, int chunk, KTimer *T,
+            KS &out);
 
 // mixed.hip
 struct DcResult {

@@ -746,8 +746,10 @@ static int pp_solve(Flow &F, double *x, double *b) {
     case 0:
       return krylov(c, 0, F.cfg.imat_p, F.cfg.reduc_p, F.cfg.n_cg, 1, x, b, true,
                     F.cfg.n_cg > 64 ? chunk : 0);
-    case 3:
-      return krylov(c, 0, 0, F.cfg.p_tol, cap, 1, x, b, true, chunk);
+    case 3: {
+      KS h;
+      return pcg_tol(c, 0, b, x, F.cfg.p_tol, cap, chunk, nullptr, h);
+    }
     default: {
       int rc;
       const bool fac = F.cfg.p_solver == 2;
@@ -760,11 +762,8 @@ static int pp_solve(Flow &F, double *x, double *b) {
       if ((rc = dc_solve(c, fac, b, x, F.cfg.p_tol, cap, R, nullptr))) return rc;
       if (!R.converged) {
         // the plain FP64 Jacobi PCG from x = 0 takes over
-        if (fill(c, x, c.n, 0.0)) return FVK_ERR_HIP;
-        if ((rc = krylov(c, 0, 0, F.cfg.p_tol, cap, 1, x, b, true, chunk))) return rc;
         KS h;
-        FVK_HIP(hipMemcpyAsync(&h, c.ks, sizeof h, hipMemcpyDeviceToHost, c.stream));
-        FVK_HIP(hipStreamSynchronize(c.stream));
+        if ((rc = pcg_tol(c, 0, b, x, F.cfg.p_tol, cap, chunk, nullptr, h))) return rc;
         k_ks_store<<<1, 1, 0, c.stream>>>(c.ks, R.b_l1, R.iters + h.iter, h.res_n);
         FlowSc s;
         FVK_HIP(hipMemcpy(&s, F.sc, sizeof s, hipMemcpyDeviceToHost));

@@ -760,6 +760,33 @@ int krylov(Ctx &c, int kind, int imat, double reduc, int n_iter, int nvc, double
   return bicg_run(c, imat, n_iter, nvc, x, r0, not_r0);
 }
 
+__global__ __launch_bounds__(BS) void k_scale64(size_t n, double *__restrict__ a, const double *__restrict__ b,
+                                                double s) {
+  for (size_t i = blockIdx.x * (size_t)BS + threadIdx.x; i < n; i += (size_t)gridDim.x * BS) a[i] = s * b[i];
+}
+
+int pcg_tol(Ctx &c, int imat, const double *b, double *x, double tol, int max_iter, int chunk, KTimer *T,
+            KS &out) {
+  const int n = c.n;
+  int rc;
+  if ((rc = fill(c, x, n, 0.0))) return rc;
+  double tr[4];
+  if ((rc = true_residual(c, x, b, c.stage, tr))) return rc;
+  const double l1b = tr[2], sigma = l1b > 0 ? l1b / n : 1.0;
+  double *bs = c.wk + (size_t)7 * 3 * n;
+  const int g = (int)std::min<size_t>(((size_t)n + BS - 1) / BS, 4096);
+  k_scale64<<<g, BS, 0, c.stream>>>(n, bs, b, 1.0 / sigma);
+  if ((rc = krylov(c, 0, imat, tol, max_iter, 1, x, bs, true, chunk, T))) return rc;
+  k_scale64<<<g, BS, 0, c.stream>>>(n, x, x, sigma);
+  FVK_HIP(hipGetLastError());
+  FVK_HIP(hipMemcpyAsync(&out, c.ks, sizeof out, hipMemcpyDeviceToHost, c.stream));
+  FVK_HIP(hipStreamSynchronize(c.stream));
+  // the caller sees the norms of its own right-hand side
+  out.res0_n = l1b;
+  FVK_HIP(hipMemcpy(c.ks, &out, sizeof out, hipMemcpyHostToDevice));
+  return 0;
+}
+
 int true_residual(Ctx &c, const double *x, const double *b, double *dev_out4, double *host_out4) {
   const int G = red_grid(c.n);
 #define L(W) k_true_res<W><<<G, RB, 0, c.stream>>>(c.M, c.diag, c.spv, x, b, pbuf(c, 3))

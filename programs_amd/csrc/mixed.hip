@@ -316,11 +316,17 @@ __global__ __launch_bounds__(RB) void k_mx_tot4(const double *part, int P, doubl
     for (int j = 0; j < 4; ++j) out[j] = s[j];
 }
 
-// x += D^-1/2 eh
-__global__ __launch_bounds__(BS) void k_mx_add(int n, double *__restrict__ x, const float *__restrict__ ev,
-                                               const double *__restrict__ diag) {
+// a *= s (the inner right-hand side normalised to mean |r| = 1)
+__global__ __launch_bounds__(BS) void k_scale32(int n, float *__restrict__ a, float s) {
   const int i = blockIdx.x * BS + threadIdx.x;
-  if (i < n) x[i] += (double)ev[i] / sqrt(diag[i]);
+  if (i < n) a[i] *= s;
+}
+
+// x += sigma D^-1/2 eh
+__global__ __launch_bounds__(BS) void k_mx_add(int n, double *__restrict__ x, const float *__restrict__ ev,
+                                               const double *__restrict__ diag, double sigma) {
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i < n) x[i] += sigma * (double)ev[i] / sqrt(diag[i]);
 }
 
 // Factor from per-interior-face (l, sig): e = exp(2 s) with s the sum of sig
@@ -574,13 +580,19 @@ int dc_solve(Ctx &c, bool use_factor, const double *b, double *x, double tol, in
     // inner target: what is left to reach tol (with a margin), but no
     // deeper than inner_reduc, where FP32 rounding takes over
     const double eta = std::min(0.5, std::max(X.inner_reduc, 0.3 * tol / res.res_l1));
+    // The inner right-hand side is normalised to a mean |r| of 1: the PCG
+    // recurrence's alpha = s/(p.q + small) (solv_cg.f90:69-70) carries an
+    // absolute small = 1e-20 that stalls the solve once p.q is that tiny,
+    // which the residuals of later correction steps reach.
+    const double sigma = h[0] > 0 ? h[0] / n : 1.0;
+    k_scale32<<<grid_for(n), BS, 0, c.stream>>>(n, fv(c, V_R0), (float)(1.0 / sigma));
     int its = 0, broken = 0;
     rc = X.i16 ? inner_solve<short>(c, use_factor, max_iter - res.iters, eta, its, broken, T)
                : inner_solve<int>(c, use_factor, max_iter - res.iters, eta, its, broken, T);
     if (rc) return rc;
     res.iters += its;
     res.broken |= broken;
-    k_mx_add<<<grid_for(n), BS, 0, c.stream>>>(n, x, fv(c, V_E), c.diag);
+    k_mx_add<<<grid_for(n), BS, 0, c.stream>>>(n, x, fv(c, V_E), c.diag, sigma);
     FVK_HIP(hipGetLastError());
   }
   return 0;
