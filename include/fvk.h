@@ -271,7 +271,8 @@ typedef struct fvk_flow_cfg {
                        correction (Jacobi), 2 the same with the learned
                        factor of the model set by fvk_flow_set_model (plain
                        FP64 PCG when it does not converge), 3 plain FP64
-                       Jacobi PCG */
+                       Jacobi PCG; solutions of singular (pure Neumann)
+                       systems are shifted to mean zero */
   double p_tol;
 } fvk_flow_cfg;
 

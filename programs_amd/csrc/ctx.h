@@ -1,5 +1,6 @@
 // Host-side state of an fvk device context.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -124,8 +125,15 @@ struct KTimer {
     ev.resize(2 * pairs);
     for (auto &e : ev) hipEventCreate(&e);
   }
-  void begin(hipStream_t s) { if (used < (int)ev.size() / 2) hipEventRecord(ev[2 * used], s); }
-  void end(hipStream_t s) { if (used < (int)ev.size() / 2) { hipEventRecord(ev[2 * used + 1], s); ++used; } }
+  // the next free event pair, passed to hipExtLaunchKernelGGL, which stamps
+  // the kernel's own start and end (no marker packets around it)
+  bool take(hipEvent_t &a, hipEvent_t &b) {
+    if (used >= (int)ev.size() / 2) return false;
+    a = ev[2 * used];
+    b = ev[2 * used + 1];
+    ++used;
+    return true;
+  }
   int flush() {
     for (int k = 0; k < used; ++k) {
       float ms = 0.f;

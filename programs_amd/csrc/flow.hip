@@ -730,6 +730,61 @@ static int pp_assemble(Flow &F) {
   return 0;
 }
 
+// Mean relative row defect sum_i |sum_j a_ij| / a_ii: zero for the
+// pure-Neumann (singular) pressure systems of closed domains.
+template <int WS>
+__global__ __launch_bounds__(RB) void k_row_defect(DMesh M, const double *__restrict__ diag,
+                                                   const double *__restrict__ spv, double *__restrict__ part) {
+  double a[1] = {0.0};
+  int r0, r1;
+  block_rows(M.n, r0, r1);
+  for (int i = r0 + threadIdx.x; i < r1; i += RB) {
+    double s = diag[i];
+#pragma unroll
+    for (int k = 0; k < nslots<WS>(M); ++k) {
+      const int j = M.spc[(size_t)k * M.n + i];
+      const double v = spv[(size_t)k * M.n + i];
+      if (j >= 0) s += v;
+    }
+    a[0] += fabs(s) / fabs(diag[i]);
+  }
+  store_partials<1>(a, part);
+}
+
+__global__ __launch_bounds__(BS) void k_shift(int n, double *__restrict__ x, const double *__restrict__ sum) {
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i < n) x[i] -= *sum / n;
+}
+
+// A solve to tolerance of a singular (pure Neumann) system leaves a constant
+// that depends on the solver's path, and the Green-Gauss gradient of a
+// constant is not zero on warped cells (their face vectors do not close), so
+// the fields would depend on the solver.  Such solutions are shifted to mean
+// zero, whatever solver produced them.
+static int null_fix(Flow &F, double *x) {
+  Ctx &c = *F.c;
+  const int G = red_grid(c.n);
+  double *part = c.part + (size_t)3 * RG * 8;
+#define L(W) k_row_defect<W><<<G, RB, 0, c.stream>>>(c.M, c.diag, c.spv, part)
+  switch (c.M.Ws) {
+    case 3: L(3); break;
+    case 4: L(4); break;
+    case 6: L(6); break;
+    default: L(0); break;
+  }
+#undef L
+  k_sum_total<<<1, RB, 0, c.stream>>>(part, G, F.red_scratch);
+  double d;
+  FVK_HIP(hipMemcpyAsync(&d, F.red_scratch, sizeof d, hipMemcpyDeviceToHost, c.stream));
+  FVK_HIP(hipStreamSynchronize(c.stream));
+  if (d / c.n > 1e-12) return 0;
+  int rc;
+  if ((rc = sum_to(c, x, F.red_scratch))) return rc;
+  k_shift<<<grid_for(c.n), BS, 0, c.stream>>>(c.n, x, F.red_scratch);
+  FVK_HIP(hipGetLastError());
+  return 0;
+}
+
 // The pressure-correction (or SIMPLER pressure) solve: the reference's PCG
 // (cfd_setup.f90:662,742) with p_solver 0, else a solve from x = 0 to an
 // FP64 true relative L1 residual p_tol: 1 FP32-inner defect correction,
@@ -748,7 +803,8 @@ static int pp_solve(Flow &F, double *x, double *b) {
                     F.cfg.n_cg > 64 ? chunk : 0);
     case 3: {
       KS h;
-      return pcg_tol(c, 0, b, x, F.cfg.p_tol, cap, chunk, nullptr, h);
+      int rc = pcg_tol(c, 0, b, x, F.cfg.p_tol, cap, chunk, nullptr, h);
+      return rc ? rc : null_fix(F, x);
     }
     default: {
       int rc;
@@ -773,7 +829,7 @@ static int pp_solve(Flow &F, double *x, double *b) {
         k_ks_store<<<1, 1, 0, c.stream>>>(c.ks, R.b_l1, R.iters, R.res_l1);
       }
       FVK_HIP(hipGetLastError());
-      return 0;
+      return null_fix(F, x);
     }
   }
 }

@@ -652,7 +652,17 @@ static int all_done_host(Ctx &c, int nvc, bool &done) {
 
 template <int NVC>
 static void launch_cg_pq(Ctx &c, int G, int it, double *z, double *pold, double *pnew, double *q, const double *pU,
-                         const double *pR) {
+                         const double *pR, KTimer *T) {
+  hipEvent_t a, b;
+  if (T && T->take(a, b)) {
+    const double *zc = z, *pc = pold;
+#define L(W)                                                                                                   \
+  hipExtLaunchKernelGGL(k_cg_pq<NVC, W>, dim3(G), dim3(RB), 0, c.stream, a, b, 0, c.M, it, (const double *)c.diag, \
+                        (const double *)c.spv, zc, pc, pnew, q, pU, G, pR, G, pbuf(c, 1), c.ksr, c.ks)
+    FVK_WS(c.M.Ws, L);
+#undef L
+    return;
+  }
 #define L(W)                                                                                                    \
   k_cg_pq<NVC, W><<<G, RB, 0, c.stream>>>(c.M, it, c.diag, c.spv, z, pold, pnew, q, pU, G, pR, G, pbuf(c, 1), \
                                           c.ksr, c.ks)
@@ -691,11 +701,9 @@ static int cg_run(Ctx &c, int imat, int n_iter, int nvc, double *x, double *r0, 
   int it = 0;
   for (; it < n_iter; ++it) {
     double *pold = P + (size_t)(it & 1) * 3 * n, *pnew = P + (size_t)((it + 1) & 1) * 3 * n;
-    if (T) T->begin(c.stream);
-    if (nvc == 1) launch_cg_pq<1>(c, G, it, z, pold, pnew, q, pbuf(c, 0), pR);
-    else if (nvc == 2) launch_cg_pq<2>(c, G, it, z, pold, pnew, q, pbuf(c, 0), pR);
-    else launch_cg_pq<3>(c, G, it, z, pold, pnew, q, pbuf(c, 0), pR);
-    if (T) T->end(c.stream);
+    if (nvc == 1) launch_cg_pq<1>(c, G, it, z, pold, pnew, q, pbuf(c, 0), pR, T);
+    else if (nvc == 2) launch_cg_pq<2>(c, G, it, z, pold, pnew, q, pbuf(c, 0), pR, T);
+    else launch_cg_pq<3>(c, G, it, z, pold, pnew, q, pbuf(c, 0), pR, T);
     if (nvc == 1) launch_cg_upd<1>(c, G, it, x, r, pnew, q, jac, z);
     else if (nvc == 2) launch_cg_upd<2>(c, G, it, x, r, pnew, q, jac, z);
     else launch_cg_upd<3>(c, G, it, x, r, pnew, q, jac, z);

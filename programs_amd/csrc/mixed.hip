@@ -507,13 +507,21 @@ static int inner_solve(Ctx &c, bool fac, int n_iter, double reduc, int &iters, i
   int it = 0;
   for (; it < n_iter; ++it) {
     float *pold = P[it & 1], *pnew = P[(it + 1) & 1], *rin = r[it & 1], *rout = r[(it + 1) & 1];
-    if (T) T->begin(c.stream);
+    hipEvent_t ea, eb;
+    if (T && T->take(ea, eb)) {
+      const float *zc = z, *pc = pold, *sp = X.spv;
+#define L(W)                                                                                                  \
+  hipExtLaunchKernelGGL(k_mx_pq<W, I>, dim3(G), dim3(RB), 0, c.stream, ea, eb, 0, n, ws, it, col, sp, zc, pc, pnew, \
+                        q, (const double *)pb(c, 0), G, pR, G, pb(c, 1), c.ksr, c.ks)
+      FVK_W(ws, L);
+#undef L
+    } else {
 #define L(W)                                                                                                  \
   k_mx_pq<W, I><<<G, RB, 0, c.stream>>>(n, ws, it, col, X.spv, z, pold, pnew, q, pb(c, 0), G, pR, G, pb(c, 1), \
                                         c.ksr, c.ks)
-    FVK_W(ws, L);
+      FVK_W(ws, L);
 #undef L
-    if (T) T->end(c.stream);
+    }
     if (!fac) {
       k_mx_upd_jac<<<G, RB, 0, c.stream>>>(n, it, ev, r[0], pnew, q, pb(c, 1), G, pb(c, 0), c.ksr);
     } else {

@@ -1,0 +1,86 @@
+"""SIMPLE on the seeded-Delaunay channel (the C5 geometry at test size):
+2D triangles, inlet (code 11, u = 1), outlet (21), walls (32), Re = 500.
+
+The GPU flow against the oracle on the same mesh after a fixed number of
+outer iterations, reference pressure solves (relaxed update, Jacobi):
+fields and histories to 1e-10 relative, or 30x the oracle's own change under
+pairwise summation where that is larger (as in test_gpu_parity).  Then the
+same channel with the pressure-correction systems solved to 1e-11 by plain
+FP64 PCG and by the learned FP32-inner solve: fields agree to 1e-8.
+"""
+import numpy as np
+import pytest
+
+from oracle import fvo
+from programs_amd import meshgen
+from tests.helpers import assert_rel, require_lib
+from tests.test_gpu_parity import TOL
+
+pytestmark = pytest.mark.gpu
+
+MU = 0.002  # Re = rho u H / mu = 500
+
+
+def channel(n_tri, seed):
+    vx, ty, vp, vr, bnd = meshgen.channel_delaunay(n_tri, seed=seed)
+    return vx, ty, vp, vr, bnd
+
+
+def oracle_run(mesh_args, n_outer):
+    vx, ty, vp, vr, bnd = mesh_args
+    vx3 = np.zeros((len(vx), 3))
+    vx3[:, :2] = vx
+    om = fvo.OMesh.raw(2, vx3, ty, vp, vr, bnd)
+    of = fvo.OFlow(om, fvo.default_cfg(simpler=0, imat_u=0, imat_p=0, urf_p=0.3, mu=MU))
+    of.set_bnd_u(11, [1.0, 0.0])
+    H = np.array([of.outer(it) for it in range(1, n_outer + 1)])
+    F = of.fields()
+    of.free()
+    om.free()
+    return H, F
+
+
+def test_channel_matches_oracle():
+    fvk = require_lib()
+    args = channel(6000, 0x5EED0005)
+    n_outer = 15
+    H, F = oracle_run(args, n_outer)
+    with fvo.sum_order(1):
+        H2, F2 = oracle_run(args, n_outer)
+    m = fvk.Mesh.from_cells(2, *args)
+    ctx = fvk.grd_solv(m)
+    fl = fvk.Flow(ctx, fvk.flow_cfg(simpler=0, imat_u=0, imat_p=0, urf_p=0.3, mu=MU))
+    fl.set_bnd_u(11, [1.0, 0.0])
+    hist = fl.run(1, n_outer)
+    s = fl.state()
+    tol = max(TOL, 30 * max(np.max(np.abs(F2[k] - F[k])) / np.max(np.abs(F[k])) for k in ("u_cell", "p")))
+    print("channel: cells", m.n_vol, "tol", tol)
+    assert_rel([h["mx_res_n"] for h in hist], H[:, 3], tol, "REDUC history")
+    for k in ("u_cell", "p", "dpdx", "spd"):
+        assert_rel(s[k].ravel(), F[k].ravel(), tol, k)
+    assert np.max(s["u_cell"][:, 0]) > 0.5  # the inlet flow has come through
+    ctx.destroy()
+
+
+def test_channel_learned_pressure_matches_plain():
+    fvk = require_lib()
+    args = channel(6000, 0x5EED0006)
+    m = fvk.Mesh.from_cells(2, *args)
+    out = {}
+    model = fvk.Model(seed=5)
+    for ps in (3, 2):
+        ctx = fvk.grd_solv(m)
+        fl = fvk.Flow(ctx, fvk.flow_cfg(simpler=0, imat_u=0, imat_p=0, urf_p=0.3, mu=MU, p_solver=ps,
+                                        p_tol=1e-11))
+        fl.set_bnd_u(11, [1.0, 0.0])
+        if ps == 2:
+            fl.set_model(model, train_steps=20, lr=3e-3)
+        h = fl.run_graph(1, 12)
+        out[ps] = fl.state()
+        print("p_solver", ps, "p' iterations", [x["it_pp"] for x in h])
+        ctx.destroy()
+    model.destroy()
+    for k in ("u_cell", "p"):
+        e = np.max(np.abs(out[2][k] - out[3][k])) / np.max(np.abs(out[3][k]))
+        print(k, e)
+        assert e < 1e-8, (k, e)
