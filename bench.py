@@ -255,14 +255,17 @@ def main():
         rk3 = fvk.sys_solve(ctx, fvk.SOLVE_DC32_FACTOR, 0, args.tol, 200000, time_kernel=True)
         methods["dc32_learned"] = dict(s=w3, iters=r3["iters"], res_true_l1=r3["res_true_l1"],
                                        res_true_l2=r3["res_true_l2"], fallback=r3["fallback"],
-                                       speedup_vs_plain=wall / w3, k_mx_pq_us=rk3["kern_seconds"] * 1e6,
-                                       k_mx_pq_gbs=rk3["kern_bytes"] / rk3["kern_seconds"] / 1e9,
+                                       inner_iters=r3["inner_iters"], outer_res=r3["outer_res"],
+                                       speedup_vs_plain=wall / w3, k_mx_q_us=rk3["kern_seconds"] * 1e6,
+                                       k_mx_q_gbs=rk3["kern_bytes"] / rk3["kern_seconds"] / 1e9,
+                                       k_mx_q_bytes=rk3["kern_bytes"],
                                        training=tinfo)
     if "dc32" in args.methods.split(","):
         w2, r2 = timed(fvk.SOLVE_DC32, "dc32_jacobi")
         rk2 = fvk.sys_solve(ctx, fvk.SOLVE_DC32, 0, args.tol, 200000, time_kernel=True)
         methods["dc32_jacobi"] = dict(s=w2, iters=r2["iters"], res_true_l1=r2["res_true_l1"],
                                       res_true_l2=r2["res_true_l2"], speedup_vs_plain=wall / w2,
+                                      inner_iters=r2["inner_iters"], outer_res=r2["outer_res"],
                                       k_mx_pq_us=rk2["kern_seconds"] * 1e6,
                                       k_mx_pq_gbs=rk2["kern_bytes"] / rk2["kern_seconds"] / 1e9)
 
@@ -292,10 +295,10 @@ def main():
         method = ("dc32_learned: network factor G (per-face MLP, bf16 MFMA) + FP32-inner PCG with z = G G^T r "
                   "in FP64 defect correction; one step = factor build + solve")
         dtype = "f32-inner/f64-outer"
-        roof = dict(bound="hbm", kernel="k_mx_pq (FP32: p = z + beta p; q = Ah p; p.q)",
-                    achieved=head["k_mx_pq_gbs"], peak=PEAK_HBM / 1e9, unit="GB/s",
-                    frac=head["k_mx_pq_gbs"] * 1e9 / PEAK_HBM, traffic=None,
-                    us_per_launch=head["k_mx_pq_us"])
+        roof = dict(bound="hbm", kernel="k_mx_q (FP32: q = Ah p; p.q)",
+                    achieved=head["k_mx_q_gbs"], peak=PEAK_HBM / 1e9, unit="GB/s",
+                    frac=head["k_mx_q_gbs"] * 1e9 / PEAK_HBM, traffic=None,
+                    bytes_per_launch=head["k_mx_q_bytes"], us_per_launch=head["k_mx_q_us"])
     else:
         value, hr = wall, dict(iters=r["iters"], res_true_l1=r["res_true_l1"], res_true_l2=r["res_true_l2"])
         method = "plain_pcg64_jacobi"

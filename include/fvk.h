@@ -163,6 +163,9 @@ typedef struct fvk_solve_info {
   double kern_seconds;   /* mean duration of the dominant kernel (time_kernel != 0) */
   double kern_bytes;     /* its algorithmic bytes per launch */
   long kern_launches;
+  int outer;             /* defect-correction steps (DC methods) */
+  int inner_iters[8];    /* iterations of the first 8 inner solves */
+  double outer_res[8];   /* FP64 relative L1 residual before each of them */
 } fvk_solve_info;
 
 #define FVK_SOLVE_PCG64 0 /* the reference PCG recurrence in FP64 (solv_cg.f90) */
@@ -187,7 +190,8 @@ int fvk_sys_get_x(fvk_ctx *ctx, double *x);
 /* Solver settings of a context: "dc_inner_reduc" (the deepest relative L1
    reduction asked of an FP32 inner solve, default 1e-5; each inner solve
    aims at max(dc_inner_reduc, 0.3 tol / current residual)), "dc_max_outer"
-   (defect-correction steps, default 30). */
+   (defect-correction steps, default 30), "dc_carry" (1: a correction step
+   continues the previous step's search direction; 0: restart, default). */
 int fvk_ctx_set_param(fvk_ctx *ctx, const char *name, double value);
 
 /* The factorised preconditioner z = G G^T r, G = D^-1/2 (I + L) Delta, with

@@ -743,6 +743,11 @@ int fvk_sys_solve(fvk_ctx *cc, int method, int imat, double tol, int max_iter, i
                    time_kernel ? &T : nullptr));
     info->iters = R.iters;
     info->res_rec = R.res_l1;
+    info->outer = R.outer;
+    for (int k = 0; k < 8; ++k) {
+      info->inner_iters[k] = R.inner[k];
+      info->outer_res[k] = R.res_before[k];
+    }
     status = R.converged ? FVK_OK : FVK_NOT_CONVERGED;
     if (method == FVK_SOLVE_DC32_FACTOR && !R.converged) {
       // the plain FP64 Jacobi PCG takes over from x = 0
@@ -769,10 +774,14 @@ int fvk_sys_solve(fvk_ctx *cc, int method, int imat, double tol, int max_iter, i
     if (method == FVK_SOLVE_PCG64)
       // k_cg_pq: z, p_old, diag read; p_new, q written; column + value per off-diagonal
       info->kern_bytes = 40.0 * n + 12.0 * nnz_off;
-    else
+    else if (method == FVK_SOLVE_DC32)
       // k_mx_pq on the Jacobi-scaled FP32 system (unit diagonal): z, p_old
       // read; p_new, q written; 16/32-bit column + FP32 value per off-diagonal
       info->kern_bytes = 16.0 * n + (c.mx.i16 ? 6.0 : 8.0) * nnz_off;
+    else
+      // k_mx_q (factor path): p read, q written; column + value per
+      // off-diagonal
+      info->kern_bytes = 8.0 * n + (c.mx.i16 ? 6.0 : 8.0) * nnz_off;
     info->kern_launches = T.launches;
   }
   return status;
@@ -783,6 +792,7 @@ int fvk_ctx_set_param(fvk_ctx *cc, const char *name, double value) {
   Mix &X = cc->c.mx;
   if (!std::strcmp(name, "dc_inner_reduc") && value > 0 && value < 1) X.inner_reduc = value;
   else if (!std::strcmp(name, "dc_max_outer") && value >= 1 && value <= 1000) X.max_outer = (int)value;
+  else if (!std::strcmp(name, "dc_carry") && (value == 0 || value == 1)) X.carry = value != 0;
   else { set_error(std::string("unknown parameter or value out of range: ") + name); return FVK_ERR_ARG; }
   return FVK_OK;
 }

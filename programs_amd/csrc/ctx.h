@@ -65,6 +65,7 @@ struct Mix {
   bool fac_valid = false;
   double inner_reduc = 1e-5;           // deepest relative L1 reduction of an inner solve
   int max_outer = 30;
+  bool carry = false;                  // continue the Krylov recurrence across correction steps
 };
 
 struct Ctx {
@@ -168,6 +169,8 @@ int pcg_tol(Ctx &c, int imat, const double *b, double *x, double tol, int max_it
 struct DcResult {
   int iters = 0, outer = 0, broken = 0;
   double res_l1 = 0.0, res_l2 = 0.0, b_l1 = 0.0;
+  int inner[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double res_before[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   bool converged = false;
 };
 int mix_build(Ctx &c);

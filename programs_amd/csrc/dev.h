@@ -151,6 +151,29 @@ __device__ __forceinline__ void load_sel(const double *part, int P, const int (&
   rblock_sum<NV>(v);
 }
 
+// load_sel in two halves: issue() starts the loads of this thread's (at
+// most two) partials, finish() adds them in load_sel's order and reduces.
+// A kernel issues them before its own row loads, so that the reduction
+// waits only for the partials while the row loads are in flight.
+template <int NV>
+struct SelLoads {
+  double a[NV], b[NV];
+  __device__ __forceinline__ void issue(const double *part, int P, const int (&comp)[NV]) {
+    static_assert(RG <= 2 * RB, "two partials per thread");
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      a[j] = t < P ? part[comp[j] * RG + t] : 0.0;
+      b[j] = t + RB < P ? part[comp[j] * RG + t + RB] : 0.0;
+    }
+  }
+  __device__ __forceinline__ void finish(double (&v)[NV]) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) v[j] = (0.0 + a[j]) + b[j];
+    rblock_sum<NV>(v);
+  }
+};
+
 // Totals of components 0..NV
 template <int NV>
 __device__ __forceinline__ void load_totals(const double *part, int P, double (&v)[NV]) {

@@ -756,12 +756,10 @@ __global__ __launch_bounds__(BS) void k_shift(int n, double *__restrict__ x, con
   if (i < n) x[i] -= *sum / n;
 }
 
-// A solve to tolerance of a singular (pure Neumann) system leaves a constant
-// that depends on the solver's path, and the Green-Gauss gradient of a
-// constant is not zero on warped cells (their face vectors do not close), so
-// the fields would depend on the solver.  Such solutions are shifted to mean
-// zero, whatever solver produced them.
-static int null_fix(Flow &F, double *x) {
+// Whether the current matrix is singular with the constants as its null
+// space (a pure-Neumann system of a closed domain or of one whose outlets
+// fix no pressure): its rows sum to zero.
+static int is_singular(Flow &F, bool &singular) {
   Ctx &c = *F.c;
   const int G = red_grid(c.n);
   double *part = c.part + (size_t)3 * RG * 8;
@@ -777,13 +775,46 @@ static int null_fix(Flow &F, double *x) {
   double d;
   FVK_HIP(hipMemcpyAsync(&d, F.red_scratch, sizeof d, hipMemcpyDeviceToHost, c.stream));
   FVK_HIP(hipStreamSynchronize(c.stream));
-  if (d / c.n > 1e-12) return 0;
+  singular = d / c.n <= 1e-12;
+  return 0;
+}
+
+// x -= mean(x)
+static int zero_mean(Flow &F, double *x) {
+  Ctx &c = *F.c;
   int rc;
   if ((rc = sum_to(c, x, F.red_scratch))) return rc;
   k_shift<<<grid_for(c.n), BS, 0, c.stream>>>(c.n, x, F.red_scratch);
   FVK_HIP(hipGetLastError());
   return 0;
 }
+
+// A singular system is solved to tolerance on its consistent part: the
+// mean of the right-hand side (the net mass imbalance, which no pressure
+// correction can remove) is taken out of a copy of b, or the residual could
+// never fall below it.  The solution is then defined up to a constant that
+// depends on the solver's path, and the Green-Gauss gradient of a constant
+// is not zero on warped cells (their face vectors do not close), so the
+// fields would depend on the solver: such solutions are shifted to mean
+// zero, whatever solver produced them.
+struct NullSpace {
+  bool singular = false;
+  const double *b = nullptr;  // the right-hand side to solve with
+};
+
+static int null_begin(Flow &F, const double *b, NullSpace &N) {
+  int rc;
+  if ((rc = is_singular(F, N.singular))) return rc;
+  N.b = b;
+  if (!N.singular) return 0;
+  // F.g0 (the gradient sweeps' scratch) is free during the pressure solve
+  if ((rc = copy_d(*F.c, b, F.g0, F.c->n))) return rc;
+  if ((rc = zero_mean(F, F.g0))) return rc;
+  N.b = F.g0;
+  return 0;
+}
+
+static int null_end(Flow &F, double *x, const NullSpace &N) { return N.singular ? zero_mean(F, x) : 0; }
 
 // The pressure-correction (or SIMPLER pressure) solve: the reference's PCG
 // (cfd_setup.f90:662,742) with p_solver 0, else a solve from x = 0 to an
@@ -796,18 +827,24 @@ static bool pp_host_driven(const Flow &F) { return F.cfg.p_solver != 0; }
 static int pp_solve(Flow &F, double *x, double *b) {
   Ctx &c = *F.c;
   const int chunk = F.capture ? 0 : 64;
-  const int cap = 200000;
+  // iteration cap of a solve to tolerance: far above what a solve that can
+  // reach p_tol needs, low enough to end one that cannot
+  const int cap = 20000;
   switch (F.cfg.p_solver) {
     case 0:
       return krylov(c, 0, F.cfg.imat_p, F.cfg.reduc_p, F.cfg.n_cg, 1, x, b, true,
                     F.cfg.n_cg > 64 ? chunk : 0);
     case 3: {
       KS h;
-      int rc = pcg_tol(c, 0, b, x, F.cfg.p_tol, cap, chunk, nullptr, h);
-      return rc ? rc : null_fix(F, x);
+      NullSpace N;
+      int rc = null_begin(F, b, N);
+      if (!rc) rc = pcg_tol(c, 0, N.b, x, F.cfg.p_tol, cap, chunk, nullptr, h);
+      return rc ? rc : null_end(F, x, N);
     }
     default: {
       int rc;
+      NullSpace N;
+      if ((rc = null_begin(F, b, N))) return rc;
       const bool fac = F.cfg.p_solver == 2;
       if (fac) {
         if (!F.model) { set_error("p_solver 2 needs a model (fvk_flow_set_model)"); return FVK_ERR_STATE; }
@@ -815,11 +852,11 @@ static int pp_solve(Flow &F, double *x, double *b) {
         if ((rc = learned_factor(c, F.model))) return rc;
       }
       DcResult R;
-      if ((rc = dc_solve(c, fac, b, x, F.cfg.p_tol, cap, R, nullptr))) return rc;
+      if ((rc = dc_solve(c, fac, N.b, x, F.cfg.p_tol, cap, R, nullptr))) return rc;
       if (!R.converged) {
         // the plain FP64 Jacobi PCG from x = 0 takes over
         KS h;
-        if ((rc = pcg_tol(c, 0, b, x, F.cfg.p_tol, cap, chunk, nullptr, h))) return rc;
+        if ((rc = pcg_tol(c, 0, N.b, x, F.cfg.p_tol, cap, chunk, nullptr, h))) return rc;
         k_ks_store<<<1, 1, 0, c.stream>>>(c.ks, R.b_l1, R.iters + h.iter, h.res_n);
         FlowSc s;
         FVK_HIP(hipMemcpy(&s, F.sc, sizeof s, hipMemcpyDeviceToHost));
@@ -829,7 +866,7 @@ static int pp_solve(Flow &F, double *x, double *b) {
         k_ks_store<<<1, 1, 0, c.stream>>>(c.ks, R.b_l1, R.iters, R.res_l1);
       }
       FVK_HIP(hipGetLastError());
-      return null_fix(F, x);
+      return null_end(F, x, N);
     }
   }
 }

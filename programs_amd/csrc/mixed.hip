@@ -14,17 +14,31 @@
 //             eta = max(inner_reduc, 0.3 tol / (|r|_1 / |b|_1))
 //           x += D^-1/2 eh (FP64)
 //
+// With the context setting dc_carry, a correction step after the first
+// continues the Krylov recurrence of the one before (residual replacement,
+// van der Vorst and Ye 2000): its first direction is p = z + beta p_last with
+// the last direction and r.z of the previous inner solve rescaled to the new
+// normalisation, instead of p = z.  It saves iterations when the inner
+// solves stop early (inner_reduc 1e-3), but after deep FP32 inner solves the
+// replaced residual is too far from the recursive one and the continued
+// search can stall, so restarts are the default.
+//
 // The inner system is the Jacobi-scaled one, Ah = D^-1/2 A D^-1/2 (unit
 // diagonal, FP32 off-diagonals, 16-bit column offsets where they fit): its
 // SpMV reads no diagonal, Jacobi PCG on A is plain CG on Ah (z = r), and the
 // factor becomes Gh = (I + L) Delta with zh = Gh Gh^T rh (L in FP16).
-// Per inner iteration three kernels (factor) or two (Jacobi):
-//   k_mx_pq   decide beta; p = z + beta p; q = Ah p; partial p.q
-//   k_mx_upd  alpha; e += alpha p; r' = r - alpha q; partial |r'|_1 and
-//             Jacobi: partial r'.r'
-//             factor: t = Delta^2 (r' + L^T r'), formed on the fly for the
-//             higher neighbours (r is double-buffered)
-//   k_mx_gz   factor: z = t + L t, partial r'.z
+// Per inner iteration two kernels with Jacobi (z = r):
+//   k_mx_pq      decide beta; p = r + beta p; q = Ah p; partial p.q
+//   k_mx_upd_jac alpha; e += alpha p; r -= alpha q; partials |r|_1, r.r
+// and three with the factor:
+//   k_mx_zp      decide beta; e += alpha p (the last update's); z = t + L t;
+//                p = z + beta p (in place)
+//   k_mx_q       q = Ah p; partial p.q
+//   k_mx_upd_fac alpha; r' = r - alpha q; u = r' + L^T r' (r' of the higher
+//                neighbours formed on the fly from r and q, r
+//                double-buffered); t = Delta^2 u; partials |r'|_1 and
+//                r'.z = u^T Delta^2 u (z = (I + L) t never needs storing:
+//                r^T (I + L) Delta^2 (I + L^T) r is that sum).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -82,21 +96,22 @@ __global__ __launch_bounds__(RB) void k_mx_res(DMesh M, const double *__restrict
   store_partials<4>(a, part);
 }
 
-// Start of an inner solve: e = 0, p_old = 0, partials |r|_1 [0] and
-// Jacobi: r.r [3]; factor: t = Delta^2 (r + L^T r).
+// Start of an inner solve: e = 0, p_old = f p_last (p_last null: 0),
+// partials |r|_1 [0] and Jacobi: r.r [3]; factor: t = Delta^2 (r + L^T r).
+// p_last may be p0 itself (each row reads and writes its own element).
 template <int WU, class I>
 __global__ __launch_bounds__(RB) void k_mx_start(int n, int fac, const float *__restrict__ r,
-                                                 float *__restrict__ ev, float *__restrict__ p0, int wu,
-                                                 const I *__restrict__ ucol, const lval_t *__restrict__ uval,
-                                                 const float *__restrict__ ee, float *__restrict__ t,
-                                                 double *__restrict__ part) {
+                                                 float *__restrict__ ev, float *p0, const float *p_last, float f,
+                                                 int wu, const I *__restrict__ ucol,
+                                                 const lval_t *__restrict__ uval, const float *__restrict__ ee,
+                                                 float *__restrict__ t, double *__restrict__ part) {
   double a[4] = {0, 0, 0, 0};
   int r0, r1;
   block_rows(n, r0, r1);
   for (int i = r0 + threadIdx.x; i < r1; i += RB) {
     const float ri = r[i];
     ev[i] = 0.f;
-    p0[i] = 0.f;
+    p0[i] = p_last ? f * p_last[i] : 0.f;
     a[0] += fabs((double)ri);
     if (!fac) {
       a[3] += (double)ri * (double)ri;
@@ -107,13 +122,46 @@ __global__ __launch_bounds__(RB) void k_mx_start(int n, int fac, const float *__
         const size_t o = (size_t)k * n + i;
         acc += (float)uval[o] * r[colat(ucol, o, i)];
       }
-      t[i] = ee[i] * acc;
+      const float ei = ee[i];
+      t[i] = ei * acc;
+      a[3] += (double)ei * (double)acc * (double)acc;
     }
   }
   store_partials<4>(a, part);
 }
 
-// Decide iteration it of the inner PCG; p = z + beta p ; q = Ah p ; p.q
+// Decide iteration it of the inner PCG; p = z + beta p ; q = Ah p ; p.q.
+// A row's loads do not depend on beta: the first row of every thread is
+// loaded before the partials of the previous kernel are reduced, so the
+// reduction's latency hides behind it.
+template <int WS, class I>
+struct PqRow {
+  float zi, pi_old, a[WS > 0 ? WS : 1], zj[WS > 0 ? WS : 1], pj[WS > 0 ? WS : 1];
+  __device__ __forceinline__ void load(int n, int i, const I *__restrict__ col, const float *__restrict__ spv,
+                                       const float *__restrict__ z, const float *__restrict__ po) {
+    zi = z[i];
+    pi_old = po[i];
+#pragma unroll
+    for (int k = 0; k < WS; ++k) {
+      const size_t o = (size_t)k * n + i;
+      const int j = colat(col, o, i);
+      a[k] = spv[o];
+      zj[k] = z[j];
+      pj[k] = po[j];
+    }
+  }
+  __device__ __forceinline__ void finish(int i, float bf, float *__restrict__ pn, float *__restrict__ q,
+                                         double &pq) const {
+    const float p = zi + bf * pi_old;
+    float y = p;
+#pragma unroll
+    for (int k = 0; k < WS; ++k) y += a[k] * (zj[k] + bf * pj[k]);
+    pn[i] = p;
+    q[i] = y;
+    pq += (double)p * (double)y;
+  }
+};
+
 template <int WS, class I>
 __device__ __forceinline__ void mx_pq_row(int n, int i, const I *__restrict__ col, const float *__restrict__ spv,
                                           const float *__restrict__ z, const float *__restrict__ po, float bf,
@@ -131,38 +179,55 @@ __device__ __forceinline__ void mx_pq_row(int n, int i, const I *__restrict__ co
   pq += (double)pi * (double)y;
 }
 
+// The decision of iteration it from the partials |r|_1 [0] and r.z [3] of
+// the last update (issued with mx_issue; thread 0 decides for the block).
+__device__ __forceinline__ void mx_issue(SelLoads<2> &S, const double *partU, int PU) {
+  const int cs[2] = {0, 3};
+  S.issue(partU, PU, cs);
+}
+// take_alpha: the caller applies the last update's e += alpha p itself, so
+// sk[0].alpha keeps that alpha for the block while the committed scalars say
+// 0 (unless the solve is done, when the host applies it).
+__device__ __forceinline__ void mx_decide(int it, SelLoads<2> &S, KS *ring, KS *out, KS *sk,
+                                          bool take_alpha = false) {
+  double t[2];
+  S.finish(t);
+  if (threadIdx.x == 0) {
+    const KS *prev = ring + ((it + 1) & 1) * 3;
+    cg_decide(it, 1, t, t + 1, prev, sk);
+    if (prev->pad) sk[0].done = 1;  // breakdown flagged by the last update
+    if (blockIdx.x == 0) {
+      KS k = sk[0];
+      if (take_alpha && !k.done) k.alpha = 0.0;
+      ring[(it & 1) * 3] = out[0] = k;
+    }
+  }
+  __syncthreads();
+}
+
 template <int WS, class I>
-__global__ __launch_bounds__(RB) void k_mx_pq(int n, int ws, int it, const I *__restrict__ col,
+__global__ __launch_bounds__(RB, 8) void k_mx_pq(int n, int ws, int it, const I *__restrict__ col,
                                               const float *__restrict__ spv, const float *__restrict__ z,
                                               const float *__restrict__ pold, float *__restrict__ pnew,
                                               float *__restrict__ q, const double *__restrict__ partU, int PU,
-                                              const double *__restrict__ partR, int PR, double *__restrict__ partQ,
-                                              KS *ring, KS *out) {
-  double L1[1], rz[1];
-  if (partR) {
-    load_totals<1>(partU, PU, L1);
-    load_totals<1>(partR, PR, rz);
-  } else {
-    const int cs[2] = {0, 3};
-    double t[2];
-    load_sel<2>(partU, PU, cs, t);
-    L1[0] = t[0];
-    rz[0] = t[1];
-  }
+                                              double *__restrict__ partQ, KS *ring, KS *out) {
+  int r0, r1;
+  block_rows(n, r0, r1);
+  const int i0 = r0 + threadIdx.x;
+  SelLoads<2> S;
+  mx_issue(S, partU, PU);
+  PqRow<WS, I> A0;
+  if (WS > 0 && i0 < r1) A0.load(n, i0, col, spv, z, pold);
   __shared__ KS sk[1];
-  if (threadIdx.x == 0) {
-    const KS *prev = ring + ((it + 1) & 1) * 3;
-    cg_decide(it, 1, L1, rz, prev, sk);
-    if (prev->pad) sk[0].done = 1;  // breakdown flagged by the last update
-    if (blockIdx.x == 0) ring[(it & 1) * 3] = out[0] = sk[0];
-  }
-  __syncthreads();
+  mx_decide(it, S, ring, out, sk);
   if (sk[0].done) return;
   const float bf = (float)sk[0].beta;
   double pq = 0.0;
-  int r0, r1;
-  block_rows(n, r0, r1);
-  int i = r0 + threadIdx.x;
+  int i = i0;
+  if (WS > 0) {
+    if (i0 < r1) A0.finish(i0, bf, pnew, q, pq);
+    i = i0 + RB;
+  }
   for (; i + RB < r1; i += 2 * RB) {
     mx_pq_row<WS, I>(n, i, col, spv, z, pold, bf, pnew, q, ws, pq);
     mx_pq_row<WS, I>(n, i + RB, col, spv, z, pold, bf, pnew, q, ws, pq);
@@ -172,11 +237,129 @@ __global__ __launch_bounds__(RB) void k_mx_pq(int n, int ws, int it, const I *__
   store_partials<1>(v, partQ);
 }
 
+// Factor, iteration it: decide beta; e += alpha p with the last update's
+// alpha and direction; z = t + L t; p = z + beta p in place.  The first two
+// rows of a thread are loaded before the decision is known.
+template <int WL, class I>
+__device__ __forceinline__ float mx_z_row(int n, int i, int wl, const I *__restrict__ lcol,
+                                          const lval_t *__restrict__ lval, const float *__restrict__ t) {
+  float acc = t[i];
+#pragma unroll
+  for (int k = 0; k < nw<WL>(wl); ++k) {
+    const size_t o = (size_t)k * n + i;
+    acc += (float)lval[o] * t[colat(lcol, o, i)];
+  }
+  return acc;
+}
+
+template <int WL, class I>
+struct ZRow {
+  float ti, pi, ei, l[WL > 0 ? WL : 1], tj[WL > 0 ? WL : 1];
+  __device__ __forceinline__ void load(int n, int i, const I *__restrict__ lcol, const lval_t *__restrict__ lval,
+                                       const float *__restrict__ t, const float *__restrict__ p,
+                                       const float *__restrict__ ev) {
+    ti = t[i];
+    pi = p[i];
+    ei = ev[i];
+#pragma unroll
+    for (int k = 0; k < WL; ++k) {
+      const size_t o = (size_t)k * n + i;
+      l[k] = (float)lval[o];
+      tj[k] = t[colat(lcol, o, i)];
+    }
+  }
+  __device__ __forceinline__ float z() const {
+    float acc = ti;
+#pragma unroll
+    for (int k = 0; k < WL; ++k) acc += l[k] * tj[k];
+    return acc;
+  }
+};
+
+template <int WL, class I>
+__global__ __launch_bounds__(RB, 8) void k_mx_zp(int n, int wl, int it, const I *__restrict__ lcol,
+                                              const lval_t *__restrict__ lval, const float *__restrict__ t,
+                                              float *__restrict__ p, float *__restrict__ ev,
+                                              const double *__restrict__ partU, int PU, KS *ring, KS *out) {
+  int r0, r1;
+  block_rows(n, r0, r1);
+  const int i0 = r0 + threadIdx.x, i1 = i0 + RB;
+  SelLoads<2> S;
+  mx_issue(S, partU, PU);
+  ZRow<WL, I> A0, A1;
+  if (WL > 0) {
+    if (i0 < r1) A0.load(n, i0, lcol, lval, t, p, ev);
+    if (i1 < r1) A1.load(n, i1, lcol, lval, t, p, ev);
+  }
+  __shared__ KS sk[1];
+  mx_decide(it, S, ring, out, sk, true);
+  if (sk[0].done) return;
+  const float bf = (float)sk[0].beta, af = (float)sk[0].alpha;
+  int i = i0;
+  if (WL > 0) {
+    if (i0 < r1) {
+      ev[i0] = A0.ei + af * A0.pi;
+      p[i0] = A0.z() + bf * A0.pi;
+    }
+    if (i1 < r1) {
+      ev[i1] = A1.ei + af * A1.pi;
+      p[i1] = A1.z() + bf * A1.pi;
+    }
+    i = i0 + 2 * RB;
+  }
+  auto row = [&](int i) {
+    const float z = mx_z_row<WL, I>(n, i, wl, lcol, lval, t), pi = p[i];
+    ev[i] += af * pi;
+    p[i] = z + bf * pi;
+  };
+  for (; i + RB < r1; i += 2 * RB) {
+    row(i);
+    row(i + RB);
+  }
+  if (i < r1) row(i);
+}
+
+// Factor, iteration it: q = Ah p; partial p.q (nothing once the solve is done)
+template <int WS, class I>
+__global__ __launch_bounds__(RB) void k_mx_q(int n, int ws, int it, const I *__restrict__ col,
+                                             const float *__restrict__ spv, const float *__restrict__ p,
+                                             float *__restrict__ q, double *__restrict__ partQ, const KS *ring) {
+  if (ring[(it & 1) * 3].done) return;
+  int r0, r1;
+  block_rows(n, r0, r1);
+  double pq = 0.0;
+  auto row = [&](int i) {
+    const float pi = p[i];
+    float y = pi;
+#pragma unroll
+    for (int k = 0; k < nw<WS>(ws); ++k) {
+      const size_t o = (size_t)k * n + i;
+      y += spv[o] * p[colat(col, o, i)];
+    }
+    q[i] = y;
+    pq += (double)pi * (double)y;
+  };
+  int i = r0 + threadIdx.x;
+  for (; i + RB < r1; i += 2 * RB) {
+    row(i);
+    row(i + RB);
+  }
+  if (i < r1) row(i);
+  double v[1] = {pq};
+  store_partials<1>(v, partQ);
+}
+
+// The partials of p.q, issued before an update kernel's row loads
+__device__ __forceinline__ void mx_issue_q(SelLoads<1> &S, const double *partQ, int PQ) {
+  const int cs[1] = {0};
+  S.issue(partQ, PQ, cs);
+}
+
 // alpha = s/(p.q + small) from the partials; false on breakdown (a
 // non-positive or non-finite p.q), flagged for the next decision.
-__device__ __forceinline__ bool mx_alpha(KS *cur, const double *partQ, int PQ, double &alpha) {
+__device__ __forceinline__ bool mx_alpha(KS *cur, SelLoads<1> &S, double &alpha) {
   double tq[1];
-  load_totals<1>(partQ, PQ, tq);
+  S.finish(tq);
   alpha = cur->s / (tq[0] + SMALL);
   if (!(tq[0] > 0.0) || !isfinite(alpha)) {
     if (blockIdx.x == 0 && threadIdx.x == 0) cur->pad = 1;
@@ -185,27 +368,37 @@ __device__ __forceinline__ bool mx_alpha(KS *cur, const double *partQ, int PQ, d
   return true;
 }
 
-// Jacobi: e += alpha p ; r -= alpha q (in place) ; partials |r|_1, r.r
+// Jacobi: e += alpha p ; r -= alpha q (in place) ; partials |r|_1, r.r.
+// The first two rows of a thread are loaded before alpha is known.
 __global__ __launch_bounds__(RB) void k_mx_upd_jac(int n, int it, float *__restrict__ ev, float *__restrict__ r,
                                                    const float *__restrict__ p, const float *__restrict__ q,
                                                    const double *__restrict__ partQ, int PQ,
                                                    double *__restrict__ partU, KS *ring) {
+  int r0, r1;
+  block_rows(n, r0, r1);
+  const int i0 = r0 + threadIdx.x, i1 = i0 + RB;
+  SelLoads<1> S;
+  mx_issue_q(S, partQ, PQ);
+  float e0 = 0.f, e1 = 0.f, ra0 = 0.f, ra1 = 0.f, p0 = 0.f, p1 = 0.f, q0 = 0.f, q1 = 0.f;
+  if (i0 < r1) { e0 = ev[i0]; ra0 = r[i0]; p0 = p[i0]; q0 = q[i0]; }
+  if (i1 < r1) { e1 = ev[i1]; ra1 = r[i1]; p1 = p[i1]; q1 = q[i1]; }
   KS *cur = ring + (it & 1) * 3;
   if (cur->done) return;
   double alpha;
-  if (!mx_alpha(cur, partQ, PQ, alpha)) return;
+  if (!mx_alpha(cur, S, alpha)) return;
   const float af = (float)alpha;
   double a[4] = {0, 0, 0, 0};
-  int r0, r1;
-  block_rows(n, r0, r1);
-  auto row = [&](int i) {
-    ev[i] += af * p[i];
-    const float ri = r[i] - af * q[i];
+  auto fin = [&](int i, float e, float ri_, float pi, float qi) {
+    ev[i] = e + af * pi;
+    const float ri = ri_ - af * qi;
     r[i] = ri;
     a[0] += fabs((double)ri);
     a[3] += (double)ri * (double)ri;
   };
-  int i = r0 + threadIdx.x;
+  if (i0 < r1) fin(i0, e0, ra0, p0, q0);
+  if (i1 < r1) fin(i1, e1, ra1, p1, q1);
+  auto row = [&](int i) { fin(i, ev[i], r[i], p[i], q[i]); };
+  int i = i0 + 2 * RB;
   for (; i + RB < r1; i += 2 * RB) {
     row(i);
     row(i + RB);
@@ -215,26 +408,65 @@ __global__ __launch_bounds__(RB) void k_mx_upd_jac(int n, int it, float *__restr
   if (blockIdx.x == 0 && threadIdx.x == 0) cur->alpha = alpha;
 }
 
-// Factor: e += alpha p ; r' = r - alpha q ; t = Delta^2 (r' + L^T r') with
-// r' of the higher neighbours formed from (r, q) ; partial |r'|_1
+// Factor: r' = r - alpha q ; u = r' + L^T r' with r' of the higher
+// neighbours formed from (r, q) ; t = Delta^2 u ; partials |r'|_1 [0] and
+// r'.z = u Delta^2 u [3] (e += alpha p is left to the next k_mx_zp, which
+// reads p anyway).  The first rows of a thread are loaded before alpha is
+// known.
 template <int WU, class I>
-__global__ __launch_bounds__(RB) void k_mx_upd_fac(int n, int it, float *__restrict__ ev,
-                                                   const float *__restrict__ rin, float *__restrict__ rout,
-                                                   const float *__restrict__ p, const float *__restrict__ q, int wu,
+struct UpdRow {
+  float ri, qi, ee, u[WU > 0 ? WU : 1], rj[WU > 0 ? WU : 1], qj[WU > 0 ? WU : 1];
+  __device__ __forceinline__ void load(int n, int i, const float *__restrict__ rin, const float *__restrict__ q,
+                                       const I *__restrict__ ucol, const lval_t *__restrict__ uval,
+                                       const float *__restrict__ eev) {
+    ri = rin[i];
+    qi = q[i];
+    ee = eev[i];
+#pragma unroll
+    for (int k = 0; k < WU; ++k) {
+      const size_t o = (size_t)k * n + i;
+      const int j = colat(ucol, o, i);
+      u[k] = (float)uval[o];
+      rj[k] = rin[j];
+      qj[k] = q[j];
+    }
+  }
+};
+
+template <int WU, class I>
+__global__ __launch_bounds__(RB, 8) void k_mx_upd_fac(int n, int it, const float *__restrict__ rin,
+                                                      float *__restrict__ rout, const float *__restrict__ q, int wu,
                                                    const I *__restrict__ ucol, const lval_t *__restrict__ uval,
                                                    const float *__restrict__ ee, float *__restrict__ t,
                                                    const double *__restrict__ partQ, int PQ,
                                                    double *__restrict__ partU, KS *ring) {
+  int r0, r1;
+  block_rows(n, r0, r1);
+  const int i0 = r0 + threadIdx.x, i1 = i0 + RB;
+  SelLoads<1> S;
+  mx_issue_q(S, partQ, PQ);
+  // rows loaded ahead: two, one for the widest tables (registers)
+  constexpr int NA = WU == 0 ? 0 : (WU <= 4 ? 2 : 1);
+  UpdRow<WU, I> A0, A1;
+  if (NA > 0 && i0 < r1) A0.load(n, i0, rin, q, ucol, uval, ee);
+  if (NA > 1 && i1 < r1) A1.load(n, i1, rin, q, ucol, uval, ee);
   KS *cur = ring + (it & 1) * 3;
   if (cur->done) return;
   double alpha;
-  if (!mx_alpha(cur, partQ, PQ, alpha)) return;
+  if (!mx_alpha(cur, S, alpha)) return;
   const float af = (float)alpha;
   double a[4] = {0, 0, 0, 0};
-  int r0, r1;
-  block_rows(n, r0, r1);
+  auto fin = [&](int i, const UpdRow<WU, I> &R) {
+    const float ri = R.ri - af * R.qi;
+    rout[i] = ri;
+    a[0] += fabs((double)ri);
+    float acc = ri;
+#pragma unroll
+    for (int k = 0; k < WU; ++k) acc += R.u[k] * (R.rj[k] - af * R.qj[k]);
+    t[i] = R.ee * acc;
+    a[3] += (double)R.ee * (double)acc * (double)acc;
+  };
   auto row = [&](int i) {
-    ev[i] += af * p[i];
     const float ri = rin[i] - af * q[i];
     rout[i] = ri;
     a[0] += fabs((double)ri);
@@ -245,9 +477,14 @@ __global__ __launch_bounds__(RB) void k_mx_upd_fac(int n, int it, float *__restr
       const int j = colat(ucol, o, i);
       acc += (float)uval[o] * (rin[j] - af * q[j]);
     }
-    t[i] = ee[i] * acc;
+    const float ei = ee[i];
+    t[i] = ei * acc;
+    a[3] += (double)ei * (double)acc * (double)acc;
   };
-  int i = r0 + threadIdx.x;
+  int i = i0;
+  if (NA > 0 && i0 < r1) fin(i0, A0);
+  if (NA > 1 && i1 < r1) fin(i1, A1);
+  i = i0 + NA * RB;
   for (; i + RB < r1; i += 2 * RB) {
     row(i);
     row(i + RB);
@@ -287,23 +524,14 @@ __global__ __launch_bounds__(RB) void k_mx_gz(int n, int wl, const I *__restrict
 }
 
 // Final decision after the last queued inner iteration (one block).
-__global__ __launch_bounds__(RB) void k_mx_final(int it, const double *partU, int PU, const double *partR, int PR,
-                                                 KS *ring, KS *out) {
-  double L1[1], rz[1];
-  if (partR) {
-    load_totals<1>(partU, PU, L1);
-    load_totals<1>(partR, PR, rz);
-  } else {
-    const int cs[2] = {0, 3};
-    double t[2];
-    load_sel<2>(partU, PU, cs, t);
-    L1[0] = t[0];
-    rz[0] = t[1];
-  }
+__global__ __launch_bounds__(RB) void k_mx_final(int it, const double *partU, int PU, KS *ring, KS *out) {
+  const int cs[2] = {0, 3};
+  double tt[2];
+  load_sel<2>(partU, PU, cs, tt);
   if (threadIdx.x == 0) {
     KS cur[1];
     const KS *prev = ring + ((it + 1) & 1) * 3;
-    cg_decide(it, 1, L1, rz, prev, cur);
+    cg_decide(it, 1, tt, tt + 1, prev, cur);
     if (prev->pad) cur[0].done = 1;
     out[0] = cur[0];
   }
@@ -322,11 +550,15 @@ __global__ __launch_bounds__(BS) void k_scale32(int n, float *__restrict__ a, fl
   if (i < n) a[i] *= s;
 }
 
-// x += sigma D^-1/2 eh
+// x += sigma D^-1/2 eh with eh = e + alpha p (the factor path's last update,
+// not yet applied to e; p null: none)
 __global__ __launch_bounds__(BS) void k_mx_add(int n, double *__restrict__ x, const float *__restrict__ ev,
+                                               const float *__restrict__ p, float alpha,
                                                const double *__restrict__ diag, double sigma) {
   const int i = blockIdx.x * BS + threadIdx.x;
-  if (i < n) x[i] += sigma * (double)ev[i] / sqrt(diag[i]);
+  if (i >= n) return;
+  const float e = p ? ev[i] + alpha * p[i] : ev[i];
+  x[i] += sigma * (double)e / sqrt(diag[i]);
 }
 
 // Factor from per-interior-face (l, sig): e = exp(2 s) with s the sum of sig
@@ -474,64 +706,86 @@ int factor_assemble(Ctx &c) {
 enum { V_E = 0, V_R0, V_R1, V_Z, V_P0, V_P1, V_Q, V_T, V_A, V_B };
 static inline float *fv(Ctx &c, int k) { return c.mx.vec + (size_t)k * c.n; }
 
+// State carried from one inner solve to the next: the buffer holding the
+// last search direction and the r.z it was formed with.
+struct InnerCarry {
+  int p_buf = -1;  // index into the direction pair, -1: none (restart)
+  double s = 0.0;
+};
+
+// One inner PCG solve of Ah eh = rh (rh in V_R0).  carry in: the previous
+// solve's last direction and r.z, to be scaled by f (rows) and f^2 (r.z);
+// carry out: this solve's.
+// alpha_pend: the factor path's last alpha, whose e += alpha p (p in P[0])
+// is still to be applied.
 template <class I>
-static int inner_solve(Ctx &c, bool fac, int n_iter, double reduc, int &iters, int &broken, KTimer *T) {
+static int inner_solve(Ctx &c, bool fac, int n_iter, double reduc, InnerCarry &carry, float f, int &iters,
+                       int &broken, double &alpha_pend, KTimer *T) {
   Mix &X = c.mx;
   const int n = c.n, G = red_grid(n), ws = c.Ws, wl = X.WL, wu = X.WU;
   const I *col = (const I *)X.col, *lcol = (const I *)X.lcol, *ucol = (const I *)X.ucol;
   const lval_t *lval = (const lval_t *)X.lval, *uval = (const lval_t *)X.uval;
-  // scalars: reduc, n_iter, s = large
+  float *r[2] = {fv(c, V_R0), fv(c, V_R1)}, *P[2] = {fv(c, V_P0), fv(c, V_P1)};
+  const float *p_last = carry.p_buf >= 0 ? P[carry.p_buf] : nullptr;
+  // scalars: reduc, n_iter, s = large (restart) or the carried r.z
   KS h[6];
   std::memset(h, 0, sizeof h);
   for (int s = 0; s < 2; ++s) {
     h[s * 3].reduc = reduc;
     h[s * 3].n_iter = n_iter;
-    h[s * 3].s = 1.0 / SMALL;
+    h[s * 3].s = p_last ? carry.s * (double)f * (double)f : 1.0 / SMALL;
   }
   FVK_HIP(hipMemcpyAsync(c.ksr, h, sizeof h, hipMemcpyHostToDevice, c.stream));
-  float *r[2] = {fv(c, V_R0), fv(c, V_R1)}, *P[2] = {fv(c, V_P0), fv(c, V_P1)};
-  // Jacobi: z is r itself, updated in place
-  float *z = fac ? fv(c, V_Z) : r[0], *q = fv(c, V_Q), *t = fv(c, V_T), *ev = fv(c, V_E);
-#define L(W) \
-  k_mx_start<W, I><<<G, RB, 0, c.stream>>>(n, fac ? 1 : 0, r[0], ev, P[0], wu, ucol, uval, X.e, t, pb(c, 0))
+  float *q = fv(c, V_Q), *t = fv(c, V_T), *ev = fv(c, V_E);
+  // partials: [0] |r|_1 and [3] r.z of the last update, [1] p.q
+  double *pU = pb(c, 0), *pQ = pb(c, 1);
+#define L(W)                                                                                                   \
+  k_mx_start<W, I><<<G, RB, 0, c.stream>>>(n, fac ? 1 : 0, r[0], ev, P[0], p_last, f, wu, ucol, uval, X.e, t, \
+                                           pU)
   FVK_W(wu, L);
 #undef L
-  const double *pR = nullptr;
-  if (fac) {
-#define L(W) k_mx_gz<W, I><<<G, RB, 0, c.stream>>>(n, wl, lcol, lval, t, r[0], z, pb(c, 2), nullptr)
-    FVK_W(wl, L);
-#undef L
-    pR = pb(c, 2);
-  }
   const int chunk = 32;
   int it = 0;
   for (; it < n_iter; ++it) {
-    float *pold = P[it & 1], *pnew = P[(it + 1) & 1], *rin = r[it & 1], *rout = r[(it + 1) & 1];
     hipEvent_t ea, eb;
-    if (T && T->take(ea, eb)) {
-      const float *zc = z, *pc = pold, *sp = X.spv;
-#define L(W)                                                                                                  \
-  hipExtLaunchKernelGGL(k_mx_pq<W, I>, dim3(G), dim3(RB), 0, c.stream, ea, eb, 0, n, ws, it, col, sp, zc, pc, pnew, \
-                        q, (const double *)pb(c, 0), G, pR, G, pb(c, 1), c.ksr, c.ks)
-      FVK_W(ws, L);
-#undef L
-    } else {
-#define L(W)                                                                                                  \
-  k_mx_pq<W, I><<<G, RB, 0, c.stream>>>(n, ws, it, col, X.spv, z, pold, pnew, q, pb(c, 0), G, pR, G, pb(c, 1), \
-                                        c.ksr, c.ks)
-      FVK_W(ws, L);
-#undef L
-    }
+    const bool stamp = T && T->take(ea, eb);
     if (!fac) {
-      k_mx_upd_jac<<<G, RB, 0, c.stream>>>(n, it, ev, r[0], pnew, q, pb(c, 1), G, pb(c, 0), c.ksr);
-    } else {
-#define L(W)                                                                                                   \
-  k_mx_upd_fac<W, I><<<G, RB, 0, c.stream>>>(n, it, ev, rin, rout, pnew, q, wu, ucol, uval, X.e, t, pb(c, 1), G, \
-                                             pb(c, 0), c.ksr)
-      FVK_W(wu, L);
+      // Jacobi: z is r itself, updated in place; p double-buffered
+      const float *pold = P[it & 1], *sp = X.spv, *z = r[0];
+      float *pnew = P[(it + 1) & 1];
+      if (stamp) {
+#define L(W)                                                                                                       \
+  hipExtLaunchKernelGGL(k_mx_pq<W, I>, dim3(G), dim3(RB), 0, c.stream, ea, eb, 0, n, ws, it, col, sp, z, pold, pnew, \
+                        q, (const double *)pU, G, pQ, c.ksr, c.ks)
+        FVK_W(ws, L);
 #undef L
-#define L(W) k_mx_gz<W, I><<<G, RB, 0, c.stream>>>(n, wl, lcol, lval, t, rout, z, pb(c, 2), c.ksr + (it & 1) * 3)
+      } else {
+#define L(W) k_mx_pq<W, I><<<G, RB, 0, c.stream>>>(n, ws, it, col, sp, z, pold, pnew, q, pU, G, pQ, c.ksr, c.ks)
+        FVK_W(ws, L);
+#undef L
+      }
+      k_mx_upd_jac<<<G, RB, 0, c.stream>>>(n, it, ev, r[0], pnew, q, pQ, G, pU, c.ksr);
+    } else {
+      // factor: p in place in P[0]; r double-buffered
+      float *p = P[0], *rin = r[it & 1], *rout = r[(it + 1) & 1];
+      const float *sp = X.spv, *pc = p;
+#define L(W) k_mx_zp<W, I><<<G, RB, 0, c.stream>>>(n, wl, it, lcol, lval, t, p, ev, pU, G, c.ksr, c.ks)
       FVK_W(wl, L);
+#undef L
+      if (stamp) {
+#define L(W)                                                                                                     \
+  hipExtLaunchKernelGGL(k_mx_q<W, I>, dim3(G), dim3(RB), 0, c.stream, ea, eb, 0, n, ws, it, col, sp, pc, q, pQ, \
+                        (const KS *)c.ksr)
+        FVK_W(ws, L);
+#undef L
+      } else {
+#define L(W) k_mx_q<W, I><<<G, RB, 0, c.stream>>>(n, ws, it, col, sp, pc, q, pQ, c.ksr)
+        FVK_W(ws, L);
+#undef L
+      }
+#define L(W)                                                                                                        \
+  k_mx_upd_fac<W, I><<<G, RB, 0, c.stream>>>(n, it, rin, rout, q, wu, ucol, uval, X.e, t, pQ, G, pU, c.ksr)
+      FVK_W(wu, L);
 #undef L
     }
     if ((it + 1) % chunk == 0 && it + 1 < n_iter) {
@@ -542,7 +796,7 @@ static int inner_solve(Ctx &c, bool fac, int n_iter, double reduc, int &iters, i
       if (k.done) { ++it; break; }
     }
   }
-  k_mx_final<<<1, RB, 0, c.stream>>>(it, pb(c, 0), G, pR, G, c.ksr, c.ks);
+  k_mx_final<<<1, RB, 0, c.stream>>>(it, pU, G, c.ksr, c.ks);
   FVK_HIP(hipGetLastError());
   if (T && T->flush()) return FVK_ERR_HIP;
   KS k, kr[2];
@@ -552,8 +806,14 @@ static int inner_solve(Ctx &c, bool fac, int n_iter, double reduc, int &iters, i
   FVK_HIP(hipStreamSynchronize(c.stream));
   iters = k.iter;
   broken = kr[0].pad || kr[1].pad;
-  // the solution lives in r[..] only through e; the last residual buffer is
-  // not needed after the FP64 residual is recomputed
+  // the decision that ended the solve kept the last update's alpha (0 when
+  // k_mx_zp had applied it, or when no update ran)
+  alpha_pend = fac ? k.alpha : 0.0;
+  // The last direction: the factor path updates P[0] in place; with Jacobi
+  // iteration j writes P[(j + 1) & 1], so the last of the iters iterations
+  // left it in P[iters & 1] (P[0] when none ran).
+  carry.p_buf = broken || !(k.s > 0.0) || !std::isfinite(k.s) ? -1 : (fac ? 0 : (iters & 1));
+  carry.s = k.s;
   return 0;
 }
 
@@ -570,6 +830,8 @@ int dc_solve(Ctx &c, bool use_factor, const double *b, double *x, double tol, in
   k_mx_conv<<<grid_for(n), BS, 0, c.stream>>>(c.M, c.diag, c.spv, X.spv, X.w);
   FVK_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, c.stream));
   res = DcResult();
+  InnerCarry carry;
+  double sigma_prev = 0.0;
   for (int outer = 0;; ++outer) {
 #define L(W) k_mx_res<W><<<G, RB, 0, c.stream>>>(c.M, c.diag, c.spv, x, b, fv(c, V_R0), pb(c, 3))
     FVK_W(c.Ws, L);
@@ -595,12 +857,21 @@ int dc_solve(Ctx &c, bool use_factor, const double *b, double *x, double tol, in
     const double sigma = h[0] > 0 ? h[0] / n : 1.0;
     k_scale32<<<grid_for(n), BS, 0, c.stream>>>(n, fv(c, V_R0), (float)(1.0 / sigma));
     int its = 0, broken = 0;
-    rc = X.i16 ? inner_solve<short>(c, use_factor, max_iter - res.iters, eta, its, broken, T)
-               : inner_solve<int>(c, use_factor, max_iter - res.iters, eta, its, broken, T);
+    if (!X.carry) carry.p_buf = -1;
+    const float f = carry.p_buf >= 0 ? (float)(sigma_prev / sigma) : 1.f;
+    double alpha_pend = 0.0;
+    rc = X.i16 ? inner_solve<short>(c, use_factor, max_iter - res.iters, eta, carry, f, its, broken, alpha_pend, T)
+               : inner_solve<int>(c, use_factor, max_iter - res.iters, eta, carry, f, its, broken, alpha_pend, T);
     if (rc) return rc;
+    sigma_prev = sigma;
+    if (outer < 8) {
+      res.inner[outer] = its;
+      res.res_before[outer] = res.res_l1;
+    }
     res.iters += its;
     res.broken |= broken;
-    k_mx_add<<<grid_for(n), BS, 0, c.stream>>>(n, x, fv(c, V_E), c.diag, sigma);
+    k_mx_add<<<grid_for(n), BS, 0, c.stream>>>(n, x, fv(c, V_E), use_factor ? fv(c, V_P0) : nullptr,
+                                               (float)alpha_pend, c.diag, sigma);
     FVK_HIP(hipGetLastError());
   }
   return 0;
@@ -625,7 +896,8 @@ static int apply_t(Ctx &c, const double *r, double *zout) {
   const lval_t *lval = (const lval_t *)X.lval, *uval = (const lval_t *)X.uval;
   float *r32 = fv(c, V_A), *z = fv(c, V_B), *t = fv(c, V_T), *ev = fv(c, V_E), *p0 = fv(c, V_P0);
   k_scale_f32<<<grid_for(n), BS, 0, c.stream>>>(n, r, c.diag, r32);
-#define L(W) k_mx_start<W, I><<<G, RB, 0, c.stream>>>(n, 1, r32, ev, p0, wu, ucol, uval, X.e, t, pb(c, 0))
+#define L(W) \
+  k_mx_start<W, I><<<G, RB, 0, c.stream>>>(n, 1, r32, ev, p0, nullptr, 1.f, wu, ucol, uval, X.e, t, pb(c, 0))
   FVK_W(wu, L);
 #undef L
 #define L(W) k_mx_gz<W, I><<<G, RB, 0, c.stream>>>(n, wl, lcol, lval, t, r32, z, pb(c, 2), nullptr)

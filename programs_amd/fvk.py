@@ -53,7 +53,8 @@ class Stats(C.Structure):
 class SolveInfo(C.Structure):
     _fields_ = [("iters", C.c_int), ("fallback", C.c_int), ("seconds", C.c_double), ("res_rec", C.c_double),
                 ("res_true_l1", C.c_double), ("res_true_l2", C.c_double), ("kern_seconds", C.c_double),
-                ("kern_bytes", C.c_double), ("kern_launches", C.c_long)]
+                ("kern_bytes", C.c_double), ("kern_launches", C.c_long), ("outer", C.c_int),
+                ("inner_iters", C.c_int * 8), ("outer_res", C.c_double * 8)]
 
 
 SOLVE_PCG64 = 0         # reference PCG recurrence, FP64
@@ -392,6 +393,8 @@ def sys_solve(ctx: Ctx, method=SOLVE_PCG64, imat=0, tol=1e-8, max_iter=100000, t
     info = SolveInfo()
     rc = _check(lib().fvk_sys_solve(ctx.h, method, imat, tol, max_iter, int(bool(time_kernel)), C.byref(info)))
     out = {k: getattr(info, k) for k, _ in SolveInfo._fields_}
+    out["inner_iters"] = list(info.inner_iters)[:max(info.outer, 0)]
+    out["outer_res"] = list(info.outer_res)[:max(info.outer, 0)]
     out["status"] = rc
     return out
 

@@ -5,8 +5,12 @@ The GPU flow against the oracle on the same mesh after a fixed number of
 outer iterations, reference pressure solves (relaxed update, Jacobi):
 fields and histories to 1e-10 relative, or 30x the oracle's own change under
 pairwise summation where that is larger (as in test_gpu_parity).  Then the
-same channel with the pressure-correction systems solved to 1e-11 by plain
-FP64 PCG and by the learned FP32-inner solve: fields agree to 1e-8.
+same channel with the pressure-correction systems solved to 1e-10 by plain
+FP64 PCG and by the learned FP32-inner solve (replayed from HIP graphs): every
+solve converges and the fields agree to 1e-6 after 12 outer iterations.  The
+outlet fixes no pressure, so each pressure-correction system is singular and
+its right-hand side carries the net mass imbalance, which no correction can
+remove: solves to tolerance work on the consistent part (mean taken out).
 """
 import numpy as np
 import pytest
@@ -71,16 +75,18 @@ def test_channel_learned_pressure_matches_plain():
     for ps in (3, 2):
         ctx = fvk.grd_solv(m)
         fl = fvk.Flow(ctx, fvk.flow_cfg(simpler=0, imat_u=0, imat_p=0, urf_p=0.3, mu=MU, p_solver=ps,
-                                        p_tol=1e-11))
+                                        p_tol=1e-10))
         fl.set_bnd_u(11, [1.0, 0.0])
         if ps == 2:
             fl.set_model(model, train_steps=20, lr=3e-3)
         h = fl.run_graph(1, 12)
         out[ps] = fl.state()
-        print("p_solver", ps, "p' iterations", [x["it_pp"] for x in h])
+        its = [x["it_pp"] for x in h]
+        print("p_solver", ps, "p' iterations", its)
+        assert max(its) < 20000, "a pressure solve hit the iteration cap"
         ctx.destroy()
     model.destroy()
     for k in ("u_cell", "p"):
         e = np.max(np.abs(out[2][k] - out[3][k])) / np.max(np.abs(out[3][k]))
         print(k, e)
-        assert e < 1e-8, (k, e)
+        assert e < 1e-6, (k, e)
