@@ -163,6 +163,8 @@ static int ctx_build(Ctx &c, const Topo &t, int device, int flags) {
   FVK_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
   FVK_HIP(hipEventCreate(&c.ev0));
   FVK_HIP(hipEventCreate(&c.ev1));
+  FVK_HIP(hipHostMalloc((void **)&c.poll_h, 2 * sizeof(KS), hipHostMallocDefault));
+  for (int k = 0; k < 2; ++k) FVK_HIP(hipEventCreateWithFlags(&c.poll_ev[k], hipEventDisableTiming));
   const int n = t.n, nf = t.nf;
   c.n = n; c.nf = nf; c.nbf = t.nbf; c.ndx = t.n_dx;
   // row order
@@ -253,6 +255,10 @@ static void ctx_free(Ctx &c) {
   c.owned.clear();
   if (c.ev0) hipEventDestroy(c.ev0);
   if (c.ev1) hipEventDestroy(c.ev1);
+  for (int k = 0; k < 2; ++k)
+    if (c.poll_ev[k]) hipEventDestroy(c.poll_ev[k]);
+  if (c.poll_h) hipHostFree(c.poll_h);
+  c.poll_h = nullptr;
   if (c.stream) hipStreamDestroy(c.stream);
   c.stream = nullptr;
 }

@@ -95,6 +95,9 @@ struct Ctx {
   double *bnd_buf = nullptr;      // 3 x n_bf boundary values (fvk_solv_grad)
   Mix mx;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // host polls of a solve's decisions: two pinned slots and their events
+  KS *poll_h = nullptr;
+  hipEvent_t poll_ev[2] = {nullptr, nullptr};
 };
 
 template <class T>

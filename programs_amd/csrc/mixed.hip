@@ -59,6 +59,14 @@ __device__ __forceinline__ int nw(int w) { return W > 0 ? W : w; }
 
 __device__ __forceinline__ bool gate_done(const KS *g) { return g && g->done; }
 
+// Stride of the FP32 work vectors and of the slot-major tables of the
+// inner system (Ah's columns and values, L's columns and values): n plus
+// 64 KiB + 256 B, so that the same row of different vectors or slots does
+// not sit at the same offset of its memory page.  With strides a power of
+// two apart the streams of one kernel collided in the same HBM channels
+// (5 % slower on the 2^21-row system for the vectors alone).
+__host__ __device__ __forceinline__ size_t mx_ld(int n) { return (size_t)n + 16448; }
+
 // Jacobi-scaled FP32 copy of the matrix, w = a_ii^-1/2
 __global__ __launch_bounds__(BS) void k_mx_conv(DMesh M, const double *__restrict__ diag,
                                                 const double *__restrict__ spv, float *__restrict__ s32,
@@ -71,7 +79,7 @@ __global__ __launch_bounds__(BS) void k_mx_conv(DMesh M, const double *__restric
   for (int k = 0; k < M.Ws; ++k) {
     const size_t o = (size_t)k * n + i;
     const int j = M.spc[o];
-    s32[o] = j >= 0 ? (float)(spv[o] * wi / sqrt(diag[j])) : 0.f;
+    s32[(size_t)k * mx_ld(n) + i] = j >= 0 ? (float)(spv[o] * wi / sqrt(diag[j])) : 0.f;
   }
 }
 
@@ -119,7 +127,7 @@ __global__ __launch_bounds__(RB) void k_mx_start(int n, int fac, const float *__
       float acc = ri;
 #pragma unroll
       for (int k = 0; k < nw<WU>(wu); ++k) {
-        const size_t o = (size_t)k * n + i;
+        const size_t o = (size_t)k * mx_ld(n) + i;
         acc += (float)uval[o] * r[colat(ucol, o, i)];
       }
       const float ei = ee[i];
@@ -143,7 +151,7 @@ struct PqRow {
     pi_old = po[i];
 #pragma unroll
     for (int k = 0; k < WS; ++k) {
-      const size_t o = (size_t)k * n + i;
+      const size_t o = (size_t)k * mx_ld(n) + i;
       const int j = colat(col, o, i);
       a[k] = spv[o];
       zj[k] = z[j];
@@ -170,7 +178,7 @@ __device__ __forceinline__ void mx_pq_row(int n, int i, const I *__restrict__ co
   float y = pi;
 #pragma unroll
   for (int k = 0; k < nw<WS>(ws); ++k) {
-    const size_t o = (size_t)k * n + i;
+    const size_t o = (size_t)k * mx_ld(n) + i;
     const int j = colat(col, o, i);
     y += spv[o] * (z[j] + bf * po[j]);
   }
@@ -238,15 +246,16 @@ __global__ __launch_bounds__(RB, 8) void k_mx_pq(int n, int ws, int it, const I 
 }
 
 // Factor, iteration it: decide beta; e += alpha p with the last update's
-// alpha and direction; z = t + L t; p = z + beta p in place.  The first two
-// rows of a thread are loaded before the decision is known.
+// alpha and direction; z = t + L t; p = z + beta p in place.  The first row
+// of a thread is loaded before the decision is known (more rows ahead
+// measured slower).
 template <int WL, class I>
 __device__ __forceinline__ float mx_z_row(int n, int i, int wl, const I *__restrict__ lcol,
                                           const lval_t *__restrict__ lval, const float *__restrict__ t) {
   float acc = t[i];
 #pragma unroll
   for (int k = 0; k < nw<WL>(wl); ++k) {
-    const size_t o = (size_t)k * n + i;
+    const size_t o = (size_t)k * mx_ld(n) + i;
     acc += (float)lval[o] * t[colat(lcol, o, i)];
   }
   return acc;
@@ -263,7 +272,7 @@ struct ZRow {
     ei = ev[i];
 #pragma unroll
     for (int k = 0; k < WL; ++k) {
-      const size_t o = (size_t)k * n + i;
+      const size_t o = (size_t)k * mx_ld(n) + i;
       l[k] = (float)lval[o];
       tj[k] = t[colat(lcol, o, i)];
     }
@@ -283,13 +292,12 @@ __global__ __launch_bounds__(RB, 8) void k_mx_zp(int n, int wl, int it, const I 
                                               const double *__restrict__ partU, int PU, KS *ring, KS *out) {
   int r0, r1;
   block_rows(n, r0, r1);
-  const int i0 = r0 + threadIdx.x, i1 = i0 + RB;
+  const int i0 = r0 + threadIdx.x;
   SelLoads<2> S;
   mx_issue(S, partU, PU);
-  ZRow<WL, I> A0, A1;
+  ZRow<WL, I> A0;
   if (WL > 0) {
     if (i0 < r1) A0.load(n, i0, lcol, lval, t, p, ev);
-    if (i1 < r1) A1.load(n, i1, lcol, lval, t, p, ev);
   }
   __shared__ KS sk[1];
   mx_decide(it, S, ring, out, sk, true);
@@ -301,11 +309,7 @@ __global__ __launch_bounds__(RB, 8) void k_mx_zp(int n, int wl, int it, const I 
       ev[i0] = A0.ei + af * A0.pi;
       p[i0] = A0.z() + bf * A0.pi;
     }
-    if (i1 < r1) {
-      ev[i1] = A1.ei + af * A1.pi;
-      p[i1] = A1.z() + bf * A1.pi;
-    }
-    i = i0 + 2 * RB;
+    i = i0 + RB;
   }
   auto row = [&](int i) {
     const float z = mx_z_row<WL, I>(n, i, wl, lcol, lval, t), pi = p[i];
@@ -319,27 +323,65 @@ __global__ __launch_bounds__(RB, 8) void k_mx_zp(int n, int wl, int it, const I 
   if (i < r1) row(i);
 }
 
-// Factor, iteration it: q = Ah p; partial p.q (nothing once the solve is done)
+// Factor, iteration it: q = Ah p; partial p.q (nothing once the solve is
+// done).  The first two rows of a thread are loaded before the done flag is
+// read, so that its latency hides behind them.
 template <int WS, class I>
-__global__ __launch_bounds__(RB) void k_mx_q(int n, int ws, int it, const I *__restrict__ col,
+struct QRow {
+  float pi, a[WS > 0 ? WS : 1], pj[WS > 0 ? WS : 1];
+  __device__ __forceinline__ void load(int n, int i, const I *__restrict__ col, const float *__restrict__ spv,
+                                       const float *__restrict__ p) {
+    pi = p[i];
+#pragma unroll
+    for (int k = 0; k < WS; ++k) {
+      const size_t o = (size_t)k * mx_ld(n) + i;
+      a[k] = spv[o];
+      pj[k] = p[colat(col, o, i)];
+    }
+  }
+  __device__ __forceinline__ float y() const {
+    float acc = pi;
+#pragma unroll
+    for (int k = 0; k < WS; ++k) acc += a[k] * pj[k];
+    return acc;
+  }
+};
+
+template <int WS, class I>
+__global__ __launch_bounds__(RB, 8) void k_mx_q(int n, int ws, int it, const I *__restrict__ col,
                                              const float *__restrict__ spv, const float *__restrict__ p,
                                              float *__restrict__ q, double *__restrict__ partQ, const KS *ring) {
-  if (ring[(it & 1) * 3].done) return;
   int r0, r1;
   block_rows(n, r0, r1);
+  const int i0 = r0 + threadIdx.x;
+  constexpr int NA = WS == 0 ? 0 : (WS <= 6 ? 2 : 1);
+  QRow<WS, I> A[NA > 0 ? NA : 1];
+#pragma unroll
+  for (int a = 0; a < NA; ++a)
+    if (i0 + a * RB < r1) A[a].load(n, i0 + a * RB, col, spv, p);
+  if (ring[(it & 1) * 3].done) return;
   double pq = 0.0;
+#pragma unroll
+  for (int a = 0; a < NA; ++a) {
+    const int i = i0 + a * RB;
+    if (i < r1) {
+      const float y = A[a].y();
+      q[i] = y;
+      pq += (double)A[a].pi * (double)y;
+    }
+  }
   auto row = [&](int i) {
     const float pi = p[i];
     float y = pi;
 #pragma unroll
     for (int k = 0; k < nw<WS>(ws); ++k) {
-      const size_t o = (size_t)k * n + i;
+      const size_t o = (size_t)k * mx_ld(n) + i;
       y += spv[o] * p[colat(col, o, i)];
     }
     q[i] = y;
     pq += (double)pi * (double)y;
   };
-  int i = r0 + threadIdx.x;
+  int i = i0 + NA * RB;
   for (; i + RB < r1; i += 2 * RB) {
     row(i);
     row(i + RB);
@@ -424,7 +466,7 @@ struct UpdRow {
     ee = eev[i];
 #pragma unroll
     for (int k = 0; k < WU; ++k) {
-      const size_t o = (size_t)k * n + i;
+      const size_t o = (size_t)k * mx_ld(n) + i;
       const int j = colat(ucol, o, i);
       u[k] = (float)uval[o];
       rj[k] = rin[j];
@@ -445,8 +487,8 @@ __global__ __launch_bounds__(RB, 8) void k_mx_upd_fac(int n, int it, const float
   const int i0 = r0 + threadIdx.x, i1 = i0 + RB;
   SelLoads<1> S;
   mx_issue_q(S, partQ, PQ);
-  // rows loaded ahead: two, one for the widest tables (registers)
-  constexpr int NA = WU == 0 ? 0 : (WU <= 4 ? 2 : 1);
+  // one row loaded ahead (two measured slower: register pressure)
+  constexpr int NA = WU == 0 ? 0 : 1;
   UpdRow<WU, I> A0, A1;
   if (NA > 0 && i0 < r1) A0.load(n, i0, rin, q, ucol, uval, ee);
   if (NA > 1 && i1 < r1) A1.load(n, i1, rin, q, ucol, uval, ee);
@@ -473,7 +515,7 @@ __global__ __launch_bounds__(RB, 8) void k_mx_upd_fac(int n, int it, const float
     float acc = ri;
 #pragma unroll
     for (int k = 0; k < nw<WU>(wu); ++k) {
-      const size_t o = (size_t)k * n + i;
+      const size_t o = (size_t)k * mx_ld(n) + i;
       const int j = colat(ucol, o, i);
       acc += (float)uval[o] * (rin[j] - af * q[j]);
     }
@@ -508,7 +550,7 @@ __global__ __launch_bounds__(RB) void k_mx_gz(int n, int wl, const I *__restrict
     float acc = t[i];
 #pragma unroll
     for (int k = 0; k < nw<WL>(wl); ++k) {
-      const size_t o = (size_t)k * n + i;
+      const size_t o = (size_t)k * mx_ld(n) + i;
       acc += (float)lval[o] * t[colat(lcol, o, i)];
     }
     z[i] = acc;
@@ -572,12 +614,12 @@ __global__ __launch_bounds__(BS) void k_fac_rows(int n, int WL, int WU, const in
   float s = 0.f;
   for (int k = 0; k < WU; ++k) {
     const int f = ufi[(size_t)k * n + i];
-    uval[(size_t)k * n + i] = (lval_t)(f >= 0 ? fl[f] : 0.f);
+    uval[(size_t)k * mx_ld(n) + i] = (lval_t)(f >= 0 ? fl[f] : 0.f);
     if (f >= 0) s += fsig[f];
   }
   for (int k = 0; k < WL; ++k) {
     const int f = lfi[(size_t)k * n + i];
-    lval[(size_t)k * n + i] = (lval_t)(f >= 0 ? fl[f] : 0.f);
+    lval[(size_t)k * mx_ld(n) + i] = (lval_t)(f >= 0 ? fl[f] : 0.f);
   }
   ee[i] = expf(2.f * s);
 }
@@ -619,31 +661,34 @@ int mix_build(Ctx &c) {
   for (int i = 0; i < n; ++i) { WL = std::max(WL, nl[i]); WU = std::max(WU, nu[i]); }
   WL = std::max(WL, 1);
   WU = std::max(WU, 1);
-  std::vector<int> lc((size_t)WL * n), uc((size_t)WU * n), lfi((size_t)WL * n, -1), ufi((size_t)WU * n, -1);
-  for (int i = 0; i < n; ++i) {
-    for (int k = 0; k < WL; ++k) lc[(size_t)k * n + i] = i;
-    for (int k = 0; k < WU; ++k) uc[(size_t)k * n + i] = i;
-  }
+  // factor columns at the padded stride (self on padding), face ids at n
+  const size_t ld = mx_ld(n);
+  std::vector<int> lc((size_t)WL * ld), uc((size_t)WU * ld), lfi((size_t)WL * n, -1), ufi((size_t)WU * n, -1);
+  for (size_t o = 0; o < lc.size(); ++o) lc[o] = (int)(o % ld);
+  for (size_t o = 0; o < uc.size(); ++o) uc[o] = (int)(o % ld);
   std::fill(nl.begin(), nl.end(), 0);
   std::fill(nu.begin(), nu.end(), 0);
   for (int q = 0; q < nfi; ++q) {
     const int p = fi_p[q], m = fi_n[q];
-    const size_t ou = (size_t)(nu[p]++) * n + p, ol = (size_t)(nl[m]++) * n + m;
-    uc[ou] = m; ufi[ou] = q;
-    lc[ol] = p; lfi[ol] = q;
+    const int ku = nu[p]++, kl = nl[m]++;
+    uc[(size_t)ku * ld + p] = m; ufi[(size_t)ku * n + p] = q;
+    lc[(size_t)kl * ld + m] = p; lfi[(size_t)kl * n + m] = q;
   }
-  // A's columns, self-padded
-  std::vector<int> ac(spc);
-  for (int i = 0; i < n; ++i)
-    for (int k = 0; k < Ws; ++k)
-      if (ac[(size_t)k * n + i] < 0) ac[(size_t)k * n + i] = i;
+  // A's columns at the padded stride, self on padding
+  std::vector<int> ac((size_t)Ws * ld);
+  for (size_t o = 0; o < ac.size(); ++o) ac[o] = (int)(o % ld);
+  for (int k = 0; k < Ws; ++k)
+    for (int i = 0; i < n; ++i) {
+      const int j = spc[(size_t)k * n + i];
+      if (j >= 0) ac[(size_t)k * ld + i] = j;
+    }
   long span = 0;
-  for (size_t o = 0; o < ac.size(); ++o) span = std::max(span, std::labs((long)ac[o] - (long)(o % n)));
+  for (size_t o = 0; o < ac.size(); ++o) span = std::max(span, std::labs((long)ac[o] - (long)(o % ld)));
   X.i16 = span <= 32767;
   auto up_cols = [&](const std::vector<int> &v, void **dst) -> int {
     if (X.i16) {
       std::vector<short> s(v.size());
-      for (size_t o = 0; o < v.size(); ++o) s[o] = (short)(v[o] - (int)(o % n));
+      for (size_t o = 0; o < v.size(); ++o) s[o] = (short)(v[o] - (int)(o % ld));
       short *d;
       int rc = dalloc(c, &d, s.size());
       if (rc) return rc;
@@ -674,17 +719,17 @@ int mix_build(Ctx &c) {
   if ((rc = up_int(face_fi, &X.face_fi))) return rc;
   if ((rc = up_int(fi_p, &X.fi_p))) return rc;
   if ((rc = up_int(fi_n, &X.fi_n))) return rc;
-  if ((rc = dalloc(c, &X.spv, (size_t)Ws * n))) return rc;
+  if ((rc = dalloc(c, &X.spv, (size_t)Ws * ld))) return rc;
   lval_t *lv, *uv;
-  if ((rc = dalloc(c, &lv, (size_t)WL * n))) return rc;
-  if ((rc = dalloc(c, &uv, (size_t)WU * n))) return rc;
+  if ((rc = dalloc(c, &lv, (size_t)WL * ld))) return rc;
+  if ((rc = dalloc(c, &uv, (size_t)WU * ld))) return rc;
   X.lval = lv;
   X.uval = uv;
   if ((rc = dalloc(c, &X.w, n))) return rc;
   if ((rc = dalloc(c, &X.e, n))) return rc;
   if ((rc = dalloc(c, &X.fl, std::max(nfi, 1)))) return rc;
   if ((rc = dalloc(c, &X.fsig, std::max(nfi, 1)))) return rc;
-  if ((rc = dalloc(c, &X.vec, (size_t)10 * n))) return rc;
+  if ((rc = dalloc(c, &X.vec, (size_t)10 * mx_ld(n)))) return rc;
   if ((rc = dalloc(c, &X.tot, 8))) return rc;
   X.WL = WL;
   X.WU = WU;
@@ -704,7 +749,47 @@ int factor_assemble(Ctx &c) {
 
 // Work vectors (FP32, n each)
 enum { V_E = 0, V_R0, V_R1, V_Z, V_P0, V_P1, V_Q, V_T, V_A, V_B };
-static inline float *fv(Ctx &c, int k) { return c.mx.vec + (size_t)k * c.n; }
+static inline float *fv(Ctx &c, int k) { return c.mx.vec + (size_t)k * mx_ld(c.n); }
+
+// Host polls of the done flag that keep the device busy: the decision after
+// a chunk of iterations is copied to pinned memory behind an event, the next
+// chunk is queued, and only then does the host wait for the copy.  When the
+// observed convergence rate says that the solve ends within the queued work,
+// the host waits for the latest poll before queueing more (a chunk queued
+// after the end costs one launch per kernel that returns at its gate).
+struct Poller {
+  Ctx &c;
+  int j = 0;            // chunk index
+  bool pend = false;    // the poll of chunk j - 1 is posted and not taken
+  int last_it = 0, it_prev = 0, n_obs = 0;
+  double lr_last = 0.0, lr_prev = 0.0;
+  explicit Poller(Ctx &cc) : c(cc) {}
+  hipError_t post() {
+    hipError_t e = hipMemcpyAsync(&c.poll_h[j & 1], c.ks, sizeof(KS), hipMemcpyDeviceToHost, c.stream);
+    return e != hipSuccess ? e : hipEventRecord(c.poll_ev[j & 1], c.stream);
+  }
+  hipError_t take(int jj, KS &k) {
+    const hipError_t e = hipEventSynchronize(c.poll_ev[jj & 1]);
+    k = c.poll_h[jj & 1];
+    return e;
+  }
+  void note(const KS &k) {
+    if (!(k.res_n > 0.0)) return;
+    lr_prev = lr_last;
+    it_prev = last_it;
+    lr_last = std::log(k.res_n);
+    last_it = k.iter;
+    ++n_obs;
+  }
+  // iterations from the last observation to a relative residual of reduc
+  // (large when not known yet)
+  double predict(double reduc) const {
+    if (n_obs < 2 || last_it <= it_prev) return 1e30;
+    const double rate = (lr_last - lr_prev) / (last_it - it_prev);
+    if (!(rate < 0.0)) return 1e30;
+    return (std::log(reduc) - lr_last) / rate;
+  }
+};
 
 // State carried from one inner solve to the next: the buffer holding the
 // last search direction and the r.z it was formed with.
@@ -745,6 +830,7 @@ static int inner_solve(Ctx &c, bool fac, int n_iter, double reduc, InnerCarry &c
   FVK_W(wu, L);
 #undef L
   const int chunk = 32;
+  Poller Pl(c);
   int it = 0;
   for (; it < n_iter; ++it) {
     hipEvent_t ea, eb;
@@ -789,11 +875,36 @@ static int inner_solve(Ctx &c, bool fac, int n_iter, double reduc, InnerCarry &c
 #undef L
     }
     if ((it + 1) % chunk == 0 && it + 1 < n_iter) {
-      KS k;
-      if (T && T->flush()) return FVK_ERR_HIP;
-      FVK_HIP(hipMemcpyAsync(&k, c.ks, sizeof k, hipMemcpyDeviceToHost, c.stream));
-      FVK_HIP(hipStreamSynchronize(c.stream));
-      if (k.done) { ++it; break; }
+      if (T) {
+        KS k;
+        if (T->flush()) return FVK_ERR_HIP;
+        FVK_HIP(hipMemcpyAsync(&k, c.ks, sizeof k, hipMemcpyDeviceToHost, c.stream));
+        FVK_HIP(hipStreamSynchronize(c.stream));
+        if (k.done) { ++it; break; }
+        continue;
+      }
+      bool stop = false;
+      FVK_HIP(Pl.post());
+      if (Pl.pend) {
+        KS k;
+        FVK_HIP(Pl.take(Pl.j - 1, k));
+        if (k.done) stop = true;
+        else Pl.note(k);
+      }
+      if (!stop) {
+        const double left = Pl.predict(reduc) - (it + 1 - Pl.last_it);
+        if (left < chunk) {
+          KS k;
+          FVK_HIP(Pl.take(Pl.j, k));
+          Pl.pend = false;
+          if (k.done) stop = true;
+          else Pl.note(k);
+        } else {
+          Pl.pend = true;
+        }
+      }
+      ++Pl.j;
+      if (stop) { ++it; break; }
     }
   }
   k_mx_final<<<1, RB, 0, c.stream>>>(it, pU, G, c.ksr, c.ks);
