@@ -240,7 +240,7 @@ static int ctx_build(Ctx &c, const Topo &t, int device, int flags) {
   if ((rc = dalloc(c, &c.ngn, nf))) return rc;
   if ((rc = dalloc(c, &c.pc_c, n))) return rc;
   if ((rc = dalloc(c, &c.pc_cr, n))) return rc;
-  if ((rc = dalloc(c, &c.wk, (size_t)8 * 3 * n))) return rc;
+  if ((rc = dalloc(c, &c.wk, (size_t)8 * wk_ld(n)))) return rc;
   if ((rc = dalloc(c, &c.ks, 3))) return rc;
   if ((rc = dalloc(c, &c.ksr, 6))) return rc;
   if ((rc = dalloc(c, &c.part, (size_t)4 * RG * 8))) return rc;
@@ -536,7 +536,7 @@ int fvk_solv_direct(fvk_ctx *cc, int imat, const double *cv, const double *c_r, 
   } else if (!c.pc_valid || c.pc_imat != imat) {
     GUARD(precond_build(c, imat));
   }
-  double *db = c.wk + (size_t)5 * 3 * c.n, *dx = c.wk + (size_t)6 * 3 * c.n;
+  double *db = wk_slot(c, 5), *dx = wk_slot(c, 6);
   GUARD(to_dev(c, b, 1, 1, c.n, &c.perm, db));
   GUARD(precond_apply(c, imat, 1, db, dx, nullptr));
   GUARD(to_host(c, dx, 1, 1, c.n, &c.perm, x));
@@ -564,7 +564,7 @@ static int col_to_host(Ctx &c, const double *d, int nvc, double *res0) {
 int fvk_solv_res(fvk_ctx *cc, int n_vc, const double *phi, double *res0, double *res0_n, int not_r0) {
   if (!cc || !res0 || !res0_n || n_vc < 1 || n_vc > 3 || (!not_r0 && !phi)) { set_error("bad argument"); return FVK_ERR_ARG; }
   Ctx &c = cc->c;
-  double *dx = c.wk + (size_t)5 * 3 * c.n, *dr = c.wk + (size_t)6 * 3 * c.n;
+  double *dx = wk_slot(c, 5), *dr = wk_slot(c, 6);
   if (!not_r0) GUARD(to_dev(c, phi, n_vc, n_vc, c.n, &c.perm, dx));
   GUARD(col_to_dev(c, res0, n_vc, dr));
   GUARD(residual(c, n_vc, dx, dr, not_r0 != 0));
@@ -582,7 +582,7 @@ static int solve_common(fvk_ctx *cc, int kind, int imat, double reduc, int n_ite
     return FVK_ERR_ARG;
   }
   Ctx &c = cc->c;
-  double *dx = c.wk + (size_t)5 * 3 * c.n, *dr = c.wk + (size_t)6 * 3 * c.n;
+  double *dx = wk_slot(c, 5), *dr = wk_slot(c, 6);
   GUARD(to_dev(c, phi, n_vc, n_vc, c.n, &c.perm, dx));
   GUARD(col_to_dev(c, res0, n_vc, dr));
   FVK_HIP(hipEventRecord(c.ev0, c.stream));
@@ -617,7 +617,8 @@ int fvk_solv_grad(fvk_ctx *cc, int n_vc, const double *phi, double *gout, int n_
   Ctx &c = cc->c;
   const int n = c.n, nb = c.nbf, nd = c.ndx;
   if (!c.bnd_buf) GUARD(dalloc(c, &c.bnd_buf, (size_t)3 * std::max(nb, 1)));
-  double *dphi = c.wk, *dphib = c.bnd_buf, *g = c.wk + (size_t)2 * 3 * n, *g0 = c.wk + (size_t)5 * 3 * n;
+  // g and g0 hold 3 n_vc x n doubles: three slots each
+  double *dphi = wk_slot(c, 0), *dphib = c.bnd_buf, *g = wk_slot(c, 2), *g0 = wk_slot(c, 5);
   GUARD(to_dev(c, phi, n_vc, n_vc, n, &c.perm, dphi));
   if (nb) GUARD(to_dev(c, phi + (size_t)n * n_vc, n_vc, n_vc, nb, nullptr, dphib));
   GUARD(grad(c, n_vc, dphi, dphib, g, n_iter, neu, nullptr, g0));
@@ -833,7 +834,7 @@ int fvk_factor_apply(fvk_ctx *cc, const double *r, double *z) {
 int fvk_bench_spmv(fvk_ctx *cc, int reps, double *sec) {
   if (!cc || !sec || reps < 1) { set_error("bad argument"); return FVK_ERR_ARG; }
   Ctx &c = cc->c;
-  double *x = c.wk, *y = c.wk + (size_t)3 * c.n;
+  double *x = wk_slot(c, 0), *y = wk_slot(c, 1);
   for (int k = 0; k < 3; ++k) GUARD(spmv(c, 1, x, y));
   FVK_HIP(hipEventRecord(c.ev0, c.stream));
   for (int k = 0; k < reps; ++k) GUARD(spmv(c, 1, x, y));

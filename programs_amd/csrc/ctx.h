@@ -3,6 +3,7 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -99,6 +100,79 @@ struct Ctx {
   KS *poll_h = nullptr;
   hipEvent_t poll_ev[2] = {nullptr, nullptr};
 };
+
+// Host polls of the done flag that keep the device busy: the decision after
+// a chunk of iterations is copied to pinned memory behind an event, the next
+// chunk is queued, and only then does the host wait for the copy.  When the
+// observed convergence rate says that the solve ends within the queued work,
+// the host waits for the latest poll before queueing more (a chunk queued
+// after the end costs one launch per kernel that returns at its gate).
+struct Poller {
+  Ctx &c;
+  int j = 0;            // chunk index
+  bool pend = false;    // the poll of chunk j - 1 is posted and not taken
+  int last_it = 0, it_prev = 0, n_obs = 0;
+  double lr_last = 0.0, lr_prev = 0.0;
+  explicit Poller(Ctx &cc) : c(cc) {}
+  hipError_t post() {
+    hipError_t e = hipMemcpyAsync(&c.poll_h[j & 1], c.ks, sizeof(KS), hipMemcpyDeviceToHost, c.stream);
+    return e != hipSuccess ? e : hipEventRecord(c.poll_ev[j & 1], c.stream);
+  }
+  hipError_t take(int jj, KS &k) {
+    const hipError_t e = hipEventSynchronize(c.poll_ev[jj & 1]);
+    k = c.poll_h[jj & 1];
+    return e;
+  }
+  void note(const KS &k) {
+    if (!(k.res_n > 0.0)) return;
+    lr_prev = lr_last;
+    it_prev = last_it;
+    lr_last = std::log(k.res_n);
+    last_it = k.iter;
+    ++n_obs;
+  }
+  // iterations from the last observation to a relative residual of reduc
+  // (large when not known yet)
+  double predict(double reduc) const {
+    if (n_obs < 2 || last_it <= it_prev) return 1e30;
+    const double rate = (lr_last - lr_prev) / (last_it - it_prev);
+    if (!(rate < 0.0)) return 1e30;
+    return (std::log(reduc) - lr_last) / rate;
+  }
+  // After a chunk that brought the queued iterations to `queued`: post its
+  // poll, take the previous one, and take this one too when the solve is
+  // predicted to end within the queued work.  stop: a taken poll says done.
+  hipError_t step(int queued, int chunk, double reduc, bool &stop) {
+    stop = false;
+    hipError_t e = post();
+    if (e != hipSuccess) return e;
+    KS k;
+    if (pend) {
+      if ((e = take(j - 1, k)) != hipSuccess) return e;
+      if (k.done) stop = true;
+      else note(k);
+    }
+    if (!stop) {
+      if (predict(reduc) - (queued - last_it) < chunk) {
+        if ((e = take(j, k)) != hipSuccess) return e;
+        pend = false;
+        if (k.done) stop = true;
+        else note(k);
+      } else {
+        pend = true;
+      }
+    }
+    ++j;
+    return hipSuccess;
+  }
+};
+
+// Slots of the FP64 work buffer c.wk (8 slots of up to 3n doubles): their
+// stride is 3n plus 64 KiB + 256 B, so that the same row of different vectors
+// does not sit at the same offset of its memory page (vectors a power of two
+// apart collide in the same HBM channels).
+inline size_t wk_ld(int n) { return (size_t)3 * n + 8224; }
+inline double *wk_slot(Ctx &c, int k) { return c.wk + (size_t)k * wk_ld(c.n); }
 
 template <class T>
 int dalloc(Ctx &c, T **p, size_t n) {

@@ -18,7 +18,6 @@
 
 namespace fvk {
 
-static inline double *wvf(Ctx &c, int k) { return c.wk + (size_t)k * 3 * c.n; }
 
 // ---------------------------------------------------------------- gradient
 // One Green-Gauss sweep.  phi: nvc x n (cells), phib: nvc x nbf (boundary

@@ -621,7 +621,7 @@ int spmv(Ctx &c, int nvc, const double *x, double *y) {
   return 0;
 }
 
-static inline double *wv(Ctx &c, int k) { return c.wk + (size_t)k * 3 * c.n; }
+static inline double *wv(Ctx &c, int k) { return wk_slot(c, k); }
 static inline double *pbuf(Ctx &c, int k) { return c.part + (size_t)k * RG * 8; }
 
 static int ks_init(Ctx &c, int nvc, double reduc, int n_iter, bool bicg) {
@@ -676,7 +676,8 @@ static void launch_cg_upd(Ctx &c, int G, int it, double *x, double *r, double *p
 
 // PCG: r0 holds b on entry, b - A x0 (the initial residual) on exit; x is
 // updated.  chunk > 0: stop queueing iterations once converged.
-static int cg_run(Ctx &c, int imat, int n_iter, int nvc, double *x, double *r0, bool not_r0, int chunk, KTimer *T) {
+static int cg_run(Ctx &c, int imat, double reduc, int n_iter, int nvc, double *x, double *r0, bool not_r0, int chunk,
+                  KTimer *T) {
   const int n = c.n, G = red_grid(n);
   double *r = wv(c, 0), *z = wv(c, 1), *P = wv(c, 2), *q = wv(c, 4);
   const int jac = imat == 0;
@@ -684,7 +685,8 @@ static int cg_run(Ctx &c, int imat, int n_iter, int nvc, double *x, double *r0, 
     int rc = precond_build(c, imat);
     if (rc) return rc;
   }
-  if (fill(c, P, (size_t)2 * 3 * n, 0.0)) return FVK_ERR_HIP;
+  // both direction buffers (slots 2 and 3)
+  if (fill(c, P, wk_ld(n) + (size_t)3 * n, 0.0)) return FVK_ERR_HIP;
 #define L(W)                                                                                                \
   k_residual<W><<<G, RB, 0, c.stream>>>(c.M, nvc, c.diag, c.spv, x, r0, not_r0 ? 1 : 0, jac ? 1 : 0, c.pc_c, z, \
                                         pbuf(c, 0))
@@ -698,9 +700,10 @@ static int cg_run(Ctx &c, int imat, int n_iter, int nvc, double *x, double *r0, 
     k_dot<<<G, RB, 0, c.stream>>>(n, nvc, r, z, pbuf(c, 2), nullptr);
     pR = pbuf(c, 2);
   }
+  Poller Pl(c);
   int it = 0;
   for (; it < n_iter; ++it) {
-    double *pold = P + (size_t)(it & 1) * 3 * n, *pnew = P + (size_t)((it + 1) & 1) * 3 * n;
+    double *pold = P + (size_t)(it & 1) * wk_ld(n), *pnew = P + (size_t)((it + 1) & 1) * wk_ld(n);
     if (nvc == 1) launch_cg_pq<1>(c, G, it, z, pold, pnew, q, pbuf(c, 0), pR, T);
     else if (nvc == 2) launch_cg_pq<2>(c, G, it, z, pold, pnew, q, pbuf(c, 0), pR, T);
     else launch_cg_pq<3>(c, G, it, z, pold, pnew, q, pbuf(c, 0), pR, T);
@@ -714,8 +717,13 @@ static int cg_run(Ctx &c, int imat, int n_iter, int nvc, double *x, double *r0, 
     }
     if (chunk > 0 && (it + 1) % chunk == 0 && it + 1 < n_iter) {
       bool done;
-      if (T && T->flush()) return FVK_ERR_HIP;
-      if (all_done_host(c, nvc, done)) return FVK_ERR_HIP;
+      if (nvc == 1 && !T) {
+        // the next chunk is queued before the host waits for this one
+        FVK_HIP(Pl.step(it + 1, chunk, reduc, done));
+      } else {
+        if (T && T->flush()) return FVK_ERR_HIP;
+        if (all_done_host(c, nvc, done)) return FVK_ERR_HIP;
+      }
       if (done) { ++it; break; }
     }
   }
@@ -733,7 +741,8 @@ static int bicg_run(Ctx &c, int imat, int n_iter, int nvc, double *x, double *r0
     int rc = precond_build(c, imat);
     if (rc) return rc;
   }
-  if (fill(c, p, (size_t)4 * 3 * n, 0.0)) return FVK_ERR_HIP;
+  // slots 1-4
+  if (fill(c, p, (size_t)3 * wk_ld(n) + (size_t)3 * n, 0.0)) return FVK_ERR_HIP;
 #define L(W)                                                                                              \
   k_residual<W><<<G, RB, 0, c.stream>>>(c.M, nvc, c.diag, c.spv, x, r0, not_r0 ? 1 : 0, 2, nullptr, nullptr, \
                                         pbuf(c, 0))
@@ -764,7 +773,7 @@ int krylov(Ctx &c, int kind, int imat, double reduc, int n_iter, int nvc, double
            int chunk, KTimer *T) {
   int rc = ks_init(c, nvc, reduc, n_iter, kind == 1);
   if (rc) return rc;
-  if (kind == 0) return cg_run(c, imat, n_iter, nvc, x, r0, not_r0, chunk, T);
+  if (kind == 0) return cg_run(c, imat, reduc, n_iter, nvc, x, r0, not_r0, chunk, T);
   return bicg_run(c, imat, n_iter, nvc, x, r0, not_r0);
 }
 
@@ -781,7 +790,7 @@ int pcg_tol(Ctx &c, int imat, const double *b, double *x, double tol, int max_it
   double tr[4];
   if ((rc = true_residual(c, x, b, c.stage, tr))) return rc;
   const double l1b = tr[2], sigma = l1b > 0 ? l1b / n : 1.0;
-  double *bs = c.wk + (size_t)7 * 3 * n;
+  double *bs = wk_slot(c, 7);
   const int g = (int)std::min<size_t>(((size_t)n + BS - 1) / BS, 4096);
   k_scale64<<<g, BS, 0, c.stream>>>(n, bs, b, 1.0 / sigma);
   if ((rc = krylov(c, 0, imat, tol, max_iter, 1, x, bs, true, chunk, T))) return rc;

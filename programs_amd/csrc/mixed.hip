@@ -751,46 +751,6 @@ int factor_assemble(Ctx &c) {
 enum { V_E = 0, V_R0, V_R1, V_Z, V_P0, V_P1, V_Q, V_T, V_A, V_B };
 static inline float *fv(Ctx &c, int k) { return c.mx.vec + (size_t)k * mx_ld(c.n); }
 
-// Host polls of the done flag that keep the device busy: the decision after
-// a chunk of iterations is copied to pinned memory behind an event, the next
-// chunk is queued, and only then does the host wait for the copy.  When the
-// observed convergence rate says that the solve ends within the queued work,
-// the host waits for the latest poll before queueing more (a chunk queued
-// after the end costs one launch per kernel that returns at its gate).
-struct Poller {
-  Ctx &c;
-  int j = 0;            // chunk index
-  bool pend = false;    // the poll of chunk j - 1 is posted and not taken
-  int last_it = 0, it_prev = 0, n_obs = 0;
-  double lr_last = 0.0, lr_prev = 0.0;
-  explicit Poller(Ctx &cc) : c(cc) {}
-  hipError_t post() {
-    hipError_t e = hipMemcpyAsync(&c.poll_h[j & 1], c.ks, sizeof(KS), hipMemcpyDeviceToHost, c.stream);
-    return e != hipSuccess ? e : hipEventRecord(c.poll_ev[j & 1], c.stream);
-  }
-  hipError_t take(int jj, KS &k) {
-    const hipError_t e = hipEventSynchronize(c.poll_ev[jj & 1]);
-    k = c.poll_h[jj & 1];
-    return e;
-  }
-  void note(const KS &k) {
-    if (!(k.res_n > 0.0)) return;
-    lr_prev = lr_last;
-    it_prev = last_it;
-    lr_last = std::log(k.res_n);
-    last_it = k.iter;
-    ++n_obs;
-  }
-  // iterations from the last observation to a relative residual of reduc
-  // (large when not known yet)
-  double predict(double reduc) const {
-    if (n_obs < 2 || last_it <= it_prev) return 1e30;
-    const double rate = (lr_last - lr_prev) / (last_it - it_prev);
-    if (!(rate < 0.0)) return 1e30;
-    return (std::log(reduc) - lr_last) / rate;
-  }
-};
-
 // State carried from one inner solve to the next: the buffer holding the
 // last search direction and the r.z it was formed with.
 struct InnerCarry {
@@ -884,26 +844,7 @@ static int inner_solve(Ctx &c, bool fac, int n_iter, double reduc, InnerCarry &c
         continue;
       }
       bool stop = false;
-      FVK_HIP(Pl.post());
-      if (Pl.pend) {
-        KS k;
-        FVK_HIP(Pl.take(Pl.j - 1, k));
-        if (k.done) stop = true;
-        else Pl.note(k);
-      }
-      if (!stop) {
-        const double left = Pl.predict(reduc) - (it + 1 - Pl.last_it);
-        if (left < chunk) {
-          KS k;
-          FVK_HIP(Pl.take(Pl.j, k));
-          Pl.pend = false;
-          if (k.done) stop = true;
-          else Pl.note(k);
-        } else {
-          Pl.pend = true;
-        }
-      }
-      ++Pl.j;
+      FVK_HIP(Pl.step(it + 1, chunk, reduc, stop));
       if (stop) { ++it; break; }
     }
   }
