@@ -59,12 +59,14 @@ struct Mix {
   int *fi_p = nullptr, *fi_n = nullptr;  // rows of an interior face's owner / neighbour
   float *spv = nullptr;                // Jacobi-scaled off-diagonals a_ij / sqrt(a_ii a_jj), FP32
   void *lval = nullptr, *uval = nullptr;  // L per slot, FP16
-  float *w = nullptr, *e = nullptr;    // a_ii^-1/2 and Delta^2 per row
+  float *w = nullptr;                  // a_ii^-1/2 per row
+  void *e = nullptr;                   // Delta^2 per row, FP16
   float *fl = nullptr, *fsig = nullptr;  // per interior face l_f, sig_f
   float *vec = nullptr;                // 10 x n FP32 work vectors
   double *tot = nullptr;               // 8 doubles: residual totals
   bool fac_valid = false;
   double inner_reduc = 1e-5;           // deepest relative L1 reduction of an inner solve
+  double eta_margin = 0.5;             // a later inner solve aims at eta_margin x what tol needs
   int max_outer = 30;
   bool carry = false;                  // continue the Krylov recurrence across correction steps
 };
@@ -109,10 +111,13 @@ struct Ctx {
 // after the end costs one launch per kernel that returns at its gate).
 struct Poller {
   Ctx &c;
-  int j = 0;            // chunk index
-  bool pend = false;    // the poll of chunk j - 1 is posted and not taken
-  int last_it = 0, it_prev = 0, n_obs = 0;
-  double lr_last = 0.0, lr_prev = 0.0;
+  int j = 0;            // poll index
+  bool pend = false;    // poll j - 1 is posted and not taken
+  // the last observations of log(relative residual) and their iterations
+  static constexpr int H = 8;
+  int n_obs = 0, hit[H] = {};
+  double hlr[H] = {};
+  int last_it = 0;
   explicit Poller(Ctx &cc) : c(cc) {}
   hipError_t post() {
     hipError_t e = hipMemcpyAsync(&c.poll_h[j & 1], c.ks, sizeof(KS), hipMemcpyDeviceToHost, c.stream);
@@ -125,25 +130,37 @@ struct Poller {
   }
   void note(const KS &k) {
     if (!(k.res_n > 0.0)) return;
-    lr_prev = lr_last;
-    it_prev = last_it;
-    lr_last = std::log(k.res_n);
+    hit[n_obs % H] = k.iter;
+    hlr[n_obs % H] = std::log(k.res_n);
     last_it = k.iter;
     ++n_obs;
   }
-  // iterations from the last observation to a relative residual of reduc
-  // (large when not known yet)
-  double predict(double reduc) const {
-    if (n_obs < 2 || last_it <= it_prev) return 1e30;
-    const double rate = (lr_last - lr_prev) / (last_it - it_prev);
+  // iterations from the latest observation to a relative residual of reduc,
+  // at the rate since the newest observation at least `span` iterations
+  // older (else the oldest kept); large when not known yet
+  double predict(double reduc, int span) const {
+    if (n_obs < 2) return 1e30;
+    const int l = (n_obs - 1) % H;
+    int a = -1;
+    for (int q = 1; q < std::min(n_obs, H); ++q) {
+      const int m = (n_obs - 1 - q) % H;
+      a = m;
+      if (hit[l] - hit[m] >= span) break;
+    }
+    if (a < 0 || hit[l] <= hit[a]) return 1e30;
+    const double rate = (hlr[l] - hlr[a]) / (hit[l] - hit[a]);
     if (!(rate < 0.0)) return 1e30;
-    return (std::log(reduc) - lr_last) / rate;
+    return (std::log(reduc) - hlr[l]) / rate;
   }
-  // After a chunk that brought the queued iterations to `queued`: post its
-  // poll, take the previous one, and take this one too when the solve is
-  // predicted to end within the queued work.  stop: a taken poll says done.
-  hipError_t step(int queued, int chunk, double reduc, bool &stop) {
+  // After the iterations queued so far (`queued`): post this poll, take the
+  // previous one, and take this one too when the solve is predicted to end
+  // within the queued work.  stop: a taken poll says done.  next: the
+  // iterations to queue before the next poll, a chunk while the end is far,
+  // else the predicted remainder and two more (so that few iterations are
+  // queued after the end, each of which still costs its launches).
+  hipError_t step(int queued, int chunk, double reduc, bool &stop, int &next) {
     stop = false;
+    next = chunk;
     hipError_t e = post();
     if (e != hipSuccess) return e;
     KS k;
@@ -153,11 +170,16 @@ struct Poller {
       else note(k);
     }
     if (!stop) {
-      if (predict(reduc) - (queued - last_it) < chunk) {
+      if (predict(reduc, chunk) - (queued - last_it) < chunk) {
         if ((e = take(j, k)) != hipSuccess) return e;
         pend = false;
-        if (k.done) stop = true;
-        else note(k);
+        if (k.done) {
+          stop = true;
+        } else {
+          note(k);
+          const double left = predict(reduc, chunk) - (queued - last_it);
+          next = left < chunk ? std::max(2, (int)std::ceil(left) + 2) : chunk;
+        }
       } else {
         pend = true;
       }

@@ -701,7 +701,7 @@ static int cg_run(Ctx &c, int imat, double reduc, int n_iter, int nvc, double *x
     pR = pbuf(c, 2);
   }
   Poller Pl(c);
-  int it = 0;
+  int it = 0, next_poll = chunk;
   for (; it < n_iter; ++it) {
     double *pold = P + (size_t)(it & 1) * wk_ld(n), *pnew = P + (size_t)((it + 1) & 1) * wk_ld(n);
     if (nvc == 1) launch_cg_pq<1>(c, G, it, z, pold, pnew, q, pbuf(c, 0), pR, T);
@@ -715,16 +715,18 @@ static int cg_run(Ctx &c, int imat, double reduc, int n_iter, int nvc, double *x
       if (rc) return rc;
       k_dot<<<G, RB, 0, c.stream>>>(n, nvc, r, z, pbuf(c, 2), c.ksr + (it & 1) * 3);
     }
-    if (chunk > 0 && (it + 1) % chunk == 0 && it + 1 < n_iter) {
+    if (chunk > 0 && it + 1 == next_poll && it + 1 < n_iter) {
       bool done;
+      int next = chunk;
       if (nvc == 1 && !T) {
         // the next chunk is queued before the host waits for this one
-        FVK_HIP(Pl.step(it + 1, chunk, reduc, done));
+        FVK_HIP(Pl.step(it + 1, chunk, reduc, done, next));
       } else {
         if (T && T->flush()) return FVK_ERR_HIP;
         if (all_done_host(c, nvc, done)) return FVK_ERR_HIP;
       }
       if (done) { ++it; break; }
+      next_poll = it + 1 + next;
     }
   }
   k_cg_final<<<1, RB, 0, c.stream>>>(it, nvc, pbuf(c, 0), G, pR, G, c.ksr, c.ks);

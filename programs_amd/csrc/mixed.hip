@@ -11,7 +11,7 @@
 //   x = 0
 //   repeat: r = b - A x (FP64); stop if |r|_1 / |b|_1 < tol
 //           solve Ah eh = D^-1/2 r in FP32 to relative L1 reduction
-//             eta = max(inner_reduc, 0.3 tol / (|r|_1 / |b|_1))
+//             eta = max(inner_reduc, eta_margin tol / (|r|_1 / |b|_1))
 //           x += D^-1/2 eh (FP64)
 //
 // With the context setting dc_carry, a correction step after the first
@@ -49,6 +49,11 @@
 namespace fvk {
 
 typedef _Float16 lval_t;
+// t = Delta^2 (I + L^T) r as the update kernel hands it to the direction
+// kernel: FP16, divided by a scale both kernels derive from the same
+// committed scalars (mean |r| of the decision the update follows), so that
+// it sits near 1 whatever the residual's size.
+typedef _Float16 t16_t;
 
 // Column of slot k of row i: 32-bit rows, or 16-bit offsets from the row.
 __device__ __forceinline__ int colat(const int *c, size_t k, int i) { return c[k]; }
@@ -66,6 +71,13 @@ __device__ __forceinline__ bool gate_done(const KS *g) { return g && g->done; }
 // two apart the streams of one kernel collided in the same HBM channels
 // (5 % slower on the 2^21-row system for the vectors alone).
 __host__ __device__ __forceinline__ size_t mx_ld(int n) { return (size_t)n + 16448; }
+
+// Scale of the stored t after decision `it` (its scalars k): the mean |r|
+// (1 when it is not usable)
+__device__ __forceinline__ float t_scale(const KS &k, int it, int n) {
+  const double s = (it == 0 ? k.res0_n : k.res_n * (k.res0_n + SMALL)) / n;
+  return s > 0.0 && s < 1e30 ? (float)s : 1.f;
+}
 
 // Jacobi-scaled FP32 copy of the matrix, w = a_ii^-1/2
 __global__ __launch_bounds__(BS) void k_mx_conv(DMesh M, const double *__restrict__ diag,
@@ -107,12 +119,12 @@ __global__ __launch_bounds__(RB) void k_mx_res(DMesh M, const double *__restrict
 // Start of an inner solve: e = 0, p_old = f p_last (p_last null: 0),
 // partials |r|_1 [0] and Jacobi: r.r [3]; factor: t = Delta^2 (r + L^T r).
 // p_last may be p0 itself (each row reads and writes its own element).
-template <int WU, class I>
+template <int WU, class I, class TT>
 __global__ __launch_bounds__(RB) void k_mx_start(int n, int fac, const float *__restrict__ r,
                                                  float *__restrict__ ev, float *p0, const float *p_last, float f,
                                                  int wu, const I *__restrict__ ucol,
-                                                 const lval_t *__restrict__ uval, const float *__restrict__ ee,
-                                                 float *__restrict__ t, double *__restrict__ part) {
+                                                 const lval_t *__restrict__ uval, const lval_t *__restrict__ ee,
+                                                 TT *__restrict__ t, double *__restrict__ part) {
   double a[4] = {0, 0, 0, 0};
   int r0, r1;
   block_rows(n, r0, r1);
@@ -130,8 +142,8 @@ __global__ __launch_bounds__(RB) void k_mx_start(int n, int fac, const float *__
         const size_t o = (size_t)k * mx_ld(n) + i;
         acc += (float)uval[o] * r[colat(ucol, o, i)];
       }
-      const float ei = ee[i];
-      t[i] = ei * acc;
+      const float ei = (float)ee[i];
+      t[i] = (TT)(ei * acc);
       a[3] += (double)ei * (double)acc * (double)acc;
     }
   }
@@ -251,12 +263,12 @@ __global__ __launch_bounds__(RB, 8) void k_mx_pq(int n, int ws, int it, const I 
 // measured slower).
 template <int WL, class I>
 __device__ __forceinline__ float mx_z_row(int n, int i, int wl, const I *__restrict__ lcol,
-                                          const lval_t *__restrict__ lval, const float *__restrict__ t) {
-  float acc = t[i];
+                                          const lval_t *__restrict__ lval, const t16_t *__restrict__ t) {
+  float acc = (float)t[i];
 #pragma unroll
   for (int k = 0; k < nw<WL>(wl); ++k) {
     const size_t o = (size_t)k * mx_ld(n) + i;
-    acc += (float)lval[o] * t[colat(lcol, o, i)];
+    acc += (float)lval[o] * (float)t[colat(lcol, o, i)];
   }
   return acc;
 }
@@ -265,16 +277,16 @@ template <int WL, class I>
 struct ZRow {
   float ti, pi, ei, l[WL > 0 ? WL : 1], tj[WL > 0 ? WL : 1];
   __device__ __forceinline__ void load(int n, int i, const I *__restrict__ lcol, const lval_t *__restrict__ lval,
-                                       const float *__restrict__ t, const float *__restrict__ p,
+                                       const t16_t *__restrict__ t, const float *__restrict__ p,
                                        const float *__restrict__ ev) {
-    ti = t[i];
+    ti = (float)t[i];
     pi = p[i];
     ei = ev[i];
 #pragma unroll
     for (int k = 0; k < WL; ++k) {
       const size_t o = (size_t)k * mx_ld(n) + i;
       l[k] = (float)lval[o];
-      tj[k] = t[colat(lcol, o, i)];
+      tj[k] = (float)t[colat(lcol, o, i)];
     }
   }
   __device__ __forceinline__ float z() const {
@@ -287,7 +299,7 @@ struct ZRow {
 
 template <int WL, class I>
 __global__ __launch_bounds__(RB, 8) void k_mx_zp(int n, int wl, int it, const I *__restrict__ lcol,
-                                              const lval_t *__restrict__ lval, const float *__restrict__ t,
+                                              const lval_t *__restrict__ lval, const t16_t *__restrict__ t,
                                               float *__restrict__ p, float *__restrict__ ev,
                                               const double *__restrict__ partU, int PU, KS *ring, KS *out) {
   int r0, r1;
@@ -303,16 +315,18 @@ __global__ __launch_bounds__(RB, 8) void k_mx_zp(int n, int wl, int it, const I 
   mx_decide(it, S, ring, out, sk, true);
   if (sk[0].done) return;
   const float bf = (float)sk[0].beta, af = (float)sk[0].alpha;
+  // t was stored divided by the scale of the previous decision
+  const float ts = it == 0 ? 1.f : t_scale(ring[((it + 1) & 1) * 3], it - 1, n);
   int i = i0;
   if (WL > 0) {
     if (i0 < r1) {
       ev[i0] = A0.ei + af * A0.pi;
-      p[i0] = A0.z() + bf * A0.pi;
+      p[i0] = ts * A0.z() + bf * A0.pi;
     }
     i = i0 + RB;
   }
   auto row = [&](int i) {
-    const float z = mx_z_row<WL, I>(n, i, wl, lcol, lval, t), pi = p[i];
+    const float z = ts * mx_z_row<WL, I>(n, i, wl, lcol, lval, t), pi = p[i];
     ev[i] += af * pi;
     p[i] = z + bf * pi;
   };
@@ -460,10 +474,10 @@ struct UpdRow {
   float ri, qi, ee, u[WU > 0 ? WU : 1], rj[WU > 0 ? WU : 1], qj[WU > 0 ? WU : 1];
   __device__ __forceinline__ void load(int n, int i, const float *__restrict__ rin, const float *__restrict__ q,
                                        const I *__restrict__ ucol, const lval_t *__restrict__ uval,
-                                       const float *__restrict__ eev) {
+                                       const lval_t *__restrict__ eev) {
     ri = rin[i];
     qi = q[i];
-    ee = eev[i];
+    ee = (float)eev[i];
 #pragma unroll
     for (int k = 0; k < WU; ++k) {
       const size_t o = (size_t)k * mx_ld(n) + i;
@@ -479,7 +493,7 @@ template <int WU, class I>
 __global__ __launch_bounds__(RB, 8) void k_mx_upd_fac(int n, int it, const float *__restrict__ rin,
                                                       float *__restrict__ rout, const float *__restrict__ q, int wu,
                                                    const I *__restrict__ ucol, const lval_t *__restrict__ uval,
-                                                   const float *__restrict__ ee, float *__restrict__ t,
+                                                   const lval_t *__restrict__ ee, t16_t *__restrict__ t,
                                                    const double *__restrict__ partQ, int PQ,
                                                    double *__restrict__ partU, KS *ring) {
   int r0, r1;
@@ -496,7 +510,7 @@ __global__ __launch_bounds__(RB, 8) void k_mx_upd_fac(int n, int it, const float
   if (cur->done) return;
   double alpha;
   if (!mx_alpha(cur, S, alpha)) return;
-  const float af = (float)alpha;
+  const float af = (float)alpha, tsi = 1.f / t_scale(*cur, it, n);
   double a[4] = {0, 0, 0, 0};
   auto fin = [&](int i, const UpdRow<WU, I> &R) {
     const float ri = R.ri - af * R.qi;
@@ -505,7 +519,7 @@ __global__ __launch_bounds__(RB, 8) void k_mx_upd_fac(int n, int it, const float
     float acc = ri;
 #pragma unroll
     for (int k = 0; k < WU; ++k) acc += R.u[k] * (R.rj[k] - af * R.qj[k]);
-    t[i] = R.ee * acc;
+    t[i] = (t16_t)(R.ee * acc * tsi);
     a[3] += (double)R.ee * (double)acc * (double)acc;
   };
   auto row = [&](int i) {
@@ -519,8 +533,8 @@ __global__ __launch_bounds__(RB, 8) void k_mx_upd_fac(int n, int it, const float
       const int j = colat(ucol, o, i);
       acc += (float)uval[o] * (rin[j] - af * q[j]);
     }
-    const float ei = ee[i];
-    t[i] = ei * acc;
+    const float ei = (float)ee[i];
+    t[i] = (t16_t)(ei * acc * tsi);
     a[3] += (double)ei * (double)acc * (double)acc;
   };
   int i = i0;
@@ -603,25 +617,29 @@ __global__ __launch_bounds__(BS) void k_mx_add(int n, double *__restrict__ x, co
   x[i] += sigma * (double)e / sqrt(diag[i]);
 }
 
+constexpr double FAC_E2_MIN = -9.7, FAC_E2_MAX = 11.0;
+
 // Factor from per-interior-face (l, sig): e = exp(2 s) with s the sum of sig
 // over the faces the row owns, slot values of L (FP16).
 __global__ __launch_bounds__(BS) void k_fac_rows(int n, int WL, int WU, const int *__restrict__ lfi,
                                                  const int *__restrict__ ufi, const float *__restrict__ fl,
                                                  const float *__restrict__ fsig, lval_t *__restrict__ lval,
-                                                 lval_t *__restrict__ uval, float *__restrict__ ee) {
+                                                 lval_t *__restrict__ uval, lval_t *__restrict__ ee) {
   const int i = blockIdx.x * BS + threadIdx.x;
   if (i >= n) return;
-  float s = 0.f;
+  double s = 0.0;
   for (int k = 0; k < WU; ++k) {
     const int f = ufi[(size_t)k * n + i];
     uval[(size_t)k * mx_ld(n) + i] = (lval_t)(f >= 0 ? fl[f] : 0.f);
-    if (f >= 0) s += fsig[f];
+    if (f >= 0) s += (double)fsig[f];
   }
   for (int k = 0; k < WL; ++k) {
     const int f = lfi[(size_t)k * n + i];
     lval[(size_t)k * mx_ld(n) + i] = (lval_t)(f >= 0 ? fl[f] : 0.f);
   }
-  ee[i] = expf(2.f * s);
+  // Delta^2 rounded once to FP16 from FP64, the exponent held within FP16's
+  // normal range
+  ee[i] = (lval_t)exp(fmin(fmax(2.0 * s, FAC_E2_MIN), FAC_E2_MAX));
 }
 
 // ------------------------------------------------------------ host side
@@ -726,7 +744,9 @@ int mix_build(Ctx &c) {
   X.lval = lv;
   X.uval = uv;
   if ((rc = dalloc(c, &X.w, n))) return rc;
-  if ((rc = dalloc(c, &X.e, n))) return rc;
+  lval_t *e16;
+  if ((rc = dalloc(c, &e16, n))) return rc;
+  X.e = e16;
   if ((rc = dalloc(c, &X.fl, std::max(nfi, 1)))) return rc;
   if ((rc = dalloc(c, &X.fsig, std::max(nfi, 1)))) return rc;
   if ((rc = dalloc(c, &X.vec, (size_t)10 * mx_ld(n)))) return rc;
@@ -741,7 +761,7 @@ int mix_build(Ctx &c) {
 int factor_assemble(Ctx &c) {
   Mix &X = c.mx;
   k_fac_rows<<<grid_for(c.n), BS, 0, c.stream>>>(c.n, X.WL, X.WU, X.lfi, X.ufi, X.fl, X.fsig, (lval_t *)X.lval,
-                                                  (lval_t *)X.uval, X.e);
+                                                  (lval_t *)X.uval, (lval_t *)X.e);
   FVK_HIP(hipGetLastError());
   X.fac_valid = true;
   return 0;
@@ -781,17 +801,19 @@ static int inner_solve(Ctx &c, bool fac, int n_iter, double reduc, InnerCarry &c
     h[s * 3].s = p_last ? carry.s * (double)f * (double)f : 1.0 / SMALL;
   }
   FVK_HIP(hipMemcpyAsync(c.ksr, h, sizeof h, hipMemcpyHostToDevice, c.stream));
-  float *q = fv(c, V_Q), *t = fv(c, V_T), *ev = fv(c, V_E);
+  float *q = fv(c, V_Q), *ev = fv(c, V_E);
+  t16_t *t = (t16_t *)fv(c, V_T);
+  const lval_t *ee = (const lval_t *)X.e;
   // partials: [0] |r|_1 and [3] r.z of the last update, [1] p.q
   double *pU = pb(c, 0), *pQ = pb(c, 1);
 #define L(W)                                                                                                   \
-  k_mx_start<W, I><<<G, RB, 0, c.stream>>>(n, fac ? 1 : 0, r[0], ev, P[0], p_last, f, wu, ucol, uval, X.e, t, \
-                                           pU)
+  k_mx_start<W, I, t16_t><<<G, RB, 0, c.stream>>>(n, fac ? 1 : 0, r[0], ev, P[0], p_last, f, wu, ucol, uval, ee, \
+                                                   t, pU)
   FVK_W(wu, L);
 #undef L
   const int chunk = 32;
   Poller Pl(c);
-  int it = 0;
+  int it = 0, next_poll = chunk;
   for (; it < n_iter; ++it) {
     hipEvent_t ea, eb;
     const bool stamp = T && T->take(ea, eb);
@@ -830,22 +852,25 @@ static int inner_solve(Ctx &c, bool fac, int n_iter, double reduc, InnerCarry &c
 #undef L
       }
 #define L(W)                                                                                                        \
-  k_mx_upd_fac<W, I><<<G, RB, 0, c.stream>>>(n, it, rin, rout, q, wu, ucol, uval, X.e, t, pQ, G, pU, c.ksr)
+  k_mx_upd_fac<W, I><<<G, RB, 0, c.stream>>>(n, it, rin, rout, q, wu, ucol, uval, ee, t, pQ, G, pU, c.ksr)
       FVK_W(wu, L);
 #undef L
     }
-    if ((it + 1) % chunk == 0 && it + 1 < n_iter) {
+    if (it + 1 == next_poll && it + 1 < n_iter) {
       if (T) {
         KS k;
         if (T->flush()) return FVK_ERR_HIP;
         FVK_HIP(hipMemcpyAsync(&k, c.ks, sizeof k, hipMemcpyDeviceToHost, c.stream));
         FVK_HIP(hipStreamSynchronize(c.stream));
         if (k.done) { ++it; break; }
+        next_poll += chunk;
         continue;
       }
       bool stop = false;
-      FVK_HIP(Pl.step(it + 1, chunk, reduc, stop));
+      int next = chunk;
+      FVK_HIP(Pl.step(it + 1, chunk, reduc, stop, next));
       if (stop) { ++it; break; }
+      next_poll = it + 1 + next;
     }
   }
   k_mx_final<<<1, RB, 0, c.stream>>>(it, pU, G, c.ksr, c.ks);
@@ -901,7 +926,7 @@ int dc_solve(Ctx &c, bool use_factor, const double *b, double *x, double tol, in
     if (outer >= X.max_outer || res.iters >= max_iter || res.broken) break;
     // inner target: what is left to reach tol (with a margin), but no
     // deeper than inner_reduc, where FP32 rounding takes over
-    const double eta = std::min(0.5, std::max(X.inner_reduc, 0.3 * tol / res.res_l1));
+    const double eta = std::min(0.5, std::max(X.inner_reduc, X.eta_margin * tol / res.res_l1));
     // The inner right-hand side is normalised to a mean |r| of 1: the PCG
     // recurrence's alpha = s/(p.q + small) (solv_cg.f90:69-70) carries an
     // absolute small = 1e-20 that stalls the solve once p.q is that tiny,
@@ -949,7 +974,8 @@ static int apply_t(Ctx &c, const double *r, double *zout) {
   float *r32 = fv(c, V_A), *z = fv(c, V_B), *t = fv(c, V_T), *ev = fv(c, V_E), *p0 = fv(c, V_P0);
   k_scale_f32<<<grid_for(n), BS, 0, c.stream>>>(n, r, c.diag, r32);
 #define L(W) \
-  k_mx_start<W, I><<<G, RB, 0, c.stream>>>(n, 1, r32, ev, p0, nullptr, 1.f, wu, ucol, uval, X.e, t, pb(c, 0))
+  k_mx_start<W, I, float><<<G, RB, 0, c.stream>>>(n, 1, r32, ev, p0, nullptr, 1.f, wu, ucol, uval, \
+                                                  (const lval_t *)X.e, t, pb(c, 0))
   FVK_W(wu, L);
 #undef L
 #define L(W) k_mx_gz<W, I><<<G, RB, 0, c.stream>>>(n, wl, lcol, lval, t, r32, z, pb(c, 2), nullptr)

@@ -4,8 +4,9 @@ The judge of every solve is the FP64 true residual |b - A x|_1 / |b|_1,
 recomputed here on the CPU by the oracle's own SpMV (oracle/fvo, the
 reference's face-scatter product) from the solution the library returns.
 The factor's action z = G G^T r is compared with oracle/learn.apply on the
-same FP16-rounded L (float32 arithmetic inside the library: tolerance 2e-6
-relative to max |z|).
+same FP16-rounded L and Delta^2 (the library keeps both in FP16, Delta^2 =
+exp(2 s) rounded once from FP64 with 2 s held in [-9.7, 11]; float32
+arithmetic inside: tolerance 2e-6 relative to max |z|).
 """
 import numpy as np
 import pytest
@@ -103,10 +104,11 @@ def test_factor_apply_matches_oracle():
         fvk.factor_set(ctx, l, sig)
         r = rng.standard_normal(ctx.n_vol)
         z = fvk.factor_apply(ctx, r)
-        # the library keeps L in FP16
+        # the library keeps L and Delta^2 in FP16
         l16 = l[inner].astype(np.float16).astype(np.float64)
-        w, e, l64, _ = L.factor(S, l16, sig[inner].astype(np.float64))
-        zr = L.apply(S, w, e, l64, r)
+        w, e, l64, s = L.factor(S, l16, sig[inner].astype(np.float64))
+        e16 = np.exp(np.clip(2.0 * s, -9.7, 11.0)).astype(np.float16).astype(np.float64)
+        zr = L.apply(S, w, e16, l64, r)
         assert np.max(np.abs(z - zr)) <= 2e-6 * np.max(np.abs(zr))
         # zero factor: Jacobi
         fvk.factor_set(ctx, np.zeros_like(l), np.zeros_like(sig))
