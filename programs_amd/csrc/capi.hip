@@ -182,7 +182,8 @@ static int ctx_build(Ctx &c, const Topo &t, int device, int flags) {
   }
   if (Ws == 0) Ws = 1;
   c.W = W; c.Ws = Ws;
-  std::vector<int> cf((size_t)W * n, PAD), spc((size_t)Ws * n, -1), spf((size_t)Ws * n, -1);
+  const int ld = slot_ld(n);
+  std::vector<int> cf((size_t)W * n, PAD), spc((size_t)Ws * ld, -1), spf((size_t)Ws * ld, -1);
   for (int r = 0; r < n; ++r) {
     const int o = c.perm[r];
     int k = 0, ks = 0;
@@ -192,8 +193,8 @@ static int ctx_build(Ctx &c, const Topo &t, int device, int flags) {
       cf[(size_t)k * n + r] = own ? f : ~f;
       if (t.fin[f] >= 0) {
         const int j = own ? t.fin[f] : t.fip[f];
-        spc[(size_t)ks * n + r] = c.iperm[j];
-        spf[(size_t)ks * n + r] = f;
+        spc[(size_t)ks * ld + r] = c.iperm[j];
+        spf[(size_t)ks * ld + r] = f;
         ++ks;
       }
     }
@@ -216,7 +217,7 @@ static int ctx_build(Ctx &c, const Topo &t, int device, int flags) {
     return o;
   };
   DMesh &M = c.M;
-  M.n = n; M.nf = nf; M.nbf = t.nbf; M.W = W; M.Ws = Ws; M.ndx = t.n_dx;
+  M.n = n; M.nf = nf; M.nbf = t.nbf; M.W = W; M.Ws = Ws; M.ndx = t.n_dx; M.ld = ld;
   double *p;
   int *ip;
   int rc = 0;
@@ -235,7 +236,7 @@ static int ctx_build(Ctx &c, const Topo &t, int device, int flags) {
   if ((rc = build_levels(c, t, c.iperm, true, c.lf))) return rc;
   if ((rc = build_levels(c, t, c.iperm, false, c.lb))) return rc;
   if ((rc = dalloc(c, &c.diag, n))) return rc;
-  if ((rc = dalloc(c, &c.spv, (size_t)Ws * n))) return rc;
+  if ((rc = dalloc(c, &c.spv, (size_t)Ws * ld))) return rc;
   if ((rc = dalloc(c, &c.ngp, nf))) return rc;
   if ((rc = dalloc(c, &c.ngn, nf))) return rc;
   if ((rc = dalloc(c, &c.pc_c, n))) return rc;
@@ -295,7 +296,7 @@ __global__ __launch_bounds__(BS) void k_spv_from_faces(DMesh M, const double *ng
     if (e == PAD) break;
     const int f = e >= 0 ? e : ~e;
     if (M.fin[f] < 0) continue;
-    spv[ks * M.n + i] = e >= 0 ? ngp[f] : ngn[f];
+    spv[(size_t)ks * M.ld + i] = e >= 0 ? ngp[f] : ngn[f];
     ++ks;
   }
 }

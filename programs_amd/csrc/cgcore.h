@@ -17,12 +17,12 @@ template <int WS>
 __device__ __forceinline__ double row_spmv(const DMesh &M, const double *__restrict__ diag,
                                            const double *__restrict__ spv, const double *__restrict__ x,
                                            int i) {
-  const int n = M.n;
+  const size_t ld = M.ld;
   double y = diag[i] * x[i];
 #pragma unroll
   for (int k = 0; k < nslots<WS>(M); ++k) {
-    const int j = M.spc[k * n + i];
-    const double a = spv[k * n + i];
+    const int j = M.spc[k * ld + i];
+    const double a = spv[k * ld + i];
     if (j >= 0) y = y + a * x[j];
   }
   return y;

@@ -20,6 +20,7 @@ constexpr int PAD = INT_MIN;     // empty slot of the cell-face table
 // arrays keep the reference's face order.  Vectors are structure-of-arrays.
 struct DMesh {
   int n, nf, nbf, W, Ws, ndx;
+  int ld;  // stride of the interior-slot tables spc, spf and the matrix's spv
   const double *cx, *cy, *cz, *v, *vr;
   const int *rank;                 // reference cell id (0-based) of each row
   const int *fip, *fin, *fib, *fbty;  // rows; fin = -1 and fib >= 0 on boundary faces
@@ -27,8 +28,8 @@ struct DMesh {
   const double *fx, *fy, *fz, *fnx, *fny, *fnz;
   const double *dpx, *dpy, *dpz, *dnx, *dny, *dnz;
   const int *cf;    // W*n, slot-major: f (owner side), ~f (neighbour side), PAD
-  const int *spc;   // Ws*n interior slots: column row (row itself on padding)
-  const int *spf;   // Ws*n: face id of the slot, -1 on padding
+  const int *spc;   // Ws x ld interior slots: column row, -1 on padding
+  const int *spf;   // Ws x ld: face id of the slot, -1 on padding
 };
 
 __device__ __forceinline__ double fx_(const DMesh &M, const double *a, int f) { return a[f]; }
@@ -201,5 +202,11 @@ inline int red_grid(int n) {
   return g;
 }
 inline int grid_for(int n) { return n > 0 ? (n + BS - 1) / BS : 1; }
+
+// Stride of the slot-major FP64 tables: n plus 128 KiB + 512 B of doubles
+// (64 KiB + 256 B of ints), so that the same row of different slots does not
+// sit at the same offset of its memory page (slots a power of two apart
+// collide in the same HBM channels).
+inline int slot_ld(int n) { return n + 16448; }
 
 }  // namespace fvk

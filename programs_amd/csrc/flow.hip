@@ -410,13 +410,13 @@ __global__ __launch_bounds__(BS) void k_mom_cell(DMesh M, MomPar P, int nvc, dou
         pl = pl - a;
         for (int q = 0; q < nvc; ++q) r[q] = r[q] + cor[q * nf + f];
         c_s = c_s + cfl[f];
-        spv[ks * n + i] = a;
+        spv[(size_t)ks * M.ld + i] = a;
       } else {
         const double a = ngn[f];
         pl = pl - a;
         for (int q = 0; q < nvc; ++q) r[q] = r[q] - cor[q * nf + f];
         c_s = c_s - cfl[f];
-        spv[ks * n + i] = a;
+        spv[(size_t)ks * M.ld + i] = a;
       }
       ++ks;
     } else {
@@ -546,7 +546,7 @@ __global__ __launch_bounds__(BS) void k_pp_cell(DMesh M, const double *__restric
       const double a = own ? ngp[f] : ngn[f];
       pl = pl - a;
       r = own ? r - cpp[f] : r + cpp[f];
-      spv[ks * n + i] = a;
+      spv[(size_t)ks * M.ld + i] = a;
       ++ks;
     } else if (M.fbty[f] - M.fbty[f] % 10 == 20) {
       r = r - cpp[f];
@@ -741,8 +741,8 @@ __global__ __launch_bounds__(RB) void k_row_defect(DMesh M, const double *__rest
     double s = diag[i];
 #pragma unroll
     for (int k = 0; k < nslots<WS>(M); ++k) {
-      const int j = M.spc[(size_t)k * M.n + i];
-      const double v = spv[(size_t)k * M.n + i];
+      const int j = M.spc[(size_t)k * M.ld + i];
+      const double v = spv[(size_t)k * M.ld + i];
       if (j >= 0) s += v;
     }
     a[0] += fabs(s) / fabs(diag[i]);

@@ -51,10 +51,10 @@ __global__ __launch_bounds__(BS) void k_precond_level(DMesh M, const int *__rest
   const int r = rows[t], n = M.n, rr = M.rank[r];
   double d = diag[r];
   for (int k = 0; k < M.Ws; ++k) {
-    const int j = M.spc[k * n + r];
+    const int j = M.spc[(size_t)k * M.ld + r];
     if (j < 0) break;
     if (M.rank[j] < rr) {
-      const int f = M.spf[k * n + r];
+      const int f = M.spf[(size_t)k * M.ld + r];
       if (imat == 1) d = d - ngn[f] * ngn[f] * c[j];
       else d = d - ngp[f] * c[j] * ngn[f];
     }
@@ -82,10 +82,10 @@ __global__ __launch_bounds__(BS) void k_sweep_fwd(DMesh M, const int *__restrict
   double acc[3];
   for (int q = 0; q < nvc; ++q) acc[q] = b[q * n + r];
   for (int k = 0; k < M.Ws; ++k) {
-    const int j = M.spc[k * n + r];
+    const int j = M.spc[(size_t)k * M.ld + r];
     if (j < 0) break;
     if (M.rank[j] < rr) {
-      const double a = spv[k * n + r];
+      const double a = spv[(size_t)k * M.ld + r];
       for (int q = 0; q < nvc; ++q) acc[q] = acc[q] - a * x[q * n + j];
     }
   }
@@ -105,10 +105,10 @@ __global__ __launch_bounds__(BS) void k_sweep_bwd(DMesh M, const int *__restrict
   const double crr = cr[r];
   for (int q = 0; q < nvc; ++q) acc[q] = x[q * n + r] * crr;
   for (int k = 0; k < M.Ws; ++k) {
-    const int j = M.spc[k * n + r];
+    const int j = M.spc[(size_t)k * M.ld + r];
     if (j < 0) break;
     if (M.rank[j] > rr) {
-      const double a = spv[k * n + r];
+      const double a = spv[(size_t)k * M.ld + r];
       for (int q = 0; q < nvc; ++q) acc[q] = acc[q] - a * x[q * n + j];
     }
   }
@@ -146,8 +146,8 @@ __global__ __launch_bounds__(RB) void k_residual(DMesh M, int nvc, const double 
         v = v - diag[i] * xq[i];
 #pragma unroll
         for (int k = 0; k < nslots<WS>(M); ++k) {
-          const int j = M.spc[k * n + i];
-          const double a = spv[k * n + i];
+          const int j = M.spc[(size_t)k * M.ld + i];
+          const double a = spv[(size_t)k * M.ld + i];
           if (j >= 0) v = v - a * xq[j];
         }
         r[q * n + i] = v;
@@ -178,8 +178,8 @@ __device__ __forceinline__ void cg_pq_row(const DMesh &M, int i, const double *_
   double y = diag[i] * pi;
 #pragma unroll
   for (int k = 0; k < nslots<WS>(M); ++k) {
-    const int j = M.spc[k * n + i];
-    const double a = spv[k * n + i];
+    const int j = M.spc[(size_t)k * M.ld + i];
+    const double a = spv[(size_t)k * M.ld + i];
     if (j >= 0) y = y + a * (zq[j] + beta * po[j]);
   }
   pn[i] = pi;

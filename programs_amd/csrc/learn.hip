@@ -385,8 +385,8 @@ __global__ __launch_bounds__(BS) void k_loss_face(FaceIn F, int WU, const int *_
     if (q2 < 0 || q2 == q) continue;
     const int j = F.fi_n[q2];
     for (int s = 0; s < F.M.Ws; ++s) {
-      if (F.M.spc[(size_t)s * n + i] != j) continue;
-      const int f3 = F.M.spf[(size_t)s * n + i];
+      if (F.M.spc[(size_t)s * F.M.ld + i] != j) continue;
+      const int f3 = F.M.spf[(size_t)s * F.M.ld + i];
       v += F.ngp[f3] / sqrt(F.diag[i] * F.diag[j]) * (double)fl[q2];
     }
   }

@@ -89,7 +89,7 @@ __global__ __launch_bounds__(BS) void k_mx_conv(DMesh M, const double *__restric
   const double wi = 1.0 / sqrt(diag[i]);
   w[i] = (float)wi;
   for (int k = 0; k < M.Ws; ++k) {
-    const size_t o = (size_t)k * n + i;
+    const size_t o = (size_t)k * M.ld + i;
     const int j = M.spc[o];
     s32[(size_t)k * mx_ld(n) + i] = j >= 0 ? (float)(spv[o] * wi / sqrt(diag[j])) : 0.f;
   }
@@ -659,7 +659,7 @@ int mix_build(Ctx &c) {
   if (X.built) return 0;
   const int n = c.n, nf = c.nf, Ws = c.Ws;
   // A's interior slots back on the host
-  std::vector<int> spc((size_t)Ws * n);
+  std::vector<int> spc((size_t)Ws * c.M.ld);
   FVK_HIP(hipStreamSynchronize(c.stream));
   FVK_HIP(hipMemcpy(spc.data(), c.M.spc, spc.size() * sizeof(int), hipMemcpyDeviceToHost));
   // interior faces in face order; per-row lower / upper lists
@@ -697,7 +697,7 @@ int mix_build(Ctx &c) {
   for (size_t o = 0; o < ac.size(); ++o) ac[o] = (int)(o % ld);
   for (int k = 0; k < Ws; ++k)
     for (int i = 0; i < n; ++i) {
-      const int j = spc[(size_t)k * n + i];
+      const int j = spc[(size_t)k * c.M.ld + i];
       if (j >= 0) ac[(size_t)k * ld + i] = j;
     }
   long span = 0;
