@@ -61,6 +61,7 @@ struct Mix {
   void *lval = nullptr, *uval = nullptr;  // L per slot, FP16
   float *w = nullptr;                  // a_ii^-1/2 per row
   void *e = nullptr;                   // Delta^2 per row, FP16
+  void *geo = nullptr;                 // geometric MLP features per interior face (bfloat16)
   float *fl = nullptr, *fsig = nullptr;  // per interior face l_f, sig_f
   float *vec = nullptr;                // 10 x n FP32 work vectors
   double *tot = nullptr;               // 8 doubles: residual totals

@@ -42,12 +42,9 @@ __device__ __forceinline__ v4f mfma16(v4s a, v4s b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
-// float -> bfloat16 bits, round to nearest even
-__device__ __forceinline__ short bf16(float x) {
-  unsigned u = __float_as_uint(x);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (short)(u >> 16);
-}
+// float -> bfloat16 bits, round to nearest even (v_cvt_pk_bf16_f32, two
+// values per instruction when the compiler can pair them)
+__device__ __forceinline__ short bf16(float x) { return __builtin_bit_cast(short, (__bf16)x); }
 __device__ __forceinline__ v4s pack4(v4f v) {
   v4s r;
   r[0] = bf16(v[0]); r[1] = bf16(v[1]); r[2] = bf16(v[2]); r[3] = bf16(v[3]);
@@ -107,41 +104,67 @@ struct FaceIn {
   const double *diag, *ngp;
   const int *fi_face, *fi_p, *fi_n;
   int nfi;
+  // the geometric features, bfloat16, computed once per mesh: feature 3
+  // (g0) and features 4-7 and 8-11 (g1, g2) of each interior face
+  const short *g0;
+  const v4s *g1, *g2;
 };
 
-// Features 4g .. 4g+3 of interior face q (the lane's share of the B operand)
+// Geometric features 3 .. 11 of interior face q (the bfloat16 B operand
+// entries: 3 alone, 4-7, 8-11).  The mesh does not change within a context:
+// they are computed once (k_geo_feat) and read by every forward and
+// backward pass.  Lengths in FP64 from the FP64 mesh; the cube root and the
+// logarithm in FP32 (the features enter the network in bfloat16).
+__device__ void geo_features(const FaceIn &F, int q, float &x3, v4f &x47, v4f &x811) {
+  const int f = F.fi_face[q], p = F.fi_p[q], m = F.fi_n[q];
+  // |V|: the reference's volumes (x_f s_f n_x summed from the origin,
+  // grd_geom.f90:151-153,172-173) turn negative on strongly warped cells
+  const double vp = fmax(fabs(F.M.v[p]), 1e-300), vn = fmax(fabs(F.M.v[m]), 1e-300);
+  const double h = (double)cbrtf((float)(0.5 * (vp + vn)));
+  x3 = (float)(F.M.fs[f] / (h * h));
+  double nv[3];
+  face_n(F.M, f, nv);
+  x47[0] = (float)nv[0];
+  x47[1] = (float)nv[1];
+  x47[2] = (float)nv[2];
+  x47[3] = (float)(F.M.flpn[f] / h);
+  double dp[3], dn[3];
+  face_dxp(F.M, f, dp);
+  face_dxn(F.M, f, dn);
+  x811[0] = (float)F.M.fwfp[f];
+  x811[1] = (float)(sqrt(dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2]) / h);
+  x811[2] = (float)(sqrt(dn[0] * dn[0] + dn[1] * dn[1] + dn[2] * dn[2]) / h);
+  x811[3] = logf((float)(vp / vn));
+}
+
+__global__ __launch_bounds__(BS) void k_geo_feat(FaceIn F, short *g0, v4s *g1, v4s *g2) {
+  const int q = blockIdx.x * BS + threadIdx.x;
+  if (q >= F.nfi) return;
+  float x3;
+  v4f a, b;
+  geo_features(F, q, x3, a, b);
+  g0[q] = bf16(x3);
+  g1[q] = pack4(a);
+  g2[q] = pack4(b);
+}
+
+// Features 4g .. 4g+3 of interior face q (the lane's share of the B operand):
+// the coefficient ratios of group 0 from the current matrix, the rest stored
 __device__ v4s features4(const FaceIn &F, int q, int g) {
-  v4f x = {0.f, 0.f, 0.f, 0.f};
+  v4s r = {0, 0, 0, 0};
   if (q < F.nfi && g < 3) {
-    const int f = F.fi_face[q], p = F.fi_p[q], m = F.fi_n[q];
-    // |V|: the reference's volumes (x_f s_f n_x summed from the origin,
-    // grd_geom.f90:151-153,172-173) turn negative on strongly warped cells
-    const double vp = fmax(fabs(F.M.v[p]), 1e-300), vn = fmax(fabs(F.M.v[m]), 1e-300);
-    const double h = cbrt(0.5 * (vp + vn));
     if (g == 0) {
+      const int f = F.fi_face[q], p = F.fi_p[q], m = F.fi_n[q];
       const double a = -F.ngp[f], dp = F.diag[p], dn = F.diag[m];
-      x[0] = (float)(a / sqrt(dp * dn));
-      x[1] = (float)(a / dp);
-      x[2] = (float)(a / dn);
-      x[3] = (float)(F.M.fs[f] / (h * h));
-    } else if (g == 1) {
-      double nv[3];
-      face_n(F.M, f, nv);
-      x[0] = (float)nv[0];
-      x[1] = (float)nv[1];
-      x[2] = (float)nv[2];
-      x[3] = (float)(F.M.flpn[f] / h);
+      r[0] = bf16((float)(a / sqrt(dp * dn)));
+      r[1] = bf16((float)(a / dp));
+      r[2] = bf16((float)(a / dn));
+      r[3] = F.g0[q];
     } else {
-      double dp[3], dn[3];
-      face_dxp(F.M, f, dp);
-      face_dxn(F.M, f, dn);
-      x[0] = (float)F.M.fwfp[f];
-      x[1] = (float)(sqrt(dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2]) / h);
-      x[2] = (float)(sqrt(dn[0] * dn[0] + dn[1] * dn[1] + dn[2] * dn[2]) / h);
-      x[3] = (float)log(vp / vn);
+      r = g == 1 ? F.g1[q] : F.g2[q];
     }
   }
-  return pack4(x);
+  return r;
 }
 
 // Forward pass of one 16-face tile.  h1, h2: tanh activations (float, D
@@ -172,6 +195,32 @@ __device__ __forceinline__ void tile_forward(const WLds &s, v4s xb, v4f (&h1)[4]
   out = acc;
 }
 
+// The same forward pass keeping only the bfloat16 operands (the factor build
+// needs no activations for a backward pass: fewer registers, more waves).
+__device__ __forceinline__ v4f tile_forward_out(const WLds &s, v4s xb) {
+  const int l = threadIdx.x & 63, r = l & 15, g = l >> 4;
+  v4s B1[4], B2[4];
+#pragma unroll
+  for (int ob = 0; ob < 4; ++ob) {
+    v4f acc = {s.b1[ob * 16 + 4 * g], s.b1[ob * 16 + 4 * g + 1], s.b1[ob * 16 + 4 * g + 2],
+               s.b1[ob * 16 + 4 * g + 3]};
+    acc = mfma16(*(const v4s *)&s.w1t[(ob * 16 + r) * 16 + 4 * g], xb, acc);
+    B1[ob] = pack4(tanh4(acc));
+  }
+#pragma unroll
+  for (int ob = 0; ob < 4; ++ob) {
+    v4f acc = {s.b2[ob * 16 + 4 * g], s.b2[ob * 16 + 4 * g + 1], s.b2[ob * 16 + 4 * g + 2],
+               s.b2[ob * 16 + 4 * g + 3]};
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) acc = mfma16(*(const v4s *)&s.w2t[(ob * 16 + r) * 64 + kb * 16 + 4 * g], B1[kb], acc);
+    B2[ob] = pack4(tanh4(acc));
+  }
+  v4f acc = {s.b3[4 * g], s.b3[4 * g + 1], s.b3[4 * g + 2], s.b3[4 * g + 3]};
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb) acc = mfma16(*(const v4s *)&s.w3t[r * 64 + kb * 16 + 4 * g], B2[kb], acc);
+  return acc;
+}
+
 // Outputs (l_f, sig_f) of every interior face.
 __global__ __launch_bounds__(MB) void k_mlp_fwd(FaceIn F, const float *__restrict__ theta, float *__restrict__ fl,
                                                 float *__restrict__ fsig) {
@@ -182,9 +231,7 @@ __global__ __launch_bounds__(MB) void k_mlp_fwd(FaceIn F, const float *__restric
   const int tiles = (F.nfi + 15) / 16;
   for (int t = wave; t < tiles; t += nwaves) {
     const int q = t * 16 + r;
-    v4f h1[4], h2[4], out;
-    v4s B1[4], B2[4];
-    tile_forward(s, features4(F, q, g), h1, B1, h2, B2, out);
+    const v4f out = tile_forward_out(s, features4(F, q, g));
     if (g == 0 && q < F.nfi) {
       fl[q] = out[0];
       fsig[q] = out[1];
@@ -484,7 +531,26 @@ FaceIn face_in(Ctx &c) {
   F.fi_p = c.mx.fi_p;
   F.fi_n = c.mx.fi_n;
   F.nfi = c.mx.nfi;
+  const size_t nfi = c.mx.nfi;
+  short *g = (short *)c.mx.geo;
+  F.g1 = (const v4s *)g;
+  F.g2 = (const v4s *)(g + 4 * nfi);
+  F.g0 = g + 8 * nfi;
   return F;
+}
+
+// The geometric features of the context's mesh, once
+int geo_build(Ctx &c) {
+  Mix &X = c.mx;
+  if (X.geo) return 0;
+  short *g;
+  int rc = dalloc(c, &g, (size_t)9 * std::max(X.nfi, 1));
+  if (rc) return rc;
+  X.geo = g;
+  const FaceIn F = face_in(c);
+  k_geo_feat<<<grid_for(X.nfi), BS, 0, c.stream>>>(F, (short *)F.g0, (v4s *)F.g1, (v4s *)F.g2);
+  FVK_HIP(hipGetLastError());
+  return 0;
 }
 
 // Loss scratch inside the context's Krylov workspace (8 x 3 x n doubles,
@@ -519,6 +585,7 @@ int check_model(fvk_model *m, Ctx &c) {
 int run_forward(fvk_model *m, Ctx &c) {
   int rc = mix_build(c);
   if (rc) return rc;
+  if ((rc = geo_build(c))) return rc;
   const FaceIn F = face_in(c);
   const int tiles = (F.nfi + 15) / 16;
   int blocks = std::min(MLP_FWD_BLOCKS, std::max(1, (tiles + 3) / 4));
