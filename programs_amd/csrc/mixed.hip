@@ -51,8 +51,10 @@ namespace fvk {
 typedef _Float16 lval_t;
 // t = Delta^2 (I + L^T) r as the update kernel hands it to the direction
 // kernel: FP16, divided by a scale both kernels derive from the same
-// committed scalars (mean |r| of the decision the update follows), so that
-// it sits near 1 whatever the residual's size.
+// committed scalars (the decision's r.z = sum Delta^2 u^2: 64 sqrt(r.z / n),
+// so that |t| / scale stays below Delta_max sqrt(n) / 64 whatever the
+// residual's size and shape).
+// The first direction of a solve reads the start's t in FP32.
 typedef _Float16 t16_t;
 
 // Column of slot k of row i: 32-bit rows, or 16-bit offsets from the row.
@@ -72,12 +74,20 @@ __device__ __forceinline__ bool gate_done(const KS *g) { return g && g->done; }
 // (5 % slower on the 2^21-row system for the vectors alone).
 __host__ __device__ __forceinline__ size_t mx_ld(int n) { return (size_t)n + 16448; }
 
-// Scale of the stored t after decision `it` (its scalars k): the mean |r|
-// (1 when it is not usable)
-__device__ __forceinline__ float t_scale(const KS &k, int it, int n) {
-  const double s = (it == 0 ? k.res0_n : k.res_n * (k.res0_n + SMALL)) / n;
-  return s > 0.0 && s < 1e30 ? (float)s : 1.f;
+// Scale of the t stored after a decision with scalars k (1 when r.z is not
+// usable); FP32 arithmetic, the same in both kernels (an FP64 square root
+// and division here cost the update kernel 15 %)
+__device__ __forceinline__ float t_scale(const KS &k, int n) {
+  const float s = 64.f * sqrtf((float)k.s / (float)n);
+  return s > 0.f && s < 1e30f ? s : 1.f;
 }
+// FP16, rounded to nearest.  The scale keeps |t| / scale below
+// Delta_max sqrt(n) / 64 < 65504 for n < 2.8e8 rows even when all of r.z sits
+// in one row; beyond, an overflow turns into a breakdown of the inner solve
+// (and the plain FP64 solver takes over).  (Rounding toward zero, which
+// saturates, cost 13 % more iterations; a clamp before the conversion cost
+// the update kernel registers and 15 % of its time.)
+__device__ __forceinline__ t16_t to_t16(float v) { return (t16_t)v; }
 
 // Jacobi-scaled FP32 copy of the matrix, w = a_ii^-1/2
 __global__ __launch_bounds__(BS) void k_mx_conv(DMesh M, const double *__restrict__ diag,
@@ -261,9 +271,9 @@ __global__ __launch_bounds__(RB, 8) void k_mx_pq(int n, int ws, int it, const I 
 // alpha and direction; z = t + L t; p = z + beta p in place.  The first row
 // of a thread is loaded before the decision is known (more rows ahead
 // measured slower).
-template <int WL, class I>
+template <int WL, class I, class TT>
 __device__ __forceinline__ float mx_z_row(int n, int i, int wl, const I *__restrict__ lcol,
-                                          const lval_t *__restrict__ lval, const t16_t *__restrict__ t) {
+                                          const lval_t *__restrict__ lval, const TT *__restrict__ t) {
   float acc = (float)t[i];
 #pragma unroll
   for (int k = 0; k < nw<WL>(wl); ++k) {
@@ -273,11 +283,11 @@ __device__ __forceinline__ float mx_z_row(int n, int i, int wl, const I *__restr
   return acc;
 }
 
-template <int WL, class I>
+template <int WL, class I, class TT>
 struct ZRow {
   float ti, pi, ei, l[WL > 0 ? WL : 1], tj[WL > 0 ? WL : 1];
   __device__ __forceinline__ void load(int n, int i, const I *__restrict__ lcol, const lval_t *__restrict__ lval,
-                                       const t16_t *__restrict__ t, const float *__restrict__ p,
+                                       const TT *__restrict__ t, const float *__restrict__ p,
                                        const float *__restrict__ ev) {
     ti = (float)t[i];
     pi = p[i];
@@ -297,9 +307,9 @@ struct ZRow {
   }
 };
 
-template <int WL, class I>
+template <int WL, class I, class TT>
 __global__ __launch_bounds__(RB, 8) void k_mx_zp(int n, int wl, int it, const I *__restrict__ lcol,
-                                              const lval_t *__restrict__ lval, const t16_t *__restrict__ t,
+                                              const lval_t *__restrict__ lval, const TT *__restrict__ t,
                                               float *__restrict__ p, float *__restrict__ ev,
                                               const double *__restrict__ partU, int PU, KS *ring, KS *out) {
   int r0, r1;
@@ -307,7 +317,7 @@ __global__ __launch_bounds__(RB, 8) void k_mx_zp(int n, int wl, int it, const I 
   const int i0 = r0 + threadIdx.x;
   SelLoads<2> S;
   mx_issue(S, partU, PU);
-  ZRow<WL, I> A0;
+  ZRow<WL, I, TT> A0;
   if (WL > 0) {
     if (i0 < r1) A0.load(n, i0, lcol, lval, t, p, ev);
   }
@@ -315,8 +325,9 @@ __global__ __launch_bounds__(RB, 8) void k_mx_zp(int n, int wl, int it, const I 
   mx_decide(it, S, ring, out, sk, true);
   if (sk[0].done) return;
   const float bf = (float)sk[0].beta, af = (float)sk[0].alpha;
-  // t was stored divided by the scale of the previous decision
-  const float ts = it == 0 ? 1.f : t_scale(ring[((it + 1) & 1) * 3], it - 1, n);
+  // t was stored divided by the scale of the previous decision (FP32 and
+  // unscaled from the start of the solve)
+  const float ts = it == 0 ? 1.f : t_scale(ring[((it + 1) & 1) * 3], n);
   int i = i0;
   if (WL > 0) {
     if (i0 < r1) {
@@ -326,7 +337,7 @@ __global__ __launch_bounds__(RB, 8) void k_mx_zp(int n, int wl, int it, const I 
     i = i0 + RB;
   }
   auto row = [&](int i) {
-    const float z = ts * mx_z_row<WL, I>(n, i, wl, lcol, lval, t), pi = p[i];
+    const float z = ts * mx_z_row<WL, I, TT>(n, i, wl, lcol, lval, t), pi = p[i];
     ev[i] += af * pi;
     p[i] = z + bf * pi;
   };
@@ -510,7 +521,7 @@ __global__ __launch_bounds__(RB, 8) void k_mx_upd_fac(int n, int it, const float
   if (cur->done) return;
   double alpha;
   if (!mx_alpha(cur, S, alpha)) return;
-  const float af = (float)alpha, tsi = 1.f / t_scale(*cur, it, n);
+  const float af = (float)alpha, tsi = 1.f / t_scale(*cur, n);
   double a[4] = {0, 0, 0, 0};
   auto fin = [&](int i, const UpdRow<WU, I> &R) {
     const float ri = R.ri - af * R.qi;
@@ -519,7 +530,7 @@ __global__ __launch_bounds__(RB, 8) void k_mx_upd_fac(int n, int it, const float
     float acc = ri;
 #pragma unroll
     for (int k = 0; k < WU; ++k) acc += R.u[k] * (R.rj[k] - af * R.qj[k]);
-    t[i] = (t16_t)(R.ee * acc * tsi);
+    t[i] = to_t16(R.ee * acc * tsi);
     a[3] += (double)R.ee * (double)acc * (double)acc;
   };
   auto row = [&](int i) {
@@ -534,7 +545,7 @@ __global__ __launch_bounds__(RB, 8) void k_mx_upd_fac(int n, int it, const float
       acc += (float)uval[o] * (rin[j] - af * q[j]);
     }
     const float ei = (float)ee[i];
-    t[i] = (t16_t)(ei * acc * tsi);
+    t[i] = to_t16(ei * acc * tsi);
     a[3] += (double)ei * (double)acc * (double)acc;
   };
   int i = i0;
@@ -803,12 +814,13 @@ static int inner_solve(Ctx &c, bool fac, int n_iter, double reduc, InnerCarry &c
   FVK_HIP(hipMemcpyAsync(c.ksr, h, sizeof h, hipMemcpyHostToDevice, c.stream));
   float *q = fv(c, V_Q), *ev = fv(c, V_E);
   t16_t *t = (t16_t *)fv(c, V_T);
+  float *t0 = fv(c, V_A);  // the start's t, FP32
   const lval_t *ee = (const lval_t *)X.e;
   // partials: [0] |r|_1 and [3] r.z of the last update, [1] p.q
   double *pU = pb(c, 0), *pQ = pb(c, 1);
 #define L(W)                                                                                                   \
-  k_mx_start<W, I, t16_t><<<G, RB, 0, c.stream>>>(n, fac ? 1 : 0, r[0], ev, P[0], p_last, f, wu, ucol, uval, ee, \
-                                                   t, pU)
+  k_mx_start<W, I, float><<<G, RB, 0, c.stream>>>(n, fac ? 1 : 0, r[0], ev, P[0], p_last, f, wu, ucol, uval, ee, \
+                                                   t0, pU)
   FVK_W(wu, L);
 #undef L
   const int chunk = 32;
@@ -837,9 +849,15 @@ static int inner_solve(Ctx &c, bool fac, int n_iter, double reduc, InnerCarry &c
       // factor: p in place in P[0]; r double-buffered
       float *p = P[0], *rin = r[it & 1], *rout = r[(it + 1) & 1];
       const float *sp = X.spv, *pc = p;
-#define L(W) k_mx_zp<W, I><<<G, RB, 0, c.stream>>>(n, wl, it, lcol, lval, t, p, ev, pU, G, c.ksr, c.ks)
-      FVK_W(wl, L);
+      if (it == 0) {
+#define L(W) k_mx_zp<W, I, float><<<G, RB, 0, c.stream>>>(n, wl, it, lcol, lval, t0, p, ev, pU, G, c.ksr, c.ks)
+        FVK_W(wl, L);
 #undef L
+      } else {
+#define L(W) k_mx_zp<W, I, t16_t><<<G, RB, 0, c.stream>>>(n, wl, it, lcol, lval, t, p, ev, pU, G, c.ksr, c.ks)
+        FVK_W(wl, L);
+#undef L
+      }
       if (stamp) {
 #define L(W)                                                                                                     \
   hipExtLaunchKernelGGL(k_mx_q<W, I>, dim3(G), dim3(RB), 0, c.stream, ea, eb, 0, n, ws, it, col, sp, pc, q, pQ, \
