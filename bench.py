@@ -24,10 +24,13 @@ and the pressure-correction systems of a SIMPLE run).
 
 Printed: one JSON line with the metric, the headline solve time and its
 speedup over the plain solve, the roofline of the headline's dominant kernel
-(the FP32 p-update + SpMV, timed live with HIP events on its own stream) and
-of the plain path's, the FP64 SpMV bandwidth, a device copy's bandwidth, and
-the CPU oracle's time for the plain solve (single core, extrapolated from a
-bounded sample of iterations).
+(the FP32 SpMV q = Ah p with its p.q partials, timed live with HIP events on
+the solve's stream; its HBM traffic from the byte counters recorded in
+profiles/hbm_traffic.json) and of the plain path's fused p-update + SpMV,
+the FP64 SpMV bandwidth, a device copy's bandwidth, SIMPLE outer iterations
+on the 128^3 cavity (reference pressure solves, eager and from HIP graphs,
+and pressure solves to 1e-8 plain and learned), and the CPU oracle's times
+(single core, extrapolated from bounded samples).
 """
 import argparse
 import json
@@ -42,6 +45,24 @@ sys.path.insert(0, HERE)
 
 METRIC = "pressure-solve s to 1e-8 rel. residual (learned vs plain PCG); SpMV HBM GB/s"
 PEAK_HBM = 8.0e12
+# HBM bytes per launch of the roofline kernels, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE runs of this script (the commands and the copy calibration are
+# in the file)
+TRAFFIC = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic.json")
+
+
+def traffic_bytes(kernel):
+    """HBM bytes per launch of `kernel` measured by the byte counters, each
+    scaled by what the same counter reads for a device copy of known size;
+    None without a measurement."""
+    try:
+        with open(TRAFFIC) as f:
+            t = json.load(f)
+        k = t["kernels"][kernel]
+        c = t["calibration"]
+        return 1024.0 * (k["fetch_kib"] / c["fetch_factor"] + k["write_kib"] / c["write_factor"])
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def log(*a):
@@ -284,7 +305,7 @@ def main():
     rk = fvk.sys_solve(ctx, fvk.SOLVE_PCG64, 0, args.tol, 200000, time_kernel=True)
     kb, ks = rk["kern_bytes"], rk["kern_seconds"]
     roof64 = dict(bound="hbm", kernel="k_cg_pq (p = z + beta p; q = A p; p.q)", achieved=kb / ks / 1e9,
-                peak=PEAK_HBM / 1e9, unit="GB/s", frac=kb / ks / PEAK_HBM, traffic=None,
+                peak=PEAK_HBM / 1e9, unit="GB/s", frac=kb / ks / PEAK_HBM, traffic=traffic_bytes("k_cg_pq"),
                 bytes_per_launch=kb, us_per_launch=ks * 1e6, launches=rk["kern_launches"])
 
     # stand-alone FP64 SpMV, BASELINE's CSR byte model nnz(w+4) + 4(n+1) + 2wn
@@ -308,7 +329,7 @@ def main():
         dtype = "f32-inner/f64-outer"
         roof = dict(bound="hbm", kernel="k_mx_q (FP32: q = Ah p; p.q)",
                     achieved=head["k_mx_q_gbs"], peak=PEAK_HBM / 1e9, unit="GB/s",
-                    frac=head["k_mx_q_gbs"] * 1e9 / PEAK_HBM, traffic=None,
+                    frac=head["k_mx_q_gbs"] * 1e9 / PEAK_HBM, traffic=traffic_bytes("k_mx_q"),
                     bytes_per_launch=head["k_mx_q_bytes"], us_per_launch=head["k_mx_q_us"])
     else:
         value, hr = wall, dict(iters=r["iters"], res_true_l1=r["res_true_l1"], res_true_l2=r["res_true_l2"])
