@@ -787,9 +787,10 @@ int fvk_sys_solve(fvk_ctx *cc, int method, int imat, double tol, int max_iter, i
       // read; p_new, q written; 16/32-bit column + FP32 value per off-diagonal
       info->kern_bytes = 16.0 * n + (c.mx.i16 ? 6.0 : 8.0) * nnz_off;
     else
-      // k_mx_q (factor path): p read, q written; column + value per
-      // off-diagonal
-      info->kern_bytes = 8.0 * n + (c.mx.i16 ? 6.0 : 8.0) * nnz_off;
+      // k_mx_q (factor path): p read, q written; per upper off-diagonal a
+      // column and an FP32 value, per lower one a column and a one-byte slot
+      // index (the value is the upper entry of the neighbour, read from cache)
+      info->kern_bytes = 8.0 * n + (c.mx.i16 ? 4.5 : 6.5) * nnz_off;
     info->kern_launches = T.launches;
   }
   return status;

@@ -58,6 +58,8 @@ struct Mix {
   int *face_fi = nullptr;              // face id -> interior face or -1 (nf)
   int *fi_p = nullptr, *fi_n = nullptr;  // rows of an interior face's owner / neighbour
   float *spv = nullptr;                // Jacobi-scaled off-diagonals a_ij / sqrt(a_ii a_jj), FP32
+  float *au = nullptr;                 // the same towards the upper neighbours only ((WU + 1) x ld)
+  unsigned char *iau = nullptr;        // lower slot -> slot in the neighbour's upper list
   void *lval = nullptr, *uval = nullptr;  // L per slot, FP16
   float *w = nullptr;                  // a_ii^-1/2 per row
   void *e = nullptr;                   // Delta^2 per row, FP16

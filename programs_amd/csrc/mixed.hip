@@ -349,41 +349,56 @@ __global__ __launch_bounds__(RB, 8) void k_mx_zp(int n, int wl, int it, const I 
 }
 
 // Factor, iteration it: q = Ah p; partial p.q (nothing once the solve is
-// done).  The first two rows of a thread are loaded before the done flag is
-// read, so that its latency hides behind them.
-template <int WS, class I>
+// done).  Ah is symmetric, so only the entries towards the upper neighbours
+// are stored (au, FP32, one slot per upper neighbour); the entry towards a
+// lower neighbour j is j's upper entry, read from au[iau][j] (iau: the slot
+// of this row in j's upper list, one byte; padding points at a zero slot).
+// Those reads go to rows just streamed, so the kernel reads half the values
+// from HBM (35n instead of 44n bytes per launch on a hexahedral mesh; 5 %
+// faster, as the kernel is not bound by HBM alone).  The first row of a
+// thread is loaded before the done flag is read, so that its latency hides
+// behind it.
+template <int WX, class I>
 struct QRow {
-  float pi, a[WS > 0 ? WS : 1], pj[WS > 0 ? WS : 1];
-  __device__ __forceinline__ void load(int n, int i, const I *__restrict__ col, const float *__restrict__ spv,
+  float pi, a[WX > 0 ? WX : 1], pu[WX > 0 ? WX : 1], b[WX > 0 ? WX : 1], pl[WX > 0 ? WX : 1];
+  __device__ __forceinline__ void load(int n, int i, const I *__restrict__ ucol, const I *__restrict__ lcol,
+                                       const float *__restrict__ au, const unsigned char *__restrict__ iau,
                                        const float *__restrict__ p) {
+    const size_t ld = mx_ld(n);
     pi = p[i];
 #pragma unroll
-    for (int k = 0; k < WS; ++k) {
-      const size_t o = (size_t)k * mx_ld(n) + i;
-      a[k] = spv[o];
-      pj[k] = p[colat(col, o, i)];
+    for (int k = 0; k < WX; ++k) {
+      const size_t o = (size_t)k * ld + i;
+      a[k] = au[o];
+      pu[k] = p[colat(ucol, o, i)];
+      const int j = colat(lcol, o, i);
+      b[k] = au[(size_t)iau[o] * ld + j];
+      pl[k] = p[j];
     }
   }
   __device__ __forceinline__ float y() const {
     float acc = pi;
 #pragma unroll
-    for (int k = 0; k < WS; ++k) acc += a[k] * pj[k];
+    for (int k = 0; k < WX; ++k) acc += a[k] * pu[k];
+#pragma unroll
+    for (int k = 0; k < WX; ++k) acc += b[k] * pl[k];
     return acc;
   }
 };
 
-template <int WS, class I>
-__global__ __launch_bounds__(RB, 8) void k_mx_q(int n, int ws, int it, const I *__restrict__ col,
-                                             const float *__restrict__ spv, const float *__restrict__ p,
-                                             float *__restrict__ q, double *__restrict__ partQ, const KS *ring) {
+template <int WX, class I>
+__global__ __launch_bounds__(RB, 8) void k_mx_q(int n, int wx, int it, const I *__restrict__ ucol,
+                                                const I *__restrict__ lcol, const float *__restrict__ au,
+                                                const unsigned char *__restrict__ iau, const float *__restrict__ p,
+                                                float *__restrict__ q, double *__restrict__ partQ, const KS *ring) {
   int r0, r1;
   block_rows(n, r0, r1);
   const int i0 = r0 + threadIdx.x;
-  constexpr int NA = WS == 0 ? 0 : (WS <= 6 ? 2 : 1);
-  QRow<WS, I> A[NA > 0 ? NA : 1];
+  constexpr int NA = WX == 0 ? 0 : 1;
+  QRow<WX, I> A[NA > 0 ? NA : 1];
 #pragma unroll
   for (int a = 0; a < NA; ++a)
-    if (i0 + a * RB < r1) A[a].load(n, i0 + a * RB, col, spv, p);
+    if (i0 + a * RB < r1) A[a].load(n, i0 + a * RB, ucol, lcol, au, iau, p);
   if (ring[(it & 1) * 3].done) return;
   double pq = 0.0;
 #pragma unroll
@@ -395,13 +410,20 @@ __global__ __launch_bounds__(RB, 8) void k_mx_q(int n, int ws, int it, const I *
       pq += (double)A[a].pi * (double)y;
     }
   }
+  const size_t ld = mx_ld(n);
   auto row = [&](int i) {
     const float pi = p[i];
     float y = pi;
 #pragma unroll
-    for (int k = 0; k < nw<WS>(ws); ++k) {
-      const size_t o = (size_t)k * mx_ld(n) + i;
-      y += spv[o] * p[colat(col, o, i)];
+    for (int k = 0; k < nw<WX>(wx); ++k) {
+      const size_t o = (size_t)k * ld + i;
+      y += au[o] * p[colat(ucol, o, i)];
+    }
+#pragma unroll
+    for (int k = 0; k < nw<WX>(wx); ++k) {
+      const size_t o = (size_t)k * ld + i;
+      const int j = colat(lcol, o, i);
+      y += au[(size_t)iau[o] * ld + j] * p[j];
     }
     q[i] = y;
     pq += (double)pi * (double)y;
@@ -414,6 +436,21 @@ __global__ __launch_bounds__(RB, 8) void k_mx_q(int n, int ws, int it, const I *
   if (i < r1) row(i);
   double v[1] = {pq};
   store_partials<1>(v, partQ);
+}
+
+// Upper entries of Ah: au[k][p] = a_pm / sqrt(a_pp a_mm) for the k-th upper
+// neighbour m of row p (zero on padding; slot WX stays zero)
+__global__ __launch_bounds__(BS) void k_mx_au(int n, int WU, const int *__restrict__ ufi,
+                                              const int *__restrict__ fi_face, const int *__restrict__ fi_n,
+                                              const double *__restrict__ ngp, const double *__restrict__ diag,
+                                              float *__restrict__ au) {
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i >= n) return;
+  const double wi = 1.0 / sqrt(diag[i]);
+  for (int k = 0; k < WU; ++k) {
+    const int q = ufi[(size_t)k * n + i];
+    au[(size_t)k * mx_ld(n) + i] = q >= 0 ? (float)(ngp[fi_face[q]] * wi / sqrt(diag[fi_n[q]])) : 0.f;
+  }
 }
 
 // The partials of p.q, issued before an update kernel's row loads
@@ -690,9 +727,17 @@ int mix_build(Ctx &c) {
   for (int i = 0; i < n; ++i) { WL = std::max(WL, nl[i]); WU = std::max(WU, nu[i]); }
   WL = std::max(WL, 1);
   WU = std::max(WU, 1);
-  // factor columns at the padded stride (self on padding), face ids at n
+  // Factor columns at the padded stride (self on padding), face ids at n.
+  // Both lists get the same width WX = max(WL, WU) so that one kernel
+  // template covers the symmetric SpMV; iau: for a lower slot of row m
+  // (face owner p), the slot of m in p's upper list (WX, a zero slot of
+  // au, on padding).
   const size_t ld = mx_ld(n);
+  const int WX = std::max(WL, WU);
+  WL = WU = WX;
   std::vector<int> lc((size_t)WL * ld), uc((size_t)WU * ld), lfi((size_t)WL * n, -1), ufi((size_t)WU * n, -1);
+  std::vector<unsigned char> iau((size_t)WL * ld, (unsigned char)WX);
+  if (WX > 254) { set_error("too many neighbours per row for the inner solve"); return FVK_ERR_ARG; }
   for (size_t o = 0; o < lc.size(); ++o) lc[o] = (int)(o % ld);
   for (size_t o = 0; o < uc.size(); ++o) uc[o] = (int)(o % ld);
   std::fill(nl.begin(), nl.end(), 0);
@@ -702,6 +747,7 @@ int mix_build(Ctx &c) {
     const int ku = nu[p]++, kl = nl[m]++;
     uc[(size_t)ku * ld + p] = m; ufi[(size_t)ku * n + p] = q;
     lc[(size_t)kl * ld + m] = p; lfi[(size_t)kl * n + m] = q;
+    iau[(size_t)kl * ld + m] = (unsigned char)ku;
   }
   // A's columns at the padded stride, self on padding
   std::vector<int> ac((size_t)Ws * ld);
@@ -749,6 +795,11 @@ int mix_build(Ctx &c) {
   if ((rc = up_int(fi_p, &X.fi_p))) return rc;
   if ((rc = up_int(fi_n, &X.fi_n))) return rc;
   if ((rc = dalloc(c, &X.spv, (size_t)Ws * ld))) return rc;
+  if ((rc = dalloc(c, &X.au, (size_t)(WX + 1) * ld))) return rc;
+  unsigned char *iau_d;
+  if ((rc = dalloc(c, &iau_d, iau.size()))) return rc;
+  FVK_HIP(hipMemcpy(iau_d, iau.data(), iau.size(), hipMemcpyHostToDevice));
+  X.iau = iau_d;
   lval_t *lv, *uv;
   if ((rc = dalloc(c, &lv, (size_t)WL * ld))) return rc;
   if ((rc = dalloc(c, &uv, (size_t)WU * ld))) return rc;
@@ -848,7 +899,7 @@ static int inner_solve(Ctx &c, bool fac, int n_iter, double reduc, InnerCarry &c
     } else {
       // factor: p in place in P[0]; r double-buffered
       float *p = P[0], *rin = r[it & 1], *rout = r[(it + 1) & 1];
-      const float *sp = X.spv, *pc = p;
+      const float *pc = p;
       if (it == 0) {
 #define L(W) k_mx_zp<W, I, float><<<G, RB, 0, c.stream>>>(n, wl, it, lcol, lval, t0, p, ev, pU, G, c.ksr, c.ks)
         FVK_W(wl, L);
@@ -858,15 +909,17 @@ static int inner_solve(Ctx &c, bool fac, int n_iter, double reduc, InnerCarry &c
         FVK_W(wl, L);
 #undef L
       }
+      const float *au = X.au;
+      const unsigned char *iau = X.iau;
       if (stamp) {
-#define L(W)                                                                                                     \
-  hipExtLaunchKernelGGL(k_mx_q<W, I>, dim3(G), dim3(RB), 0, c.stream, ea, eb, 0, n, ws, it, col, sp, pc, q, pQ, \
-                        (const KS *)c.ksr)
-        FVK_W(ws, L);
+#define L(W)                                                                                                      \
+  hipExtLaunchKernelGGL(k_mx_q<W, I>, dim3(G), dim3(RB), 0, c.stream, ea, eb, 0, n, wl, it, ucol, lcol, au, iau, pc, \
+                        q, pQ, (const KS *)c.ksr)
+        FVK_W(wl, L);
 #undef L
       } else {
-#define L(W) k_mx_q<W, I><<<G, RB, 0, c.stream>>>(n, ws, it, col, sp, pc, q, pQ, c.ksr)
-        FVK_W(ws, L);
+#define L(W) k_mx_q<W, I><<<G, RB, 0, c.stream>>>(n, wl, it, ucol, lcol, au, iau, pc, q, pQ, c.ksr)
+        FVK_W(wl, L);
 #undef L
       }
 #define L(W)                                                                                                        \
@@ -923,6 +976,8 @@ int dc_solve(Ctx &c, bool use_factor, const double *b, double *x, double tol, in
   }
   const int n = c.n, G = red_grid(n);
   k_mx_conv<<<grid_for(n), BS, 0, c.stream>>>(c.M, c.diag, c.spv, X.spv, X.w);
+  if (use_factor)
+    k_mx_au<<<grid_for(n), BS, 0, c.stream>>>(n, X.WU, X.ufi, X.fi_face, X.fi_n, c.ngp, c.diag, X.au);
   FVK_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, c.stream));
   res = DcResult();
   InnerCarry carry;
