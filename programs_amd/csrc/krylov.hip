@@ -129,7 +129,7 @@ __global__ __launch_bounds__(BS) void k_spmv(DMesh M, int nvc, const double *__r
 // and z = c r (mode 1, Jacobi PCG), s2 = r.r (mode 2, BiCGStab) or 0.
 // (solv_res.f90; solv_cg.f90:19-33; solv_bicg.f90:19-38)
 template <int WS>
-__global__ __launch_bounds__(RB) void k_residual(DMesh M, int nvc, const double *__restrict__ diag,
+__global__ __launch_bounds__(RB64) void k_residual(DMesh M, int nvc, const double *__restrict__ diag,
                                                  const double *__restrict__ spv, const double *__restrict__ x,
                                                  double *__restrict__ r, int not_r0, int mode,
                                                  const double *__restrict__ c, double *__restrict__ z,
@@ -137,8 +137,8 @@ __global__ __launch_bounds__(RB) void k_residual(DMesh M, int nvc, const double 
   const int n = M.n;
   double a[6] = {0, 0, 0, 0, 0, 0};
   int r0, r1;
-  block_rows(n, r0, r1);
-  for (int i = r0 + threadIdx.x; i < r1; i += RB)
+  block_rows<RB64>(n, r0, r1);
+  for (int i = r0 + threadIdx.x; i < r1; i += RB64)
     for (int q = 0; q < nvc; ++q) {
       double v = r[q * n + i];
       if (!not_r0) {
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(RB) void k_residual(DMesh M, int nvc, const double 
         a[3 + q] += v * v;
       }
     }
-  store_partials<6>(a, part);
+  store_partials<6, RB64>(a, part);
 }
 
 // Decide iteration it, then p = z + beta p ; q = A p ; partial p.q.  p of the
@@ -188,7 +188,7 @@ __device__ __forceinline__ void cg_pq_row(const DMesh &M, int i, const double *_
 }
 
 template <int NVC, int WS>
-__global__ __launch_bounds__(RB) void k_cg_pq(DMesh M, int it, const double *__restrict__ diag,
+__global__ __launch_bounds__(RB64) void k_cg_pq(DMesh M, int it, const double *__restrict__ diag,
                                               const double *__restrict__ spv, const double *__restrict__ z,
                                               const double *__restrict__ pold, double *__restrict__ pnew,
                                               double *__restrict__ qv, const double *__restrict__ partU, int PU,
@@ -197,13 +197,13 @@ __global__ __launch_bounds__(RB) void k_cg_pq(DMesh M, int it, const double *__r
   // |r|_1 (components 0..NVC) and, for Jacobi, r.z (components 3..3+NVC)
   double L1[NVC], rz[NVC];
   if (partR) {
-    load_totals<NVC>(partU, PU, L1);
-    load_totals<NVC>(partR, PR, rz);
+    load_totals<NVC, RB64>(partU, PU, L1);
+    load_totals<NVC, RB64>(partR, PR, rz);
   } else {
     int cs[2 * NVC];
     double t[2 * NVC];
     for (int q = 0; q < NVC; ++q) { cs[q] = q; cs[NVC + q] = 3 + q; }
-    load_sel<2 * NVC>(partU, PU, cs, t);
+    load_sel<2 * NVC, RB64>(partU, PU, cs, t);
     for (int q = 0; q < NVC; ++q) { L1[q] = t[q]; rz[q] = t[NVC + q]; }
   }
   __shared__ KS sk[3];
@@ -226,16 +226,16 @@ __global__ __launch_bounds__(RB) void k_cg_pq(DMesh M, int it, const double *__r
   double pq[NVC];
   for (int q = 0; q < NVC; ++q) pq[q] = 0.0;
   int r0, r1;
-  block_rows(n, r0, r1);
+  block_rows<RB64>(n, r0, r1);
   int i = r0 + threadIdx.x;
-  for (; i + RB < r1; i += 2 * RB)
+  for (; i + RB64 < r1; i += 2 * RB64)
 #pragma unroll
     for (int q = 0; q < NVC; ++q) {
       if (dn[q]) continue;
       const double *zq = z + q * n, *po = pold + q * n;
       double *pn = pnew + q * n, *qq = qv + q * n;
       cg_pq_row<NVC, WS>(M, i, diag, spv, zq, po, beta[q], pn, qq, pq[q]);
-      cg_pq_row<NVC, WS>(M, i + RB, diag, spv, zq, po, beta[q], pn, qq, pq[q]);
+      cg_pq_row<NVC, WS>(M, i + RB64, diag, spv, zq, po, beta[q], pn, qq, pq[q]);
     }
   if (i < r1)
 #pragma unroll
@@ -243,13 +243,13 @@ __global__ __launch_bounds__(RB) void k_cg_pq(DMesh M, int it, const double *__r
       if (dn[q]) continue;
       cg_pq_row<NVC, WS>(M, i, diag, spv, z + q * n, pold + q * n, beta[q], pnew + q * n, qv + q * n, pq[q]);
     }
-  store_partials<NVC>(pq, partQ);
+  store_partials<NVC, RB64>(pq, partQ);
 }
 
 // alpha = s/(p.q + small) ; x += alpha p ; r -= alpha q ; partials |r|_1 and
 // (Jacobi) r.z with z = c r.
 template <int NVC>
-__global__ __launch_bounds__(RB) void k_cg_upd(int n, int it, double *__restrict__ x, double *__restrict__ r,
+__global__ __launch_bounds__(RB64) void k_cg_upd(int n, int it, double *__restrict__ x, double *__restrict__ r,
                                                const double *__restrict__ p, const double *__restrict__ qv,
                                                const double *__restrict__ c, int jacobi, double *__restrict__ z,
                                                const double *__restrict__ partQ, int PQ, double *__restrict__ partU,
@@ -259,7 +259,7 @@ __global__ __launch_bounds__(RB) void k_cg_upd(int n, int it, double *__restrict
   for (int q = 0; q < NVC; ++q) any = any || !cur[q].done;
   if (!any) return;
   double t[NVC];
-  load_totals<NVC>(partQ, PQ, t);
+  load_totals<NVC, RB64>(partQ, PQ, t);
   double alpha[NVC];
   int dn[NVC];
   for (int q = 0; q < NVC; ++q) {
@@ -268,8 +268,8 @@ __global__ __launch_bounds__(RB) void k_cg_upd(int n, int it, double *__restrict
   }
   double a[6] = {0, 0, 0, 0, 0, 0};
   int r0, r1;
-  block_rows(n, r0, r1);
-  for (int i = r0 + threadIdx.x; i < r1; i += RB)
+  block_rows<RB64>(n, r0, r1);
+  for (int i = r0 + threadIdx.x; i < r1; i += RB64)
 #pragma unroll
     for (int q = 0; q < NVC; ++q) {
       if (dn[q]) continue;
@@ -284,33 +284,33 @@ __global__ __launch_bounds__(RB) void k_cg_upd(int n, int it, double *__restrict
         a[3 + q] += ri * zi;
       }
     }
-  store_partials<6>(a, partU);
+  store_partials<6, RB64>(a, partU);
   if (blockIdx.x == 0 && threadIdx.x == 0)
     for (int q = 0; q < NVC; ++q) ring[(it & 1) * 3 + q].alpha = alpha[q];
 }
 
 // r.z after the sweeps (DIC / DILU)
-__global__ __launch_bounds__(RB) void k_dot(int n, int nvc, const double *__restrict__ a,
+__global__ __launch_bounds__(RB64) void k_dot(int n, int nvc, const double *__restrict__ a,
                                             const double *__restrict__ b, double *__restrict__ part, const KS *gate) {
   double s[3] = {0, 0, 0};
   if (!all_done(gate, nvc)) {
     int r0, r1;
-    block_rows(n, r0, r1);
-    for (int i = r0 + threadIdx.x; i < r1; i += RB)
+    block_rows<RB64>(n, r0, r1);
+    for (int i = r0 + threadIdx.x; i < r1; i += RB64)
       for (int q = 0; q < nvc; ++q) s[q] += a[q * n + i] * b[q * n + i];
   }
-  store_partials<3>(s, part);
+  store_partials<3, RB64>(s, part);
 }
 
 // Final decision after the last queued iteration (1 block).
-__global__ __launch_bounds__(RB) void k_cg_final(int it, int nvc, const double *partU, int PU, const double *partR,
+__global__ __launch_bounds__(RB64) void k_cg_final(int it, int nvc, const double *partU, int PU, const double *partR,
                                                  int PR, KS *ring, KS *out) {
   double t[6];
-  load_totals<6>(partU, PU, t);
+  load_totals<6, RB64>(partU, PU, t);
   double rz[3] = {t[3], t[4], t[5]};
   if (partR) {
     double tr[3];
-    load_totals<3>(partR, PR, tr);
+    load_totals<3, RB64>(partR, PR, tr);
     rz[0] = tr[0]; rz[1] = tr[1]; rz[2] = tr[2];
   }
   if (threadIdx.x == 0) {
@@ -351,12 +351,12 @@ __device__ __forceinline__ void bi_decide(int it, int nvc, const double *L1, con
 }
 
 // Decide iteration it ; p = r + w6 (p - w2 u) ; Jacobi: z = c p
-__global__ __launch_bounds__(RB) void k_bi_p(int n, int it, int nvc, const double *__restrict__ r,
+__global__ __launch_bounds__(RB64) void k_bi_p(int n, int it, int nvc, const double *__restrict__ r,
                                              double *__restrict__ p, const double *__restrict__ u,
                                              const double *__restrict__ c, int jacobi, double *__restrict__ z,
                                              const double *__restrict__ partU, int PU, KS *ring, KS *out) {
   double t[6];
-  load_totals<6>(partU, PU, t);
+  load_totals<6, RB64>(partU, PU, t);
   __shared__ KS sk[3];
   if (threadIdx.x == 0) {
     bi_decide(it, nvc, t, t + 3, ring + ((it + 1) & 1) * 3, sk);
@@ -372,8 +372,8 @@ __global__ __launch_bounds__(RB) void k_bi_p(int n, int it, int nvc, const doubl
     dn[q] = sk[q].done;
   }
   int r0, r1;
-  block_rows(n, r0, r1);
-  for (int i = r0 + threadIdx.x; i < r1; i += RB)
+  block_rows<RB64>(n, r0, r1);
+  for (int i = r0 + threadIdx.x; i < r1; i += RB64)
     for (int q = 0; q < nvc; ++q) {
       if (dn[q]) continue;
       const int o = q * n + i;
@@ -385,7 +385,7 @@ __global__ __launch_bounds__(RB) void k_bi_p(int n, int it, int nvc, const doubl
 
 // y = A z with partials y.other (and y.y when mode = 1)
 template <int WS>
-__global__ __launch_bounds__(RB) void k_bi_spmv(DMesh M, int it, int nvc, int mode, const double *__restrict__ diag,
+__global__ __launch_bounds__(RB64) void k_bi_spmv(DMesh M, int it, int nvc, int mode, const double *__restrict__ diag,
                                                 const double *__restrict__ spv, const double *__restrict__ z,
                                                 double *__restrict__ y, const double *__restrict__ other,
                                                 double *__restrict__ part, const KS *ring) {
@@ -394,8 +394,8 @@ __global__ __launch_bounds__(RB) void k_bi_spmv(DMesh M, int it, int nvc, int mo
   if (!all_done(cur, nvc)) {
     const int n = M.n;
     int r0, r1;
-    block_rows(n, r0, r1);
-    for (int i = r0 + threadIdx.x; i < r1; i += RB)
+    block_rows<RB64>(n, r0, r1);
+    for (int i = r0 + threadIdx.x; i < r1; i += RB64)
       for (int q = 0; q < nvc; ++q) {
         if (cur[q].done) continue;
         const double yi = row_spmv<WS>(M, diag, spv, z + q * n, i);
@@ -404,18 +404,18 @@ __global__ __launch_bounds__(RB) void k_bi_spmv(DMesh M, int it, int nvc, int mo
         if (mode) a[3 + q] += yi * yi;
       }
   }
-  store_partials<6>(a, part);
+  store_partials<6, RB64>(a, part);
 }
 
 // gam = beta/(u.r0 + small) ; x += gam z ; r -= gam u ; Jacobi: z = c r
-__global__ __launch_bounds__(RB) void k_bi_upd1(int n, int it, int nvc, double *__restrict__ x,
+__global__ __launch_bounds__(RB64) void k_bi_upd1(int n, int it, int nvc, double *__restrict__ x,
                                                 double *__restrict__ r, double *__restrict__ z,
                                                 const double *__restrict__ u, const double *__restrict__ c, int jacobi,
                                                 const double *__restrict__ partB, int PB, KS *ring) {
   KS *cur = ring + (it & 1) * 3;
   if (all_done(cur, nvc)) return;
   double t[6];
-  load_totals<6>(partB, PB, t);
+  load_totals<6, RB64>(partB, PB, t);
   double w5[3];
   int dn[3];
   for (int q = 0; q < nvc; ++q) {
@@ -429,8 +429,8 @@ __global__ __launch_bounds__(RB) void k_bi_upd1(int n, int it, int nvc, double *
     for (int q = 0; q < nvc; ++q)
       if (!dn[q]) cur[q].w5 = w5[q];
   int r0, r1;
-  block_rows(n, r0, r1);
-  for (int i = r0 + threadIdx.x; i < r1; i += RB)
+  block_rows<RB64>(n, r0, r1);
+  for (int i = r0 + threadIdx.x; i < r1; i += RB64)
     for (int q = 0; q < nvc; ++q) {
       if (dn[q]) continue;
       const int o = q * n + i;
@@ -442,7 +442,7 @@ __global__ __launch_bounds__(RB) void k_bi_upd1(int n, int it, int nvc, double *
 }
 
 // omega = (v.r)/(v.v + small) ; x += omega z ; r -= omega v ; partials |r|_1, r.r0
-__global__ __launch_bounds__(RB) void k_bi_upd2(int n, int it, int nvc, double *__restrict__ x,
+__global__ __launch_bounds__(RB64) void k_bi_upd2(int n, int it, int nvc, double *__restrict__ x,
                                                 double *__restrict__ r, const double *__restrict__ z,
                                                 const double *__restrict__ v, const double *__restrict__ r0v,
                                                 const double *__restrict__ partD, int PD, double *__restrict__ partU,
@@ -451,7 +451,7 @@ __global__ __launch_bounds__(RB) void k_bi_upd2(int n, int it, int nvc, double *
   double a[6] = {0, 0, 0, 0, 0, 0};
   if (!all_done(cur, nvc)) {
     double t[6];
-    load_totals<6>(partD, PD, t);
+    load_totals<6, RB64>(partD, PD, t);
     double w1[3], w2[3];
     int dn[3];
     for (int q = 0; q < nvc; ++q) {
@@ -464,8 +464,8 @@ __global__ __launch_bounds__(RB) void k_bi_upd2(int n, int it, int nvc, double *
       for (int q = 0; q < nvc; ++q)
         if (!dn[q]) { cur[q].w1 = w1[q]; cur[q].w2 = w2[q]; }
     int r0, r1;
-    block_rows(n, r0, r1);
-    for (int i = r0 + threadIdx.x; i < r1; i += RB)
+    block_rows<RB64>(n, r0, r1);
+    for (int i = r0 + threadIdx.x; i < r1; i += RB64)
       for (int q = 0; q < nvc; ++q) {
         if (dn[q]) continue;
         const int o = q * n + i;
@@ -476,12 +476,12 @@ __global__ __launch_bounds__(RB) void k_bi_upd2(int n, int it, int nvc, double *
         a[3 + q] += ri * r0v[o];
       }
   }
-  store_partials<6>(a, partU);
+  store_partials<6, RB64>(a, partU);
 }
 
-__global__ __launch_bounds__(RB) void k_bi_final(int it, int nvc, const double *partU, int PU, KS *ring, KS *out) {
+__global__ __launch_bounds__(RB64) void k_bi_final(int it, int nvc, const double *partU, int PU, KS *ring, KS *out) {
   double t[6];
-  load_totals<6>(partU, PU, t);
+  load_totals<6, RB64>(partU, PU, t);
   if (threadIdx.x == 0) {
     KS cur[3];
     bi_decide(it, nvc, t, t + 3, ring + ((it + 1) & 1) * 3, cur);
@@ -490,34 +490,34 @@ __global__ __launch_bounds__(RB) void k_bi_final(int it, int nvc, const double *
 }
 
 // res0_n only (fvk_solv_res)
-__global__ __launch_bounds__(RB) void k_res_final(int nvc, const double *partU, int PU, KS *out) {
+__global__ __launch_bounds__(RB64) void k_res_final(int nvc, const double *partU, int PU, KS *out) {
   double t[6];
-  load_totals<6>(partU, PU, t);
+  load_totals<6, RB64>(partU, PU, t);
   if (threadIdx.x == 0)
     for (int q = 0; q < nvc; ++q) out[q].res0_n = t[q];
 }
 
 // FP64 true residual {|b-Ax|_1, |b-Ax|_2^2, |b|_1, |b|_2^2} as partials
 template <int WS>
-__global__ __launch_bounds__(RB) void k_true_res(DMesh M, const double *__restrict__ diag,
+__global__ __launch_bounds__(RB64) void k_true_res(DMesh M, const double *__restrict__ diag,
                                                  const double *__restrict__ spv, const double *__restrict__ x,
                                                  const double *__restrict__ b, double *__restrict__ part) {
   double a[4] = {0, 0, 0, 0};
   int r0, r1;
-  block_rows(M.n, r0, r1);
-  for (int i = r0 + threadIdx.x; i < r1; i += RB) {
+  block_rows<RB64>(M.n, r0, r1);
+  for (int i = r0 + threadIdx.x; i < r1; i += RB64) {
     const double bi = b[i], ri = bi - row_spmv<WS>(M, diag, spv, x, i);
     a[0] += fabs(ri);
     a[1] += ri * ri;
     a[2] += fabs(bi);
     a[3] += bi * bi;
   }
-  store_partials<4>(a, part);
+  store_partials<4, RB64>(a, part);
 }
 
-__global__ __launch_bounds__(RB) void k_totals4(const double *part, int P, double *out) {
+__global__ __launch_bounds__(RB64) void k_totals4(const double *part, int P, double *out) {
   double s[4];
-  load_totals<4>(part, P, s);
+  load_totals<4, RB64>(part, P, s);
   if (threadIdx.x == 0)
     for (int j = 0; j < 4; ++j) out[j] = s[j];
 }
@@ -631,12 +631,12 @@ static int ks_init(Ctx &c, int nvc, double reduc, int n_iter, bool bicg) {
 }
 
 int residual(Ctx &c, int nvc, const double *x, double *r, bool not_r0) {
-  const int G = red_grid(c.n);
+  const int G = red_grid64(c.n);
 #define L(W) \
-  k_residual<W><<<G, RB, 0, c.stream>>>(c.M, nvc, c.diag, c.spv, x, r, not_r0 ? 1 : 0, 0, nullptr, nullptr, pbuf(c, 0))
+  k_residual<W><<<G, RB64, 0, c.stream>>>(c.M, nvc, c.diag, c.spv, x, r, not_r0 ? 1 : 0, 0, nullptr, nullptr, pbuf(c, 0))
   FVK_WS(c.M.Ws, L);
 #undef L
-  k_res_final<<<1, RB, 0, c.stream>>>(nvc, pbuf(c, 0), G, c.ks);
+  k_res_final<<<1, RB64, 0, c.stream>>>(nvc, pbuf(c, 0), G, c.ks);
   FVK_HIP(hipGetLastError());
   return 0;
 }
@@ -657,28 +657,28 @@ static void launch_cg_pq(Ctx &c, int G, int it, double *z, double *pold, double 
   if (T && T->take(a, b)) {
     const double *zc = z, *pc = pold;
 #define L(W)                                                                                                   \
-  hipExtLaunchKernelGGL(k_cg_pq<NVC, W>, dim3(G), dim3(RB), 0, c.stream, a, b, 0, c.M, it, (const double *)c.diag, \
+  hipExtLaunchKernelGGL(k_cg_pq<NVC, W>, dim3(G), dim3(RB64), 0, c.stream, a, b, 0, c.M, it, (const double *)c.diag, \
                         (const double *)c.spv, zc, pc, pnew, q, pU, G, pR, G, pbuf(c, 1), c.ksr, c.ks)
     FVK_WS(c.M.Ws, L);
 #undef L
     return;
   }
 #define L(W)                                                                                                    \
-  k_cg_pq<NVC, W><<<G, RB, 0, c.stream>>>(c.M, it, c.diag, c.spv, z, pold, pnew, q, pU, G, pR, G, pbuf(c, 1), \
+  k_cg_pq<NVC, W><<<G, RB64, 0, c.stream>>>(c.M, it, c.diag, c.spv, z, pold, pnew, q, pU, G, pR, G, pbuf(c, 1), \
                                           c.ksr, c.ks)
   FVK_WS(c.M.Ws, L);
 #undef L
 }
 template <int NVC>
 static void launch_cg_upd(Ctx &c, int G, int it, double *x, double *r, double *p, double *q, int jac, double *z) {
-  k_cg_upd<NVC><<<G, RB, 0, c.stream>>>(c.n, it, x, r, p, q, c.pc_c, jac, z, pbuf(c, 1), G, pbuf(c, 0), c.ksr);
+  k_cg_upd<NVC><<<G, RB64, 0, c.stream>>>(c.n, it, x, r, p, q, c.pc_c, jac, z, pbuf(c, 1), G, pbuf(c, 0), c.ksr);
 }
 
 // PCG: r0 holds b on entry, b - A x0 (the initial residual) on exit; x is
 // updated.  chunk > 0: stop queueing iterations once converged.
 static int cg_run(Ctx &c, int imat, double reduc, int n_iter, int nvc, double *x, double *r0, bool not_r0, int chunk,
                   KTimer *T) {
-  const int n = c.n, G = red_grid(n);
+  const int n = c.n, G = red_grid64(n);
   double *r = wv(c, 0), *z = wv(c, 1), *P = wv(c, 2), *q = wv(c, 4);
   const int jac = imat == 0;
   if (!c.pc_valid || c.pc_imat != imat) {
@@ -688,7 +688,7 @@ static int cg_run(Ctx &c, int imat, double reduc, int n_iter, int nvc, double *x
   // both direction buffers (slots 2 and 3)
   if (fill(c, P, wk_ld(n) + (size_t)3 * n, 0.0)) return FVK_ERR_HIP;
 #define L(W)                                                                                                \
-  k_residual<W><<<G, RB, 0, c.stream>>>(c.M, nvc, c.diag, c.spv, x, r0, not_r0 ? 1 : 0, jac ? 1 : 0, c.pc_c, z, \
+  k_residual<W><<<G, RB64, 0, c.stream>>>(c.M, nvc, c.diag, c.spv, x, r0, not_r0 ? 1 : 0, jac ? 1 : 0, c.pc_c, z, \
                                         pbuf(c, 0))
   FVK_WS(c.M.Ws, L);
 #undef L
@@ -697,7 +697,7 @@ static int cg_run(Ctx &c, int imat, double reduc, int n_iter, int nvc, double *x
   if (!jac) {
     int rc = precond_apply(c, imat, nvc, r, z, nullptr);
     if (rc) return rc;
-    k_dot<<<G, RB, 0, c.stream>>>(n, nvc, r, z, pbuf(c, 2), nullptr);
+    k_dot<<<G, RB64, 0, c.stream>>>(n, nvc, r, z, pbuf(c, 2), nullptr);
     pR = pbuf(c, 2);
   }
   Poller Pl(c);
@@ -713,7 +713,7 @@ static int cg_run(Ctx &c, int imat, double reduc, int n_iter, int nvc, double *x
     if (!jac) {
       int rc = precond_apply(c, imat, nvc, r, z, c.ksr + (it & 1) * 3);
       if (rc) return rc;
-      k_dot<<<G, RB, 0, c.stream>>>(n, nvc, r, z, pbuf(c, 2), c.ksr + (it & 1) * 3);
+      k_dot<<<G, RB64, 0, c.stream>>>(n, nvc, r, z, pbuf(c, 2), c.ksr + (it & 1) * 3);
     }
     if (chunk > 0 && it + 1 == next_poll && it + 1 < n_iter) {
       bool done;
@@ -729,14 +729,14 @@ static int cg_run(Ctx &c, int imat, double reduc, int n_iter, int nvc, double *x
       next_poll = it + 1 + next;
     }
   }
-  k_cg_final<<<1, RB, 0, c.stream>>>(it, nvc, pbuf(c, 0), G, pR, G, c.ksr, c.ks);
+  k_cg_final<<<1, RB64, 0, c.stream>>>(it, nvc, pbuf(c, 0), G, pR, G, c.ksr, c.ks);
   FVK_HIP(hipGetLastError());
   if (T && T->flush()) return FVK_ERR_HIP;
   return 0;
 }
 
 static int bicg_run(Ctx &c, int imat, int n_iter, int nvc, double *x, double *r0, bool not_r0) {
-  const int n = c.n, G = red_grid(n);
+  const int n = c.n, G = red_grid64(n);
   double *r = wv(c, 0), *p = wv(c, 1), *u = wv(c, 2), *v = wv(c, 3), *z = wv(c, 4);
   const int jac = imat == 0;
   if (!c.pc_valid || c.pc_imat != imat) {
@@ -746,7 +746,7 @@ static int bicg_run(Ctx &c, int imat, int n_iter, int nvc, double *x, double *r0
   // slots 1-4
   if (fill(c, p, (size_t)3 * wk_ld(n) + (size_t)3 * n, 0.0)) return FVK_ERR_HIP;
 #define L(W)                                                                                              \
-  k_residual<W><<<G, RB, 0, c.stream>>>(c.M, nvc, c.diag, c.spv, x, r0, not_r0 ? 1 : 0, 2, nullptr, nullptr, \
+  k_residual<W><<<G, RB64, 0, c.stream>>>(c.M, nvc, c.diag, c.spv, x, r0, not_r0 ? 1 : 0, 2, nullptr, nullptr, \
                                         pbuf(c, 0))
   FVK_WS(c.M.Ws, L);
 #undef L
@@ -754,19 +754,19 @@ static int bicg_run(Ctx &c, int imat, int n_iter, int nvc, double *x, double *r0
   int it = 0;
   for (; it < n_iter; ++it) {
     const KS *gate = c.ksr + (it & 1) * 3;
-    k_bi_p<<<G, RB, 0, c.stream>>>(n, it, nvc, r, p, u, c.pc_c, jac, z, pbuf(c, 0), G, c.ksr, c.ks);
+    k_bi_p<<<G, RB64, 0, c.stream>>>(n, it, nvc, r, p, u, c.pc_c, jac, z, pbuf(c, 0), G, c.ksr, c.ks);
     if (!jac && precond_apply(c, imat, nvc, p, z, gate)) return FVK_ERR_HIP;
-#define L(W) k_bi_spmv<W><<<G, RB, 0, c.stream>>>(c.M, it, nvc, 0, c.diag, c.spv, z, u, r0, pbuf(c, 1), c.ksr)
+#define L(W) k_bi_spmv<W><<<G, RB64, 0, c.stream>>>(c.M, it, nvc, 0, c.diag, c.spv, z, u, r0, pbuf(c, 1), c.ksr)
     FVK_WS(c.M.Ws, L);
 #undef L
-    k_bi_upd1<<<G, RB, 0, c.stream>>>(n, it, nvc, x, r, z, u, c.pc_c, jac, pbuf(c, 1), G, c.ksr);
+    k_bi_upd1<<<G, RB64, 0, c.stream>>>(n, it, nvc, x, r, z, u, c.pc_c, jac, pbuf(c, 1), G, c.ksr);
     if (!jac && precond_apply(c, imat, nvc, r, z, gate)) return FVK_ERR_HIP;
-#define L(W) k_bi_spmv<W><<<G, RB, 0, c.stream>>>(c.M, it, nvc, 1, c.diag, c.spv, z, v, r, pbuf(c, 2), c.ksr)
+#define L(W) k_bi_spmv<W><<<G, RB64, 0, c.stream>>>(c.M, it, nvc, 1, c.diag, c.spv, z, v, r, pbuf(c, 2), c.ksr)
     FVK_WS(c.M.Ws, L);
 #undef L
-    k_bi_upd2<<<G, RB, 0, c.stream>>>(n, it, nvc, x, r, z, v, r0, pbuf(c, 2), G, pbuf(c, 0), c.ksr);
+    k_bi_upd2<<<G, RB64, 0, c.stream>>>(n, it, nvc, x, r, z, v, r0, pbuf(c, 2), G, pbuf(c, 0), c.ksr);
   }
-  k_bi_final<<<1, RB, 0, c.stream>>>(it, nvc, pbuf(c, 0), G, c.ksr, c.ks);
+  k_bi_final<<<1, RB64, 0, c.stream>>>(it, nvc, pbuf(c, 0), G, c.ksr, c.ks);
   FVK_HIP(hipGetLastError());
   return 0;
 }
@@ -807,11 +807,11 @@ int pcg_tol(Ctx &c, int imat, const double *b, double *x, double tol, int max_it
 }
 
 int true_residual(Ctx &c, const double *x, const double *b, double *dev_out4, double *host_out4) {
-  const int G = red_grid(c.n);
-#define L(W) k_true_res<W><<<G, RB, 0, c.stream>>>(c.M, c.diag, c.spv, x, b, pbuf(c, 3))
+  const int G = red_grid64(c.n);
+#define L(W) k_true_res<W><<<G, RB64, 0, c.stream>>>(c.M, c.diag, c.spv, x, b, pbuf(c, 3))
   FVK_WS(c.M.Ws, L);
 #undef L
-  k_totals4<<<1, RB, 0, c.stream>>>(pbuf(c, 3), G, dev_out4);
+  k_totals4<<<1, RB64, 0, c.stream>>>(pbuf(c, 3), G, dev_out4);
   FVK_HIP(hipGetLastError());
   FVK_HIP(hipMemcpyAsync(host_out4, dev_out4, 4 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   FVK_HIP(hipStreamSynchronize(c.stream));
