@@ -113,6 +113,49 @@ def test_dc32_factor_spike_rhs_converges_without_fallback():
     assert out["res_true_l1"] < 1e-8
 
 
+def test_dc32_shuffled_triangles_32bit_columns():
+    """A Delaunay channel of ~40,000 triangles in shuffled cell order, solved
+    in that order (no RCM): neighbours lie up to n rows apart, beyond 16-bit
+    column offsets, so the inner system takes 32-bit columns (seen in the
+    SpMV's byte count).  The Jacobi and the factor paths (zero factor and a
+    briefly trained network) reach the FP64 residual judged by the oracle's
+    SpMV, without fallback."""
+    from programs_amd import meshgen
+    fvk = require_lib()
+    vx, ty, vp, vr, bnd = meshgen.channel_delaunay(40000, seed=0x58)
+    nt = len(ty)
+    perm = np.random.default_rng(0x59).permutation(nt)
+    vr = vr.reshape(nt, 3)[perm].ravel()
+    bnd = bnd.reshape(nt, 3)[perm].ravel()
+    m = fvk.Mesh.from_cells(2, vx, ty, vp, vr, bnd)
+    ctx = fvk.grd_solv(m)
+    b = fvk.sys_random(ctx, 0x5A, 1.0)
+    pl, a, _ = fvk.sys_get(ctx)
+    vx3 = np.zeros((len(vx), 3))
+    vx3[:, :2] = vx
+    om = fvo.OMesh.raw(2, vx3, ty, vp, vr, bnd)
+    try:
+        A = fvo.OSys(om, pl, -a, -a)
+        nnz_off = 2 * int(np.sum(om.topology()["fce_in"] > 0))
+        out = fvk.sys_solve(ctx, fvk.SOLVE_DC32, 0, 1e-8)
+        assert out["status"] == fvk.FVK_OK and true_res(A, fvk.sys_x(ctx), b) < 1e-8
+        fvk.factor_set(ctx, np.zeros(ctx.n_fce, np.float32), np.zeros(ctx.n_fce, np.float32))
+        out = fvk.sys_solve(ctx, fvk.SOLVE_DC32_FACTOR, 0, 1e-8, time_kernel=True)
+        assert out["status"] == fvk.FVK_OK and out["fallback"] == 0
+        assert true_res(A, fvk.sys_x(ctx), b) < 1e-8
+        assert out["kern_bytes"] == 8.0 * ctx.n_vol + 6.5 * nnz_off  # 32-bit columns
+        model = fvk.Model(seed=11)
+        model.train(ctx, 20, lr=3e-3)
+        model.build_factor(ctx)
+        lrn = fvk.sys_solve(ctx, fvk.SOLVE_DC32_FACTOR, 0, 1e-8)
+        model.destroy()
+        print("shuffled triangles: jacobi-factor", out["iters"], "learned", lrn["iters"])
+        assert lrn["status"] == fvk.FVK_OK and lrn["fallback"] == 0
+        assert true_res(A, fvk.sys_x(ctx), b) < 1e-8
+    finally:
+        om.free()
+
+
 def test_factor_apply_matches_oracle():
     fvk, m, ctx, om, S, A, b = setup(12, 0.2, 0x52, rcm=True)
     try:
