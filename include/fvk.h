@@ -190,7 +190,7 @@ int fvk_sys_get_x(fvk_ctx *ctx, double *x);
 /* Solver settings of a context: "dc_inner_reduc" (the deepest relative L1
    reduction asked of an FP32 inner solve, default 1e-5; each inner solve
    aims at max(dc_inner_reduc, dc_eta_margin tol / current residual),
-   dc_eta_margin default 0.5), "dc_max_outer"
+   dc_eta_margin default 0.8), "dc_max_outer"
    (defect-correction steps, default 30), "dc_carry" (1: a correction step
    continues the previous step's search direction; 0: restart, default). */
 int fvk_ctx_set_param(fvk_ctx *ctx, const char *name, double value);

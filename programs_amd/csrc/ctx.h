@@ -69,7 +69,7 @@ struct Mix {
   double *tot = nullptr;               // 8 doubles: residual totals
   bool fac_valid = false;
   double inner_reduc = 1e-5;           // deepest relative L1 reduction of an inner solve
-  double eta_margin = 0.5;             // a later inner solve aims at eta_margin x what tol needs
+  double eta_margin = 0.8;             // a later inner solve aims at eta_margin x what tol needs
   int max_outer = 30;
   bool carry = false;                  // continue the Krylov recurrence across correction steps
 };
