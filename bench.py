@@ -244,7 +244,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is "
-              "described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+              "described with the design notes); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     from programs_amd import fvk
@@ -308,7 +308,7 @@ def main():
                 peak=PEAK_HBM / 1e9, unit="GB/s", frac=kb / ks / PEAK_HBM, traffic=traffic_bytes("k_cg_pq"),
                 bytes_per_launch=kb, us_per_launch=ks * 1e6, launches=rk["kern_launches"])
 
-    # stand-alone FP64 SpMV, BASELINE's CSR byte model nnz(w+4) + 4(n+1) + 2wn
+    # stand-alone FP64 SpMV, CSR byte model nnz(w+4) + 4(n+1) + 2wn
     t_spmv = fvk.bench_spmv(ctx, 200)
     nnz = mesh.n_vol + 2 * (mesh.n_fce - mesh.n_bf)
     b_spmv = nnz * 12 + 4 * (mesh.n_vol + 1) + 16 * mesh.n_vol

@@ -1,6 +1,6 @@
 """FP32-inner defect correction and the factorised preconditioner on the GPU.
 
-The judge of every solve is the FP64 true residual |b - A x|_1 / |b|_1,
+Every solve is decided by the FP64 true residual |b - A x|_1 / |b|_1,
 recomputed here on the CPU by the oracle's own SpMV (oracle/fvo, the
 reference's face-scatter product) from the solution the library returns.
 The factor's action z = G G^T r is compared with oracle/learn.apply on the
@@ -96,7 +96,7 @@ def test_dc32_factor_spike_rhs_converges_without_fallback():
     normalisation to mean |r| = 1 the spike is ~1e5, beyond FP16's range.
     The FP16 intermediate of the factor path is scaled by the RMS of
     Delta u, not by the mean, so the solve must converge without breaking
-    down (no fallback), judged by the FP64 true residual."""
+    down (no fallback), checked by the FP64 true residual."""
     fvk = require_lib()
     n = 48
     m = fvk.Mesh.box(n, n, n, jitter=0.2, seed=0x56)
@@ -118,7 +118,7 @@ def test_dc32_shuffled_triangles_32bit_columns():
     in that order (no RCM): neighbours lie up to n rows apart, beyond 16-bit
     column offsets, so the inner system takes 32-bit columns (seen in the
     SpMV's byte count).  The Jacobi and the factor paths (zero factor and a
-    briefly trained network) reach the FP64 residual judged by the oracle's
+    briefly trained network) reach the FP64 residual computed by the oracle's
     SpMV, without fallback."""
     from programs_amd import meshgen
     fvk = require_lib()
