@@ -222,3 +222,34 @@ def test_dc32_factor_falls_back_to_plain_pcg():
         assert true_res(A, x, b) < 1e-8
     finally:
         om.free()
+
+
+def test_zero_rhs_and_short_budget():
+    """b = 0 returns x = 0 at once for every method (solv_cg.f90:22 returns
+    when max res0_n < small); a budget below what the tolerance needs ends
+    with FVK_NOT_CONVERGED and a finite, partly converged x."""
+    fvk, m, ctx, om, S, A, b = setup(10, 0.2, 0x56)
+    try:
+        lf, sf = L.fsai_closed_form(S)
+        inner = om.topology()["fce_in"] > 0
+        l = np.zeros(ctx.n_fce, np.float32)
+        sig = np.zeros(ctx.n_fce, np.float32)
+        l[inner] = lf
+        sig[inner] = sf
+        fvk.factor_set(ctx, l, sig)
+        pl, a, _ = fvk.sys_get(ctx)
+        fvk.sys_set(ctx, pl, a, np.zeros_like(b))
+        for method in (fvk.SOLVE_PCG64, fvk.SOLVE_DC32, fvk.SOLVE_DC32_FACTOR):
+            out = fvk.sys_solve(ctx, method, 0, 1e-8)
+            assert out["status"] == fvk.FVK_OK and out["iters"] == 0 and out["fallback"] == 0, (method, out)
+            assert not np.any(fvk.sys_x(ctx)), method
+        fvk.sys_set(ctx, pl, a, b)
+        for method in (fvk.SOLVE_PCG64, fvk.SOLVE_DC32):
+            out = fvk.sys_solve(ctx, method, 0, 1e-8, max_iter=10)
+            x = fvk.sys_x(ctx)
+            print(method, out["iters"], out["res_true_l1"])
+            assert out["status"] == fvk.FVK_NOT_CONVERGED, (method, out)
+            assert out["iters"] <= 10 and np.all(np.isfinite(x))
+            assert 1e-8 < true_res(A, x, b) < 1.0
+    finally:
+        om.free()
