@@ -803,6 +803,8 @@ int fvk_ctx_set_param(fvk_ctx *cc, const char *name, double value) {
   else if (!std::strcmp(name, "dc_max_outer") && value >= 1 && value <= 1000) X.max_outer = (int)value;
   else if (!std::strcmp(name, "dc_carry") && (value == 0 || value == 1)) X.carry = value != 0;
   else if (!std::strcmp(name, "dc_eta_margin") && value > 0 && value <= 1) X.eta_margin = value;
+  else if (!std::strcmp(name, "dc_replace_every") && value >= 0 && value <= 1e6 && value == (int)value)
+    X.replace_every = (int)value;
   else { set_error(std::string("unknown parameter or value out of range: ") + name); return FVK_ERR_ARG; }
   return FVK_OK;
 }

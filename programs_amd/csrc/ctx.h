@@ -72,6 +72,7 @@ struct Mix {
   double eta_margin = 0.8;             // a later inner solve aims at eta_margin x what tol needs
   int max_outer = 30;
   bool carry = false;                  // continue the Krylov recurrence across correction steps
+  int replace_every = 160;             // >0: FP64 residual replacement every that many inner iterations
 };
 
 struct Ctx {

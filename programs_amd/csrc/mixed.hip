@@ -105,19 +105,21 @@ __global__ __launch_bounds__(BS) void k_mx_conv(DMesh M, const double *__restric
   }
 }
 
-// FP64 true residual r = b - A x, its scaled FP32 copy D^-1/2 r, partials
-// {|r|_1, |r|_2^2, |b|_1, |b|_2^2}
+// FP64 true residual r = b - A x, its scaled FP32 copy scale D^-1/2 r, partials
+// {|r|_1, |r|_2^2, |b|_1, |b|_2^2}.  gate (residual replacement inside an
+// inner solve): nothing once that solve is done or broke down.
 template <int WS>
 __global__ __launch_bounds__(RB) void k_mx_res(DMesh M, const double *__restrict__ diag,
                                                const double *__restrict__ spv, const double *__restrict__ x,
                                                const double *__restrict__ b, float *__restrict__ r32,
-                                               double *__restrict__ part) {
+                                               double *__restrict__ part, const KS *gate, double scale) {
+  if (gate && (gate->done || gate->pad)) return;
   double a[4] = {0, 0, 0, 0};
   int r0, r1;
   block_rows(M.n, r0, r1);
   for (int i = r0 + threadIdx.x; i < r1; i += RB) {
     const double bi = b[i], ri = bi - row_spmv<WS>(M, diag, spv, x, i);
-    r32[i] = (float)(ri / sqrt(diag[i]));
+    r32[i] = (float)(ri / sqrt(diag[i]) * scale);
     a[0] += fabs(ri);
     a[1] += ri * ri;
     a[2] += fabs(bi);
@@ -129,19 +131,24 @@ __global__ __launch_bounds__(RB) void k_mx_res(DMesh M, const double *__restrict
 // Start of an inner solve: e = 0, p_old = f p_last (p_last null: 0),
 // partials |r|_1 [0] and Jacobi: r.r [3]; factor: t = Delta^2 (r + L^T r).
 // p_last may be p0 itself (each row reads and writes its own element).
+// After a residual replacement (ev and p0 null: both kept) gate is the last
+// iteration's scalars: nothing once the solve is done or broke down, and the
+// last update's alpha, already applied to x, is cleared for the next
+// direction kernel.
 template <int WU, class I, class TT>
 __global__ __launch_bounds__(RB) void k_mx_start(int n, int fac, const float *__restrict__ r,
                                                  float *__restrict__ ev, float *p0, const float *p_last, float f,
                                                  int wu, const I *__restrict__ ucol,
                                                  const lval_t *__restrict__ uval, const lval_t *__restrict__ ee,
-                                                 TT *__restrict__ t, double *__restrict__ part) {
+                                                 TT *__restrict__ t, double *__restrict__ part, KS *gate) {
+  if (gate && (gate->done || gate->pad)) return;
   double a[4] = {0, 0, 0, 0};
   int r0, r1;
   block_rows(n, r0, r1);
   for (int i = r0 + threadIdx.x; i < r1; i += RB) {
     const float ri = r[i];
-    ev[i] = 0.f;
-    p0[i] = p_last ? f * p_last[i] : 0.f;
+    if (ev) ev[i] = 0.f;
+    if (p0) p0[i] = p_last ? f * p_last[i] : 0.f;
     a[0] += fabs((double)ri);
     if (!fac) {
       a[3] += (double)ri * (double)ri;
@@ -158,6 +165,7 @@ __global__ __launch_bounds__(RB) void k_mx_start(int n, int fac, const float *__
     }
   }
   store_partials<4>(a, part);
+  if (gate && blockIdx.x == 0 && threadIdx.x == 0) gate->alpha = 0.0;
 }
 
 // Decide iteration it of the inner PCG; p = z + beta p ; q = Ah p ; p.q.
@@ -326,8 +334,8 @@ __global__ __launch_bounds__(RB, 8) void k_mx_zp(int n, int wl, int it, const I 
   if (sk[0].done) return;
   const float bf = (float)sk[0].beta, af = (float)sk[0].alpha;
   // t was stored divided by the scale of the previous decision (FP32 and
-  // unscaled from the start of the solve)
-  const float ts = it == 0 ? 1.f : t_scale(ring[((it + 1) & 1) * 3], n);
+  // unscaled from the start of the solve or a residual replacement)
+  const float ts = std::is_same<TT, float>::value ? 1.f : t_scale(ring[((it + 1) & 1) * 3], n);
   int i = i0;
   if (WL > 0) {
     if (i0 < r1) {
@@ -665,6 +673,22 @@ __global__ __launch_bounds__(BS) void k_mx_add(int n, double *__restrict__ x, co
   x[i] += sigma * (double)e / sqrt(diag[i]);
 }
 
+// Residual replacement inside an inner solve, first step: x += sigma D^-1/2
+// (e + alpha p) with the last update's alpha (p null: none pending), e = 0;
+// nothing once the solve is done or broke down.  The FP64 residual of the new
+// x then replaces the recursive FP32 one (k_mx_res, k_mx_start), while the
+// search direction and r.z carry on.
+__global__ __launch_bounds__(BS) void k_mx_rep_x(int n, double *__restrict__ x, float *__restrict__ ev,
+                                                 const float *__restrict__ p, const double *__restrict__ diag,
+                                                 double sigma, const KS *prev) {
+  if (prev->done || prev->pad) return;
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i >= n) return;
+  const float e = p ? ev[i] + (float)prev->alpha * p[i] : ev[i];
+  x[i] += sigma * (double)e / sqrt(diag[i]);
+  ev[i] = 0.f;
+}
+
 constexpr double FAC_E2_MIN = -9.7, FAC_E2_MAX = 11.0;
 
 // Factor from per-interior-face (l, sig): e = exp(2 s) with s the sum of sig
@@ -840,14 +864,19 @@ struct InnerCarry {
   double s = 0.0;
 };
 
-// One inner PCG solve of Ah eh = rh (rh in V_R0).  carry in: the previous
-// solve's last direction and r.z, to be scaled by f (rows) and f^2 (r.z);
-// carry out: this solve's.
+// One inner PCG solve of Ah eh = rh (rh in V_R0, rh = D^-1/2 (b - A x) / sigma).
+// carry in: the previous solve's last direction and r.z, to be scaled by f
+// (rows) and f^2 (r.z); carry out: this solve's.
 // alpha_pend: the factor path's last alpha, whose e += alpha p (p in P[0])
 // is still to be applied.
+// Every replace_every iterations (0: never) the FP32 correction so far goes
+// into the FP64 solution x and the recursive residual is replaced by the
+// FP64 residual of that x (reliable updates in the manner of Sleijpen and
+// van der Vorst): the direction and r.z carry on, so the recurrence keeps
+// its Krylov space while the residual it works on stays the true one.
 template <class I>
 static int inner_solve(Ctx &c, bool fac, int n_iter, double reduc, InnerCarry &carry, float f, int &iters,
-                       int &broken, double &alpha_pend, KTimer *T) {
+                       int &broken, double &alpha_pend, KTimer *T, double *x, const double *b, double sigma) {
   Mix &X = c.mx;
   const int n = c.n, G = red_grid(n), ws = c.Ws, wl = X.WL, wu = X.WU;
   const I *col = (const I *)X.col, *lcol = (const I *)X.lcol, *ucol = (const I *)X.ucol;
@@ -871,13 +900,29 @@ static int inner_solve(Ctx &c, bool fac, int n_iter, double reduc, InnerCarry &c
   double *pU = pb(c, 0), *pQ = pb(c, 1);
 #define L(W)                                                                                                   \
   k_mx_start<W, I, float><<<G, RB, 0, c.stream>>>(n, fac ? 1 : 0, r[0], ev, P[0], p_last, f, wu, ucol, uval, ee, \
-                                                   t0, pU)
+                                                   t0, pU, nullptr)
   FVK_W(wu, L);
 #undef L
   const int chunk = 32;
   Poller Pl(c);
   int it = 0, next_poll = chunk;
+  const int R = X.replace_every;
   for (; it < n_iter; ++it) {
+    const bool rep = R > 0 && it > 0 && it % R == 0;
+    if (rep) {
+      // the scalars of iteration it - 1 gate the three kernels
+      KS *prev = c.ksr + ((it + 1) & 1) * 3;
+      float *rr = fac ? r[it & 1] : r[0];
+      k_mx_rep_x<<<grid_for(n), BS, 0, c.stream>>>(n, x, ev, fac ? P[0] : nullptr, c.diag, sigma, prev);
+#define L(W) k_mx_res<W><<<G, RB, 0, c.stream>>>(c.M, c.diag, c.spv, x, b, rr, pb(c, 3), prev, 1.0 / sigma)
+      FVK_W(c.Ws, L);
+#undef L
+#define L(W)                                                                                                 \
+  k_mx_start<W, I, float><<<G, RB, 0, c.stream>>>(n, fac ? 1 : 0, rr, nullptr, nullptr, nullptr, 1.f, wu, ucol, \
+                                                   uval, ee, t0, pU, prev)
+      FVK_W(wu, L);
+#undef L
+    }
     hipEvent_t ea, eb;
     const bool stamp = T && T->take(ea, eb);
     if (!fac) {
@@ -900,7 +945,7 @@ static int inner_solve(Ctx &c, bool fac, int n_iter, double reduc, InnerCarry &c
       // factor: p in place in P[0]; r double-buffered
       float *p = P[0], *rin = r[it & 1], *rout = r[(it + 1) & 1];
       const float *pc = p;
-      if (it == 0) {
+      if (it == 0 || rep) {
 #define L(W) k_mx_zp<W, I, float><<<G, RB, 0, c.stream>>>(n, wl, it, lcol, lval, t0, p, ev, pU, G, c.ksr, c.ks)
         FVK_W(wl, L);
 #undef L
@@ -983,7 +1028,7 @@ int dc_solve(Ctx &c, bool use_factor, const double *b, double *x, double tol, in
   InnerCarry carry;
   double sigma_prev = 0.0;
   for (int outer = 0;; ++outer) {
-#define L(W) k_mx_res<W><<<G, RB, 0, c.stream>>>(c.M, c.diag, c.spv, x, b, fv(c, V_R0), pb(c, 3))
+#define L(W) k_mx_res<W><<<G, RB, 0, c.stream>>>(c.M, c.diag, c.spv, x, b, fv(c, V_R0), pb(c, 3), nullptr, 1.0)
     FVK_W(c.Ws, L);
 #undef L
     k_mx_tot4<<<1, RB, 0, c.stream>>>(pb(c, 3), G, X.tot);
@@ -998,8 +1043,12 @@ int dc_solve(Ctx &c, bool use_factor, const double *b, double *x, double tol, in
     if (res.res_l1 < tol) { res.converged = true; break; }
     if (outer >= X.max_outer || res.iters >= max_iter || res.broken) break;
     // inner target: what is left to reach tol (with a margin), but no
-    // deeper than inner_reduc, where FP32 rounding takes over
-    const double eta = std::min(0.5, std::max(X.inner_reduc, X.eta_margin * tol / res.res_l1));
+    // deeper than inner_reduc, where FP32 rounding takes over.  With residual
+    // replacements the inner solve can go all the way, as long as |r|_2^2 of
+    // the normalised residual (at least n eta^2) stays far above the absolute
+    // small = 1e-20 in alpha = s/(p.q + small).
+    const double floor = X.replace_every > 0 ? std::sqrt(1e-12 / n) : X.inner_reduc;
+    const double eta = std::min(0.5, std::max(floor, X.eta_margin * tol / res.res_l1));
     // The inner right-hand side is normalised to a mean |r| of 1: the PCG
     // recurrence's alpha = s/(p.q + small) (solv_cg.f90:69-70) carries an
     // absolute small = 1e-20 that stalls the solve once p.q is that tiny,
@@ -1010,8 +1059,10 @@ int dc_solve(Ctx &c, bool use_factor, const double *b, double *x, double tol, in
     if (!X.carry) carry.p_buf = -1;
     const float f = carry.p_buf >= 0 ? (float)(sigma_prev / sigma) : 1.f;
     double alpha_pend = 0.0;
-    rc = X.i16 ? inner_solve<short>(c, use_factor, max_iter - res.iters, eta, carry, f, its, broken, alpha_pend, T)
-               : inner_solve<int>(c, use_factor, max_iter - res.iters, eta, carry, f, its, broken, alpha_pend, T);
+    rc = X.i16 ? inner_solve<short>(c, use_factor, max_iter - res.iters, eta, carry, f, its, broken, alpha_pend, T,
+                                    x, b, sigma)
+               : inner_solve<int>(c, use_factor, max_iter - res.iters, eta, carry, f, its, broken, alpha_pend, T, x,
+                                  b, sigma);
     if (rc) return rc;
     sigma_prev = sigma;
     if (outer < 8) {
@@ -1048,7 +1099,7 @@ static int apply_t(Ctx &c, const double *r, double *zout) {
   k_scale_f32<<<grid_for(n), BS, 0, c.stream>>>(n, r, c.diag, r32);
 #define L(W) \
   k_mx_start<W, I, float><<<G, RB, 0, c.stream>>>(n, 1, r32, ev, p0, nullptr, 1.f, wu, ucol, uval, \
-                                                  (const lval_t *)X.e, t, pb(c, 0))
+                                                  (const lval_t *)X.e, t, pb(c, 0), nullptr)
   FVK_W(wu, L);
 #undef L
 #define L(W) k_mx_gz<W, I><<<G, RB, 0, c.stream>>>(n, wl, lcol, lval, t, r32, z, pb(c, 2), nullptr)

@@ -75,6 +75,7 @@ def test_dc32_carried_direction_saves_iterations(factor):
             fvk.factor_set(ctx, l, sig)
             method = fvk.SOLVE_DC32_FACTOR
         fvk.set_param(ctx, "dc_inner_reduc", 1e-3)
+        fvk.set_param(ctx, "dc_replace_every", 0)
         out = {}
         for carry in (0, 1):
             fvk.set_param(ctx, "dc_carry", carry)
@@ -86,6 +87,41 @@ def test_dc32_carried_direction_saves_iterations(factor):
             print("carry", carry, info["iters"], info["inner_iters"])
         assert out[1]["outer"] >= 3
         assert out[1]["iters"] < out[0]["iters"]
+    finally:
+        om.free()
+
+
+@pytest.mark.parametrize("factor", [False, True])
+def test_dc32_residual_replacement_saves_iterations(factor):
+    """Replacing the recursive FP32 residual by the FP64 residual of the
+    solution so far every 32 inner iterations (the direction carries on)
+    reaches the FP64 residual in one inner solve and in fewer iterations than
+    restarted correction steps (dc_replace_every 0: two inner solves, the
+    first stopped at 1e-5); replacing every 8 iterations converges too."""
+    fvk, m, ctx, om, S, A, b = setup(40, 0.2, 0x5B)
+    try:
+        method = fvk.SOLVE_DC32
+        if factor:
+            lf, sf = L.fsai_closed_form(S)
+            inner = om.topology()["fce_in"] > 0
+            l = np.zeros(ctx.n_fce, np.float32)
+            sig = np.zeros(ctx.n_fce, np.float32)
+            l[inner] = lf
+            sig[inner] = sf
+            fvk.factor_set(ctx, l, sig)
+            method = fvk.SOLVE_DC32_FACTOR
+        out = {}
+        for R in (0, 32, 8):
+            fvk.set_param(ctx, "dc_replace_every", R)
+            info = fvk.sys_solve(ctx, method, 0, 1e-8)
+            x = fvk.sys_x(ctx)
+            print("replace every", R, info["iters"], info["inner_iters"], true_res(A, x, b))
+            assert info["status"] == fvk.FVK_OK and info["fallback"] == 0, (R, info)
+            assert true_res(A, x, b) < 1e-8
+            out[R] = info
+        assert out[0]["outer"] == 2 and out[32]["outer"] == 1
+        assert out[32]["iters"] < out[0]["iters"]
+        assert out[8]["iters"] < out[0]["iters"] + 10
     finally:
         om.free()
 
