@@ -187,12 +187,15 @@ int fvk_sys_set(fvk_ctx *ctx, const double *pl, const double *a, const double *b
 int fvk_sys_solve(fvk_ctx *ctx, int method, int imat, double tol, int max_iter, int time_kernel,
                   fvk_solve_info *info);
 int fvk_sys_get_x(fvk_ctx *ctx, double *x);
-/* Solver settings of a context: "dc_inner_reduc" (the deepest relative L1
-   reduction asked of an FP32 inner solve, default 1e-5; each inner solve
-   aims at max(dc_inner_reduc, dc_eta_margin tol / current residual),
-   dc_eta_margin default 0.8), "dc_max_outer"
-   (defect-correction steps, default 30), "dc_carry" (1: a correction step
-   continues the previous step's search direction; 0: restart, default). */
+/* Solver settings of a context: "dc_replace_every" (default 160: every that
+   many inner iterations the FP32 correction goes into the FP64 solution and
+   the inner solve continues on the FP64 residual, keeping its search
+   direction; each inner solve then aims at max(sqrt(1e-12/n),
+   dc_eta_margin tol / current residual); 0: no replacement, and the floor
+   is "dc_inner_reduc", default 1e-5), "dc_eta_margin" (default 0.8),
+   "dc_max_outer" (defect-correction steps, default 30), "dc_carry" (1: a
+   correction step continues the previous step's search direction; 0:
+   restart, default). */
 int fvk_ctx_set_param(fvk_ctx *ctx, const char *name, double value);
 
 /* The factorised preconditioner z = G G^T r, G = D^-1/2 (I + L) Delta, with
