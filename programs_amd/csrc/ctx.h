@@ -61,7 +61,6 @@ struct Mix {
   float *au = nullptr;                 // the same towards the upper neighbours only ((WU + 1) x ld)
   unsigned char *iau = nullptr;        // lower slot -> slot in the neighbour's upper list
   void *lval = nullptr, *uval = nullptr;  // L per slot, FP16
-  float *w = nullptr;                  // a_ii^-1/2 per row
   void *e = nullptr;                   // Delta^2 per row, FP16
   void *geo = nullptr;                 // geometric MLP features per interior face (bfloat16)
   float *fl = nullptr, *fsig = nullptr;  // per interior face l_f, sig_f

@@ -89,15 +89,13 @@ __device__ __forceinline__ float t_scale(const KS &k, int n) {
 // the update kernel registers and 15 % of its time.)
 __device__ __forceinline__ t16_t to_t16(float v) { return (t16_t)v; }
 
-// Jacobi-scaled FP32 copy of the matrix, w = a_ii^-1/2
+// Jacobi-scaled FP32 copy of the matrix (the Jacobi path's SpMV)
 __global__ __launch_bounds__(BS) void k_mx_conv(DMesh M, const double *__restrict__ diag,
-                                                const double *__restrict__ spv, float *__restrict__ s32,
-                                                float *__restrict__ w) {
+                                                const double *__restrict__ spv, float *__restrict__ s32) {
   const int i = blockIdx.x * BS + threadIdx.x;
   const int n = M.n;
   if (i >= n) return;
   const double wi = 1.0 / sqrt(diag[i]);
-  w[i] = (float)wi;
   for (int k = 0; k < M.Ws; ++k) {
     const size_t o = (size_t)k * M.ld + i;
     const int j = M.spc[o];
@@ -829,7 +827,6 @@ int mix_build(Ctx &c) {
   if ((rc = dalloc(c, &uv, (size_t)WU * ld))) return rc;
   X.lval = lv;
   X.uval = uv;
-  if ((rc = dalloc(c, &X.w, n))) return rc;
   lval_t *e16;
   if ((rc = dalloc(c, &e16, n))) return rc;
   X.e = e16;
@@ -1020,8 +1017,10 @@ int dc_solve(Ctx &c, bool use_factor, const double *b, double *x, double tol, in
     return FVK_ERR_STATE;
   }
   const int n = c.n, G = red_grid(n);
-  k_mx_conv<<<grid_for(n), BS, 0, c.stream>>>(c.M, c.diag, c.spv, X.spv, X.w);
-  if (use_factor)
+  // the factor path's SpMV reads the upper entries only (au)
+  if (!use_factor)
+    k_mx_conv<<<grid_for(n), BS, 0, c.stream>>>(c.M, c.diag, c.spv, X.spv);
+  else
     k_mx_au<<<grid_for(n), BS, 0, c.stream>>>(n, X.WU, X.ufi, X.fi_face, X.fi_n, c.ngp, c.diag, X.au);
   FVK_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, c.stream));
   res = DcResult();
