@@ -1015,14 +1015,20 @@ int flow_run_graph(Flow &F, int it0, int n, fvk_outer_hist *h, int &replayed) {
     F.graphs.resize(nseg, nullptr);
     for (int s = 0; s < nseg; ++s)
       if ((rc = capture(F, it, s, F.graphs[s]))) return rc;
+    // A replayed segment assembles matrices (and rebuilds the momentum
+    // solve's preconditioner) without the host code that marks the
+    // preconditioner vector stale, so the host does that after each launch.
     for (; it < it0 + n; ++it) {
       FVK_HIP(hipGraphLaunch(F.graphs[0], c.stream));
+      c.pc_valid = false;
       if (host) {
         if ((rc = pp_solve(F, F.pc, F.rhs))) return rc;
         FVK_HIP(hipGraphLaunch(F.graphs[1], c.stream));
+        c.pc_valid = false;
         if (F.cfg.simpler) {
           if ((rc = pp_solve(F, F.p, F.rhs_p))) return rc;
           FVK_HIP(hipGraphLaunch(F.graphs[2], c.stream));
+          c.pc_valid = false;
         }
       }
       ++replayed;
