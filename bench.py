@@ -222,6 +222,89 @@ def simple_leg(fvk, args, model):
                          "urf_p 0.3, momentum BiCGStab (3 RHS batched, Jacobi, 5 its)" % n, **out)
 
 
+def channel_leg(fvk, args):
+    """SIMPLE on the seeded-Delaunay 2D channel (length 10 x height 1, inlet
+    u = 1 (code 11), outlet (21), walls (32), Re = 500), cells in RCM order,
+    momentum BiCGStab with u, v batched, relaxed pressure update, outer
+    iterations replayed from HIP graphs; pressure-correction systems solved to
+    an FP64 relative residual of tol by plain FP64 Jacobi-PCG and by the
+    learned FP32-inner solve.  The network is trained (untimed) on small
+    channels of other seeds: one Adam step on the pressure-correction system
+    of each outer iteration of a SIMPLE run, interleaved with lognormal
+    systems; then online on the timed channel's own systems during the
+    untimed outer iterations.  Both flows then run the same timed outer
+    iterations and their fields are compared."""
+    from programs_amd import meshgen
+    t0 = time.time()
+    model = fvk.Model(seed=0x5EED0007)
+    ms = fvk.Mesh.from_cells(2, *meshgen.channel_delaunay(args.channel_train_tri, seed=0x5EED0008))
+    cs = fvk.grd_solv(ms, rcm=True)
+    ml = fvk.Mesh.from_cells(2, *meshgen.channel_delaunay(args.channel_train_tri, seed=0x5EED0009))
+    cl = fvk.grd_solv(ml, rcm=True)
+    fl = fvk.Flow(cs, fvk.flow_cfg(simpler=0, imat_u=0, imat_p=0, urf_p=0.3, mu=0.002, p_solver=2, p_tol=args.tol))
+    fl.set_bnd_u(11, [1.0, 0.0])
+    fl.set_model(model, train_steps=1, lr=args.lr)
+    n_log = 0
+    for it in range(1, args.channel_pretrain + 1):
+        fl.run(it, 1)
+        for _ in range(args.channel_pretrain_lognormal):
+            fvk.sys_random(cl, 0x9000 + n_log, args.sigma)
+            model.train(cl, 1, lr=args.lr)
+            n_log += 1
+        if it % 100 == 0:
+            log("  channel network: %d SIMPLE and %d lognormal systems (%.1f s)" % (it, n_log, time.time() - t0))
+    fl.destroy()
+    cs.destroy()
+    ms.free()
+    cl.destroy()
+    ml.free()
+    train_s = time.time() - t0
+    t0 = time.time()
+    m = fvk.Mesh.from_cells(2, *meshgen.channel_delaunay(args.channel, seed=0x5EED0005))
+    c = fvk.grd_solv(m, rcm=True)
+    log("channel: %d triangles, %d faces (%.1f s)" % (m.n_vol, m.n_fce, time.time() - t0))
+    out, fields = {}, {}
+    warm, its = args.channel_warm, args.channel_its
+    for name, ps in (("plain_pcg64", 3), ("dc32_learned", 2)):
+        cfg = fvk.flow_cfg(simpler=0, imat_u=0, imat_p=0, urf_p=0.3, mu=0.002, p_solver=ps, p_tol=args.tol)
+        fl = fvk.Flow(c, cfg)
+        fl.set_bnd_u(11, [1.0, 0.0])
+        if ps == 2:
+            fl.set_model(model, train_steps=args.channel_train, lr=3e-3)
+        h0 = []
+        for k in range(warm):
+            h0 += fl.run_graph(1 + k, 1)
+            log("channel %s: untimed outer iteration %d, p' iterations %d" % (name, 1 + k, h0[-1]["it_pp"]))
+        if ps == 2:
+            fl.set_model(model, train_steps=0)
+        c.sync()
+        t1 = time.perf_counter()
+        h = fl.run_graph(1 + warm, its)
+        c.sync()
+        dt = (time.perf_counter() - t1) / its
+        st = fl.state()
+        fields[name] = {k: st[k] for k in ("u_cell", "p")}
+        out["pressure_to_tol_" + name] = dict(ms_per_outer=dt * 1e3, outer_its_per_s=1.0 / dt,
+                                              p_iters_untimed=[x["it_pp"] for x in h0],
+                                              p_iters=[x["it_pp"] for x in h])
+        log("channel %s: %.1f ms per outer, p' iterations %s (untimed %s)"
+            % (name, dt * 1e3, out["pressure_to_tol_" + name]["p_iters"],
+               out["pressure_to_tol_" + name]["p_iters_untimed"]))
+        fl.destroy()
+    model.destroy()
+    a, b = fields["plain_pcg64"], fields["dc32_learned"]
+    out["field_rel_diff"] = {k: float(np.max(np.abs(a[k] - b[k])) / np.max(np.abs(a[k]))) for k in a}
+    out["speedup_vs_plain"] = (out["pressure_to_tol_plain_pcg64"]["ms_per_outer"]
+                               / out["pressure_to_tol_dc32_learned"]["ms_per_outer"])
+    c.destroy()
+    m.free()
+    out["training"] = dict(seconds=train_s, triangles=args.channel_train_tri, simple=args.channel_pretrain,
+                           lognormal=n_log, online_steps_per_system=args.channel_train, online_systems=warm)
+    return dict(workload="SIMPLE outer iterations %d..%d, seeded-Delaunay 2D channel (10 x 1, Re=500), %d triangles "
+                         "in RCM order, relaxed pressure update, pressure-correction solves to FP64 relative residual "
+                         "%g" % (warm + 1, warm + its, args.channel, args.tol), **out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -241,6 +324,14 @@ def main():
     ap.add_argument("--simple-its", type=int, default=20)
     ap.add_argument("--simple-tol-its", type=int, default=3)
     ap.add_argument("--no-simple", action="store_true")
+    ap.add_argument("--channel", type=int, default=0, help="triangles of the optional channel leg (0: off)")
+    ap.add_argument("--channel-warm", type=int, default=6)
+    ap.add_argument("--channel-its", type=int, default=3)
+    ap.add_argument("--channel-train", type=int, default=5, help="online Adam steps per untimed system")
+    ap.add_argument("--channel-train-tri", type=int, default=40000)
+    ap.add_argument("--channel-pretrain", type=int, default=300, help="SIMPLE outer iterations of the training run")
+    ap.add_argument("--channel-pretrain-lognormal", type=int, default=3, help="lognormal systems per outer")
+    ap.add_argument("--channel-only", action="store_true", help="run the channel leg alone")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is "
@@ -248,6 +339,11 @@ def main():
         sys.exit(2)
 
     from programs_amd import fvk
+    if args.channel_only:
+        if args.channel <= 0:
+            sys.exit("--channel-only needs --channel N (triangles)")
+        print(json.dumps({"channel": channel_leg(fvk, args)}), flush=True)
+        return
     n = args.n
     t0 = time.time()
     mesh = fvk.Mesh.box(n, n, n, jitter=args.jitter, seed=0x5EED0002)
@@ -362,6 +458,8 @@ def main():
     }
     if not args.no_simple:
         out["simple"] = simple_leg(fvk, args, model)
+    if args.channel > 0:
+        out["channel"] = channel_leg(fvk, args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ctx, fvk, n, args.jitter, r["iters"], args.cpu_sample)
         if not args.no_simple:
