@@ -152,6 +152,37 @@ static int build_levels(Ctx &c, const Topo &t, const std::vector<int> &iperm, bo
   return upload(c, &L.rows, rows);
 }
 
+// Per-slot copies of the face geometry (the slot-major tables of DMesh);
+// boundary slots take the face centre minus the cell centre in gp.
+__global__ __launch_bounds__(BS) void k_slot_geom(DMesh M, double *gs, double *gnx, double *gny, double *gnz,
+                                                  double *gw, double *gpx, double *gpy, double *gpz, double *gqx,
+                                                  double *gqy, double *gqz, int *go) {
+  const size_t total = (size_t)M.W * M.n;
+  for (size_t o = blockIdx.x * (size_t)BS + threadIdx.x; o < total; o += (size_t)gridDim.x * BS) {
+    const int e = M.cf[o], i = (int)(o % (size_t)M.n);
+    double v[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int other = -1;
+    if (e != PAD) {
+      const bool own = e >= 0;
+      const int f = own ? e : ~e;
+      v[0] = M.fs[f];
+      v[1] = M.fnx[f]; v[2] = M.fny[f]; v[3] = M.fnz[f];
+      if (M.fin[f] >= 0) {
+        other = own ? M.fin[f] : M.fip[f];
+        v[4] = M.fwfp[f];
+        v[5] = M.dpx[f]; v[6] = M.dpy[f]; v[7] = M.dpz[f];
+        v[8] = M.dnx[f]; v[9] = M.dny[f]; v[10] = M.dnz[f];
+      } else {
+        other = ~M.fib[f];
+        v[5] = M.fx[f] - M.cx[i]; v[6] = M.fy[f] - M.cy[i]; v[7] = M.fz[f] - M.cz[i];
+      }
+    }
+    gs[o] = v[0]; gnx[o] = v[1]; gny[o] = v[2]; gnz[o] = v[3]; gw[o] = v[4];
+    gpx[o] = v[5]; gpy[o] = v[6]; gpz[o] = v[7]; gqx[o] = v[8]; gqy[o] = v[9]; gqz[o] = v[10];
+    go[o] = other;
+  }
+}
+
 static int ctx_build(Ctx &c, const Topo &t, int device, int flags) {
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device) {
@@ -233,6 +264,20 @@ static int ctx_build(Ctx &c, const Topo &t, int device, int flags) {
   UPI(cf, cf); UPI(spc, spc); UPI(spf, spf);
 #undef UP
 #undef UPI
+  {
+    const size_t ns = (size_t)W * n;
+    double *g[11];
+    int *go = nullptr;
+    for (int k = 0; k < 11; ++k)
+      if ((rc = dalloc(c, &g[k], ns))) return rc;
+    if ((rc = dalloc(c, &go, ns))) return rc;
+    const int gb = (int)std::max<size_t>(1, std::min<size_t>((ns + BS - 1) / BS, 65536));
+    k_slot_geom<<<gb, BS, 0, c.stream>>>(M, g[0], g[1], g[2], g[3], g[4], g[5], g[6], g[7], g[8], g[9], g[10], go);
+    FVK_HIP(hipGetLastError());
+    M.gs = g[0]; M.gnx = g[1]; M.gny = g[2]; M.gnz = g[3]; M.gw = g[4];
+    M.gpx = g[5]; M.gpy = g[6]; M.gpz = g[7]; M.gqx = g[8]; M.gqy = g[9]; M.gqz = g[10];
+    M.go = go;
+  }
   if ((rc = build_levels(c, t, c.iperm, true, c.lf))) return rc;
   if ((rc = build_levels(c, t, c.iperm, false, c.lb))) return rc;
   if ((rc = dalloc(c, &c.diag, n))) return rc;

@@ -28,6 +28,13 @@ struct DMesh {
   const double *fx, *fy, *fz, *fnx, *fny, *fnz;
   const double *dpx, *dpy, *dpz, *dnx, *dny, *dnz;
   const int *cf;    // W*n, slot-major: f (owner side), ~f (neighbour side), PAD
+  // The face geometry again per cell slot (W*n, slot-major like cf), for the
+  // gradient's cell gather: s, n, w, dx_p, dx_n of the slot's face, and in
+  // go the other row of an interior face or ~fib of a boundary face (whose
+  // gp holds the face centre minus the cell centre).  The same values as the
+  // face arrays, laid out so that the rows of a wave read adjacent words.
+  const double *gs, *gnx, *gny, *gnz, *gw, *gpx, *gpy, *gpz, *gqx, *gqy, *gqz;
+  const int *go;
   const int *spc;   // Ws x ld interior slots: column row, -1 on padding
   const int *spf;   // Ws x ld: face id of the slot, -1 on padding
 };

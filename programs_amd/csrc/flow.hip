@@ -35,23 +35,22 @@ __global__ __launch_bounds__(BS) void k_grad(DMesh M, const double *__restrict__
     return;
   }
   const double vri = M.vr[i];
+  // the faces' geometry comes from the per-slot tables (the same values as
+  // the face arrays, read by adjacent rows at adjacent addresses)
   for (int k = 0; k < M.W; ++k) {
-    const int e = M.cf[k * n + i];
+    const size_t o = (size_t)k * n + i;
+    const int e = M.cf[o];
     if (e == PAD) break;
     const bool own = e >= 0;
-    const int f = own ? e : ~e;
-    const double s = M.fs[f];
-    double nv[3];
-    face_n(M, f, nv);
+    const double s = M.gs[o];
+    const double nv[3] = {M.gnx[o], M.gny[o], M.gnz[o]};
     double pf[3];
-    const int in = M.fin[f];
-    if (in >= 0) {
-      const int ip = M.fip[f];
-      const double w = M.fwfp[f];
+    const int other = M.go[o];
+    if (other >= 0) {
+      const int ip = own ? i : other, in = own ? other : i;
+      const double w = M.gw[o];
       if (it > 1) {
-        double tp[3], tn[3];
-        face_dxp(M, f, tp);
-        face_dxn(M, f, tn);
+        const double tp[3] = {M.gpx[o], M.gpy[o], M.gpz[o]}, tn[3] = {M.gqx[o], M.gqy[o], M.gqz[o]};
         for (int q = 0; q < nvc; ++q) {
           double gp[3] = {g0[(q * 3 + 0) * n + ip], g0[(q * 3 + 1) * n + ip], g0[(q * 3 + 2) * n + ip]};
           double gn[3] = {g0[(q * 3 + 0) * n + in], g0[(q * 3 + 1) * n + in], g0[(q * 3 + 2) * n + in]};
@@ -71,14 +70,14 @@ __global__ __launch_bounds__(BS) void k_grad(DMesh M, const double *__restrict__
       if (neu > 0) {
         for (int q = 0; q < nvc; ++q) pf[q] = phi[q * n + i];
         if (it > 1 && neu == 2) {
-          double t[3] = {M.fx[f] - M.cx[i], M.fy[f] - M.cy[i], nd > 2 ? M.fz[f] - M.cz[i] : 0.0};
+          const double t[3] = {M.gpx[o], M.gpy[o], nd > 2 ? M.gpz[o] : 0.0};
           for (int q = 0; q < nvc; ++q) {
             double gp[3] = {g0[(q * 3 + 0) * n + i], g0[(q * 3 + 1) * n + i], g0[(q * 3 + 2) * n + i]};
             pf[q] = pf[q] + dotn(gp, t, nd);
           }
         }
       } else {
-        const int b = M.fib[f];
+        const int b = ~other;
         for (int q = 0; q < nvc; ++q) pf[q] = phib[q * M.nbf + b];
       }
       for (int q = 0; q < nvc; ++q)
