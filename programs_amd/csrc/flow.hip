@@ -399,11 +399,13 @@ __global__ __launch_bounds__(BS) void k_mom_cell(DMesh M, MomPar P, int nvc, dou
   for (int q = 0; q < nvc; ++q) r[q] = r[q] + 0.0;  // eqn%rhs, zero in single-phase flow
   int ks = 0;
   for (int k = 0; k < M.W; ++k) {
-    const int e = M.cf[k * n + i];
+    const size_t o = (size_t)k * n + i;
+    const int e = M.cf[o];
     if (e == PAD) break;
     const bool own = e >= 0;
     const int f = own ? e : ~e;
-    if (M.fin[f] >= 0) {
+    const int other = M.go[o];  // the other row, or ~fib on a boundary face
+    if (other >= 0) {
       if (own) {
         const double a = ngp[f];
         pl = pl - a;
@@ -420,7 +422,7 @@ __global__ __launch_bounds__(BS) void k_mom_cell(DMesh M, MomPar P, int nvc, dou
       ++ks;
     } else {
       const double a = ngp[f];
-      const int b = M.fib[f];
+      const int b = ~other;
       pl = pl - a;
       for (int q = 0; q < nvc; ++q) r[q] = r[q] + cor[q * nf + f];
       for (int q = 0; q < nvc; ++q) r[q] = r[q] - a * ub[q * M.nbf + b];
@@ -537,11 +539,12 @@ __global__ __launch_bounds__(BS) void k_pp_cell(DMesh M, const double *__restric
   double pl = 0.0, r = -cs[i];
   int ks = 0;
   for (int k = 0; k < M.W; ++k) {
-    const int e = M.cf[k * n + i];
+    const size_t o = (size_t)k * n + i;
+    const int e = M.cf[o];
     if (e == PAD) break;
     const bool own = e >= 0;
     const int f = own ? e : ~e;
-    if (M.fin[f] >= 0) {
+    if (M.go[o] >= 0) {
       const double a = own ? ngp[f] : ngn[f];
       pl = pl - a;
       r = own ? r - cpp[f] : r + cpp[f];
@@ -620,11 +623,12 @@ __global__ __launch_bounds__(BS) void k_flux_cell(DMesh M, const double *__restr
   const int n = M.n;
   double r = 0.0;
   for (int k = 0; k < M.W; ++k) {
-    const int e = M.cf[k * n + i];
+    const size_t o = (size_t)k * n + i;
+    const int e = M.cf[o];
     if (e == PAD) break;
     const bool own = e >= 0;
     const int f = own ? e : ~e;
-    if (M.fin[f] < 0) continue;
+    if (M.go[o] < 0) continue;
     r = own ? r - fc1[f] : r + fc1[f];
   }
   rhs_p[i] = rhs[i] + r;
