@@ -29,12 +29,16 @@ the solve's stream; its HBM traffic from the byte counters recorded in
 profiles/hbm_traffic.json) and of the plain path's fused p-update + SpMV,
 the FP64 SpMV bandwidth, a device copy's bandwidth, SIMPLE outer iterations
 on the 128^3 cavity (reference pressure solves, eager and from HIP graphs,
-and pressure solves to 1e-8 plain and learned), and the CPU oracle's times
-(single core, extrapolated from bounded samples).
+and pressure solves to 1e-8 plain and learned), and CPU baselines on one
+host core extrapolated from bounded samples: the reference's own solv_cg
+(flang build of solv_cg.f90 with a timing driver, oracle/_ref/cg) on the same
+system, the oracle's C restatement of it, and the oracle's SIMPLE outer
+iteration.
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -67,6 +71,30 @@ def traffic_bytes(kernel):
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline_reference(ctx, mesh, fvk, n, gpu_iters, sample_iters):
+    """The reference's own PCG (spray-simulation/solv_cg.f90 built with flang
+    -O2, one core, imat = 0) on the same system, through its timing driver:
+    a bounded number of iterations timed inside solv_cg, scaled to the GPU
+    path's iteration count (the recurrence is the same, so is the count).
+    None where the reference build is absent."""
+    from oracle import refcg
+    if not refcg.available():
+        return None
+    pl, a, b = fvk.sys_get(ctx)
+    t = mesh.topology()
+    try:
+        sec, _ = refcg.run(t["fce_ip"], t["fce_in"], -a, -a, pl, b, sample_iters)
+    except (RuntimeError, OSError, subprocess.TimeoutExpired) as e:
+        log("reference PCG baseline not measured: %s" % e)
+        return None
+    per_it = sec / sample_iters
+    return dict(value=per_it * gpu_iters, unit="s", cores=1, kind="reference",
+                sample="%d Jacobi-PCG iterations of the reference's solv_cg (flang -O2, one host core) on the "
+                       "same %d^3 system, %.1f s inside solv_cg (writing and reading the system excluded), scaled "
+                       "to the %d iterations the GPU path needs" % (sample_iters, n, sec, gpu_iters),
+                per_iter_ms=per_it * 1e3)
 
 
 def cpu_baseline(ctx, fvk, n, jitter, gpu_iters, sample_iters):
@@ -314,7 +342,8 @@ def main():
     ap.add_argument("--jitter", type=float, default=0.2)
     ap.add_argument("--sigma", type=float, default=1.0)
     ap.add_argument("--tol", type=float, default=1e-8)
-    ap.add_argument("--cpu-sample", type=int, default=300)
+    ap.add_argument("--cpu-sample", type=int, default=300, help="iterations of the oracle's PCG timed")
+    ap.add_argument("--cpu-ref-sample", type=int, default=500, help="iterations of the reference's PCG timed")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--methods", default="dc32,learned", help="methods timed beside the plain solve")
     ap.add_argument("--train-lognormal", type=int, default=1600)
@@ -461,7 +490,10 @@ def main():
     if args.channel > 0:
         out["channel"] = channel_leg(fvk, args)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(ctx, fvk, n, args.jitter, r["iters"], args.cpu_sample)
+        port = cpu_baseline(ctx, fvk, n, args.jitter, r["iters"], args.cpu_sample)
+        ref = cpu_baseline_reference(ctx, mesh, fvk, n, r["iters"], args.cpu_ref_sample)
+        out["cpu_baseline"] = ref if ref is not None else port
+        out["cpu_baseline_port"] = port
         if not args.no_simple:
             out["cpu_baseline_simple"] = cpu_simple(args)
     print(json.dumps(out), flush=True)

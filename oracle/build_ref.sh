@@ -4,7 +4,7 @@
 # run in the development container only (the reference is not on GPU hosts).
 #
 # Objects and module files go to a scratch directory outside the repository;
-# only the two driver executables land in oracle/_ref/:
+# only the driver executables land in oracle/_ref/:
 #   _ref/shipped/fvk_ref_driver   reference sources compiled where they lie
 #   _ref/relaxed/fvk_ref_driver   same, except that cfd_setup.f90 is taken
 #       from a scratch copy with two hard-coded locals switched:
@@ -12,7 +12,9 @@
 #                                                relaxed update at :675-683)
 #         call solv_cg(1, -> call solv_cg(0,   (cfd_setup.f90:662,742; Jacobi)
 #       which is the SIMPLE/Jacobi-PCG configuration the reference cannot
-#       select without editing a literal.
+#       select without editing a literal;
+#   _ref/cg/fvk_ref_cg            the reference's solv_cg with a timing
+#       driver (oracle/ref/fvk_ref_cg.f90), the CPU baseline of bench.py.
 set -euo pipefail
 REF=${FVK_REFERENCE_DIR:-/root/reference/spray-simulation}
 HERE=$(cd "$(dirname "$0")" && pwd)
@@ -47,4 +49,15 @@ sed -e 's/simpler=\.true\./simpler=.false./' -e 's/call solv_cg(1,/call solv_cg(
 [ "$(grep -c 'simpler=.false.' "$SCRATCH/relaxed_src/cfd_setup.f90")" -ge 2 ]
 [ "$(grep -c 'call solv_cg(0,' "$SCRATCH/relaxed_src/cfd_setup.f90")" -eq 2 ]
 build relaxed "$SCRATCH/relaxed_src"
-echo "built $OUT/shipped/fvk_ref_driver $OUT/relaxed/fvk_ref_driver"
+
+# the timing driver of the reference's PCG (bench.py's CPU baseline): the
+# solver modules only, compiled where they lie
+w=$SCRATCH/cg objs=""
+mkdir -p "$w" "$OUT/cg"
+for m in precision solv_data solv_res solv_precond solv_direct solv_cg; do
+  (cd "$w" && $FC $FFLAGS -c "$REF/$m.f90" -o "$m.o")
+  objs="$objs $w/$m.o"
+done
+(cd "$w" && $FC $FFLAGS -c "$HERE/ref/fvk_ref_cg.f90" -o driver.o)
+$FC $FFLAGS -o "$OUT/cg/fvk_ref_cg" "$w/driver.o" $objs
+echo "built $OUT/shipped/fvk_ref_driver $OUT/relaxed/fvk_ref_driver $OUT/cg/fvk_ref_cg"
