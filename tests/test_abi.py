@@ -30,3 +30,14 @@ def test_error_reporting_without_device_calls():
     assert e.value.status == -5
     with pytest.raises(fvk.FvkError):
         fvk.Mesh.box(0, 3, 3)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus N > 1 exits with status 2 and says so
+    before touching a device."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
