@@ -831,6 +831,10 @@ int fvk_sys_solve(fvk_ctx *cc, int method, int imat, double tol, int max_iter, i
       // k_mx_pq on the Jacobi-scaled FP32 system (unit diagonal): z, p_old
       // read; p_new, q written; 16/32-bit column + FP32 value per off-diagonal
       info->kern_bytes = 16.0 * n + (c.mx.i16 ? 6.0 : 8.0) * nnz_off;
+    else if (c.mx.qfull)
+      // k_mx_qf (factor path, full rows): p read, q written; a column and
+      // an FP32 value per off-diagonal
+      info->kern_bytes = 8.0 * n + (c.mx.i16 ? 6.0 : 8.0) * nnz_off;
     else
       // k_mx_q (factor path): p read, q written; per upper off-diagonal a
       // column and an FP32 value, per lower one a column and a one-byte slot

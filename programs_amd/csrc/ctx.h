@@ -50,6 +50,7 @@ struct Levels {
 // at the row itself with value 0.
 struct Mix {
   bool built = false, i16 = false;
+  bool qfull = false;  // factor path's SpMV from the full rows (spv, col): fewer bytes than au/iau
   int WL = 0, WU = 0, nfi = 0;
   void *col = nullptr;                 // A's interior slots (Ws*n)
   void *lcol = nullptr, *ucol = nullptr;  // factor slots (WL*n, WU*n)

@@ -444,6 +444,40 @@ __global__ __launch_bounds__(RB, 8) void k_mx_q(int n, int wx, int it, const I *
   store_partials<1>(v, partQ);
 }
 
+// Factor path on meshes whose rows have few neighbours (triangles: three in
+// all, so the upper and lower lists above are half padding): q = Ah p from
+// the full FP32 rows of Ah (the Jacobi path's slots), partial p.q; nothing
+// once the solve is done.
+template <int WS, class I>
+__global__ __launch_bounds__(RB, 8) void k_mx_qf(int n, int ws, int it, const I *__restrict__ col,
+                                                 const float *__restrict__ spv, const float *__restrict__ p,
+                                                 float *__restrict__ q, double *__restrict__ partQ, const KS *ring) {
+  int r0, r1;
+  block_rows(n, r0, r1);
+  if (ring[(it & 1) * 3].done) return;
+  const size_t ld = mx_ld(n);
+  double pq = 0.0;
+  auto row = [&](int i) {
+    const float pi = p[i];
+    float y = pi;
+#pragma unroll
+    for (int k = 0; k < nw<WS>(ws); ++k) {
+      const size_t o = (size_t)k * ld + i;
+      y += spv[o] * p[colat(col, o, i)];
+    }
+    q[i] = y;
+    pq += (double)pi * (double)y;
+  };
+  int i = r0 + threadIdx.x;
+  for (; i + RB < r1; i += 2 * RB) {
+    row(i);
+    row(i + RB);
+  }
+  if (i < r1) row(i);
+  double v[1] = {pq};
+  store_partials<1>(v, partQ);
+}
+
 // Upper entries of Ah: au[k][p] = a_pm / sqrt(a_pp a_mm) for the k-th upper
 // neighbour m of row p (zero on padding; slot WX stays zero)
 __global__ __launch_bounds__(BS) void k_mx_au(int n, int WU, const int *__restrict__ ufi,
@@ -782,6 +816,13 @@ int mix_build(Ctx &c) {
   long span = 0;
   for (size_t o = 0; o < ac.size(); ++o) span = std::max(span, std::labs((long)ac[o] - (long)(o % ld)));
   X.i16 = span <= 32767;
+  {
+    // bytes per row of the factor path's SpMV: the full rows (value and
+    // column per slot) against the upper entries plus the gathered lower
+    // ones (value, two columns and a slot byte per padded slot pair)
+    const int cb = X.i16 ? 2 : 4;
+    X.qfull = Ws * (4 + cb) < WX * (5 + 2 * cb);
+  }
   auto up_cols = [&](const std::vector<int> &v, void **dst) -> int {
     if (X.i16) {
       std::vector<short> s(v.size());
@@ -953,7 +994,20 @@ static int inner_solve(Ctx &c, bool fac, int n_iter, double reduc, InnerCarry &c
       }
       const float *au = X.au;
       const unsigned char *iau = X.iau;
-      if (stamp) {
+      if (X.qfull) {
+        const float *sp = X.spv;
+        if (stamp) {
+#define L(W)                                                                                                     \
+  hipExtLaunchKernelGGL(k_mx_qf<W, I>, dim3(G), dim3(RB), 0, c.stream, ea, eb, 0, n, ws, it, col, sp, pc, q, pQ, \
+                        (const KS *)c.ksr)
+          FVK_W(ws, L);
+#undef L
+        } else {
+#define L(W) k_mx_qf<W, I><<<G, RB, 0, c.stream>>>(n, ws, it, col, sp, pc, q, pQ, c.ksr)
+          FVK_W(ws, L);
+#undef L
+        }
+      } else if (stamp) {
 #define L(W)                                                                                                      \
   hipExtLaunchKernelGGL(k_mx_q<W, I>, dim3(G), dim3(RB), 0, c.stream, ea, eb, 0, n, wl, it, ucol, lcol, au, iau, pc, \
                         q, pQ, (const KS *)c.ksr)
@@ -1017,8 +1071,9 @@ int dc_solve(Ctx &c, bool use_factor, const double *b, double *x, double tol, in
     return FVK_ERR_STATE;
   }
   const int n = c.n, G = red_grid(n);
-  // the factor path's SpMV reads the upper entries only (au)
-  if (!use_factor)
+  // the factor path's SpMV reads the upper entries only (au), or the full
+  // rows where those are fewer bytes
+  if (!use_factor || X.qfull)
     k_mx_conv<<<grid_for(n), BS, 0, c.stream>>>(c.M, c.diag, c.spv, X.spv);
   else
     k_mx_au<<<grid_for(n), BS, 0, c.stream>>>(n, X.WU, X.ufi, X.fi_face, X.fi_n, c.ngp, c.diag, X.au);
