@@ -101,6 +101,11 @@ int fvk_ctx_create_arrays(int n_dx, int n_vol, int n_fce, int n_bf, const int *f
                           fvk_ctx **out);
 void fvk_ctx_destroy(fvk_ctx *ctx);
 int fvk_ctx_sync(fvk_ctx *ctx);
+/* The reverse Cuthill-McKee order FVK_CTX_RCM gives a context on this
+   mesh (perm[k] = reference cell id, 1-based, of row k); host only, no
+   device needed.  No counterpart in the reference (cells keep the
+   Gambit order there). */
+int fvk_mesh_rcm(const fvk_mesh *m, int *perm);
 /* Cell order used on the device: perm[k] = reference cell id (1-based) of
    device row k; identity unless FVK_CTX_RCM. */
 int fvk_ctx_get_perm(const fvk_ctx *ctx, int *perm);
@@ -173,8 +178,14 @@ typedef struct fvk_solve_info {
 #define FVK_SOLVE_DC32_FACTOR 2 /* same, preconditioned by z = G G^T r with the
                                    factor of fvk_factor_set / fvk_factor_build;
                                    falls back to FVK_SOLVE_PCG64 (Jacobi, from
-                                   x = 0) when it does not converge within
-                                   max_iter or breaks down: FVK_FALLBACK */
+                                   x = 0, with a budget of max_iter of its
+                                   own: a call does at most 2 max_iter
+                                   iterations) when it does not converge
+                                   within max_iter or breaks down more than
+                                   twice (p.q <= 0 or non-finite; a breakdown
+                                   keeps the finite correction made so far
+                                   and restarts the inner solve):
+                                   FVK_FALLBACK */
 
 /* a_f = s/l_pn * exp(sigma*z_f), z_f ~ N(0,1) from SplitMix64(seed) by face
    index; b ~ N(0,1) by cell, projected to zero mean.  b_out (reference
@@ -183,7 +194,9 @@ int fvk_sys_random(fvk_ctx *ctx, uint64_t seed, double sigma, double *b_out);
 /* Symmetric system from host arrays: pl, a (= -ng%p = -ng%n per face), b. */
 int fvk_sys_set(fvk_ctx *ctx, const double *pl, const double *a, const double *b);
 /* Solve from x = 0 until the method's relative L1 residual is below tol (or
-   max_iter).  imat selects the plain preconditioner (0 Jacobi, 1 DIC). */
+   max_iter iterations; see the fallback above).  imat selects the plain
+   preconditioner (0 Jacobi, 1 DIC).  A non-finite right-hand side is
+   FVK_ERR_ARG. */
 int fvk_sys_solve(fvk_ctx *ctx, int method, int imat, double tol, int max_iter, int time_kernel,
                   fvk_solve_info *info);
 int fvk_sys_get_x(fvk_ctx *ctx, double *x);
@@ -235,6 +248,11 @@ int fvk_model_nparam(void);
 int fvk_model_create(int device, uint64_t seed, fvk_model **out);
 void fvk_model_destroy(fvk_model *m);
 int fvk_model_get(const fvk_model *m, float *theta);
+/* The twelve features the network reads for each interior face of the
+   context's current matrix, as it reads them (rounded to bfloat16),
+   X[12 f + k] in reference face order, 0 on boundary faces.  A diagnostic
+   of the inputs; no counterpart in the reference. */
+int fvk_factor_features(fvk_ctx *ctx, float *X);
 /* Sets the parameters and resets the optimiser state. */
 int fvk_model_set(fvk_model *m, const float *theta);
 /* Factor of the context's current matrix from the network (then

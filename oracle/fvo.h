@@ -92,6 +92,12 @@ int fvo_solv_cg(const fvo_sys *A, int imat, double reduc, int n_iter, int n_vc,
 int fvo_solv_bicg(const fvo_sys *A, int imat, double reduc, int n_iter, int n_vc,
                   double *phi, int ld, double *res0, double *res0_n, int not_r0, int *iters,
                   double *hist);
+/* Rows summing to zero (pure Neumann)? */
+int fvo_is_singular(const fvo_sys *A);
+/* Solve to a relative L1 residual tol from x = 0 (b, x 1-based); returns
+   whether the system was treated as singular. */
+int fvo_pcg_tol(const fvo_sys *A, const double *b, double *x, double tol, int max_iter, int *iters,
+                double *res0_n);
 
 /* ---- flow state and SIMPLE (cfd_setup.f90 / cfd_solve.f90) ---- */
 typedef struct fvo_cfg {
@@ -100,6 +106,12 @@ typedef struct fvo_cfg {
   int n_bicg, n_cg, n_grad, c_sch;
   double urf_u, urf_p, reduc_u, reduc_p, c_bl;
   double rho, mu;
+  /* pressure solves: 0 the reference's PCG (imat_p, reduc_p, n_cg); 3 from
+     x = 0 to the relative L1 residual p_tol (fvo_pcg_tol, at most p_cap
+     iterations); the SIMPLER pressure solve then adds its solution to p */
+  int p_solver;
+  double p_tol;
+  int p_cap;
 } fvo_cfg;
 
 typedef struct fvo_flow {
@@ -122,6 +134,8 @@ typedef struct fvo_flow {
   double h_reduc, h_redpp, h_redpp_sum, h_red_p, h_red_p_sum;
   double res0_n_u[3], res0_n_pp, res0_n_p;
   int it_u[3], it_pp, it_p;
+  /* the last p' system: its right-hand side and solution (n_vol+1) */
+  double *pc_last, *rhs_last;
 } fvo_flow;
 
 fvo_flow *fvo_flow_new(fvo_mesh *m, const fvo_cfg *cfg);

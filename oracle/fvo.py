@@ -45,7 +45,7 @@ class Cfg(C.Structure):
                 ("n_bicg", C.c_int), ("n_cg", C.c_int), ("n_grad", C.c_int), ("c_sch", C.c_int),
                 ("urf_u", C.c_double), ("urf_p", C.c_double), ("reduc_u", C.c_double),
                 ("reduc_p", C.c_double), ("c_bl", C.c_double), ("rho", C.c_double),
-                ("mu", C.c_double)]
+                ("mu", C.c_double), ("p_solver", C.c_int), ("p_tol", C.c_double), ("p_cap", C.c_int)]
 
 
 class Flow(C.Structure):
@@ -60,7 +60,8 @@ class Flow(C.Structure):
                 ("h_reduc", C.c_double), ("h_redpp", C.c_double), ("h_redpp_sum", C.c_double),
                 ("h_red_p", C.c_double), ("h_red_p_sum", C.c_double),
                 ("res0_n_u", C.c_double * 3), ("res0_n_pp", C.c_double), ("res0_n_p", C.c_double),
-                ("it_u", C.c_int * 3), ("it_pp", C.c_int), ("it_p", C.c_int)]
+                ("it_u", C.c_int * 3), ("it_pp", C.c_int), ("it_p", C.c_int),
+                ("pc_last", dp), ("rhs_last", dp)]
 
 
 def build():
@@ -105,6 +106,8 @@ def lib():
                                       dp, dp, C.c_int, ip, dp]
         L.fvo_solv_res.argtypes = [_P(Sys), C.c_int, dp, C.c_int, dp, dp, C.c_int]
         L.fvo_set_sum_order.argtypes = [C.c_int]
+        L.fvo_is_singular.argtypes = [_P(Sys)]
+        L.fvo_pcg_tol.argtypes = [_P(Sys), dp, dp, C.c_double, C.c_int, ip, dp]
         _lib = L
     return _lib
 
@@ -229,7 +232,8 @@ def default_cfg(**kw):
     n_cg=20, n_bicg=5, reduc 0.1 (cfd_setup.f90:61,528,580-581,662,742), momentum
     urf 0.7, c_bl 0.9, minmod (cfd_solve.f90:71-72), Re=100 cavity."""
     c = dict(simpler=1, imat_u=2, imat_p=1, n_bicg=5, n_cg=20, n_grad=2, c_sch=1,
-             urf_u=0.7, urf_p=0.1, reduc_u=0.1, reduc_p=0.1, c_bl=0.9, rho=1.0, mu=0.01)
+             urf_u=0.7, urf_p=0.1, reduc_u=0.1, reduc_p=0.1, c_bl=0.9, rho=1.0, mu=0.01,
+             p_solver=0, p_tol=1e-8, p_cap=20000)
     c.update(kw)
     return Cfg(**c)
 
@@ -252,6 +256,11 @@ class OFlow:
         if rc < 0:
             raise RuntimeError("oracle SIMPLE iteration diverged")
         return h
+
+    def last_pp_system(self):
+        """The last p' solve: right-hand side and solution (cell order)."""
+        nv = self.mesh.m.n_vol
+        return arr(self.f.rhs_last, nv + 1)[1:], arr(self.f.pc_last, nv + 1)[1:]
 
     def fields(self):
         m = self.mesh.m
@@ -304,6 +313,19 @@ class OSys:
         y = np.zeros(self.n + 1)
         lib().fvo_spmv(C.byref(self.s), _d(xx), _d(y))
         return y[1:]
+
+    def singular(self):
+        return bool(lib().fvo_is_singular(C.byref(self.s)))
+
+    def solve_tol(self, b, tol, max_iter=20000):
+        """Solve to a relative L1 residual tol from x = 0 (fvo_pcg_tol).
+        Returns x, iterations, |b|_1 after the projection, singular."""
+        bb = np.concatenate([[0.0], np.asarray(b, np.float64)])
+        x = np.zeros(self.n + 1)
+        its = np.zeros(1, np.int32)
+        r0 = np.zeros(1)
+        sing = lib().fvo_pcg_tol(C.byref(self.s), _d(bb), _d(x), tol, max_iter, _i(its), _d(r0))
+        return x[1:].copy(), int(its[0]), float(r0[0]), bool(sing)
 
     def solve(self, kind, imat, reduc, n_iter, b, x0=None, not_r0=False):
         """b: (n, n_vc).  Returns x (n, n_vc), residual (n, n_vc), res0_n, iters, hist."""

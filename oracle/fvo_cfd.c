@@ -53,13 +53,15 @@ fvo_flow *fvo_flow_new(fvo_mesh *m, const fvo_cfg *cfg) {
   f->A.pl = V1(nv);
   f->A.ng_p = V1(nf);
   f->A.ng_n = V1(nf);
+  f->pc_last = V1(nv);
+  f->rhs_last = V1(nv);
   return f;
 }
 
 void fvo_flow_free(fvo_flow *f) {
   if (!f) return;
   void *ptrs[] = {f->u, f->p, f->dpdx, f->spd, f->frc_dns, f->cs, f->frc_vol, f->frc_c, f->dns_c,
-                  f->vis_c, f->frc_b, f->dns_b, f->vis_b, f->co_num, f->A.pl, f->A.ng_p, f->A.ng_n};
+                  f->vis_c, f->frc_b, f->dns_b, f->vis_b, f->co_num, f->A.pl, f->A.ng_p, f->A.ng_n, f->pc_last, f->rhs_last};
   for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); ++k) free(ptrs[k]);
   free(f);
 }
@@ -417,9 +419,13 @@ int fvo_setup_pp(fvo_flow *F, int it) {
   double r0n[1];
   int its[1];
   for (int i = 1; i <= nv; ++i) pc[i] = 0.0;
-  fvo_solv_cg(&F->A, cf->imat_p, cf->reduc_p, cf->n_cg, 1, pc, 1, rhs + 1, r0n, 1, its, NULL);
+  if (cf->p_solver == 3)
+    fvo_pcg_tol(&F->A, rhs, pc, cf->p_tol, cf->p_cap, its, r0n);
+  else
+    fvo_solv_cg(&F->A, cf->imat_p, cf->reduc_p, cf->n_cg, 1, pc, 1, rhs + 1, r0n, 1, its, NULL);
   F->res0_n_pp = r0n[0];
   F->it_pp = its[0];
+  for (int i = 1; i <= nv; ++i) { F->pc_last[i] = pc[i]; F->rhs_last[i] = rhs[i]; }
   pp_norm(F, it, r0n[0], &F->h_redpp);
   double sr = 0.0;
   for (int i = 1; i <= nv; ++i) sr = sr + rhs[i];
@@ -448,7 +454,14 @@ int fvo_setup_pp(fvo_flow *F, int it) {
   if (cf->simpler) for (int i = 1; i <= nv; ++i) rhs_p[i] = rhs_p[i] + rhs[i];
   /* pressure, :736-759 */
   if (cf->simpler) {
-    fvo_solv_cg(&F->A, cf->imat_p, cf->reduc_p, cf->n_cg, 1, F->p, 1, rhs_p + 1, r0n, 1, its, NULL);
+    if (cf->p_solver == 3) {
+      /* the reference's call adds the solution of A dp = rhs_p to p
+         (phi => vol%p with not_r0, cfd_setup.f90:736-742) */
+      fvo_pcg_tol(&F->A, rhs_p, pc, cf->p_tol, cf->p_cap, its, r0n);
+      for (int i = 1; i <= nv; ++i) F->p[i] = F->p[i] + pc[i];
+    } else {
+      fvo_solv_cg(&F->A, cf->imat_p, cf->reduc_p, cf->n_cg, 1, F->p, 1, rhs_p + 1, r0n, 1, its, NULL);
+    }
     F->res0_n_p = r0n[0];
     F->it_p = its[0];
     pp_norm(F, it, r0n[0], &F->h_red_p);

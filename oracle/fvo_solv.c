@@ -224,3 +224,74 @@ int fvo_solv_bicg(const fvo_sys *A, int imat, double reduc, int n_iter, int n_vc
   free(cc); free(cr); free(r); free(p); free(u); free(v); free(z);
   return 0;
 }
+
+static double sum1(const double *a, int n) {
+  if (g_sum_order == 0) {
+    double s = 0.0;
+    for (int i = 1; i <= n; ++i) s = s + a[i];
+    return s;
+  }
+  return pairwise(a + 1, n);
+}
+
+/* Whether the rows of A sum to zero (a pure-Neumann system: closed domain,
+   or outlets that fix no pressure): sum_i |sum_j a_ij| / |a_ii| <= 1e-12 n.
+   pinned: tests/test_oracle_tol.py (closed cavity and Dirichlet-shifted
+   systems on either side of the threshold). */
+int fvo_is_singular(const fvo_sys *A) {
+  const fvo_mesh *m = A->m;
+  int n = m->n_vol;
+  double *s = V1(n);
+  for (int i = 1; i <= n; ++i) s[i] = A->pl[i];
+  for (int f = 1; f <= m->n_fce; ++f) {
+    int ip = m->f_ip[f], in = m->f_in[f];
+    if (in <= 0) continue;
+    s[ip] = s[ip] + A->ng_p[f];
+    s[in] = s[in] + A->ng_n[f];
+  }
+  double d = 0.0;
+  for (int i = 1; i <= n; ++i) d = d + fabs(s[i]) / fabs(A->pl[i]);
+  free(s);
+  return d / n <= 1e-12;
+}
+
+/* Pressure solve to a relative L1 residual tol from x = 0 (the flow's
+   p_solver 3; the reference itself runs n_cg iterations with reduc 0.1,
+   cfd_setup.f90:662,742).  The solve is the reference's solv_cg
+   (solv_cg.f90:7-87; imat 0, reduc = tol, n_iter = max_iter, not_r0) on
+   the right-hand side scaled to a mean |b| of 1 (the recurrence's
+   alpha = s/(p.q + small) carries an absolute small = 1e-20), its solution
+   scaled back.  A singular system (fvo_is_singular) is solved with the mean
+   of b taken out, and its solution is shifted to mean zero.  b and x are
+   1-based (n_vol+1); res0_n gets |b|_1 after the projection.
+   pinned: the solv_cg part bit for bit against the flang-built reference's
+   solv_cg with reduc = tol (tests/test_ref_cg.py); the scaling and the
+   projection against a dense (pseudo-)inverse on small meshes, zero mean
+   and the residual bound (tests/test_oracle_tol.py). */
+int fvo_pcg_tol(const fvo_sys *A, const double *b, double *x, double tol, int max_iter, int *iters,
+                double *res0_n) {
+  int n = A->m->n_vol;
+  int singular = fvo_is_singular(A);
+  double *bs = V1(n);
+  for (int i = 1; i <= n; ++i) bs[i] = b[i];
+  if (singular) {
+    double sb = sum1(bs, n);
+    for (int i = 1; i <= n; ++i) bs[i] = bs[i] - sb / n;
+  }
+  double l1b = l1(bs, n);
+  double sigma = l1b > 0 ? l1b / n : 1.0;
+  for (int i = 1; i <= n; ++i) bs[i] = (1.0 / sigma) * bs[i];
+  for (int i = 1; i <= n; ++i) x[i] = 0.0;
+  double r0n[1];
+  int its[1] = {0};
+  fvo_solv_cg(A, 0, tol, max_iter, 1, x, 1, bs + 1, r0n, 1, its, NULL);
+  for (int i = 1; i <= n; ++i) x[i] = sigma * x[i];
+  if (singular) {
+    double sx = sum1(x, n);
+    for (int i = 1; i <= n; ++i) x[i] = x[i] - sx / n;
+  }
+  if (iters) *iters = its[0];
+  if (res0_n) *res0_n = l1b;
+  free(bs);
+  return singular;
+}

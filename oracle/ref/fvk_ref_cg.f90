@@ -7,13 +7,14 @@
 ! the module data of solv_data_m as grd_solv.f90 and cfd_setup.f90 leave it
 ! (pl: diagonal; ng: per face ip, in (0 on boundary faces), p = A(ip,in),
 ! n = A(in,ip)), and times one call of solv_cg with imat = 0 (Jacobi) for a
-! fixed number of iterations (reduc = 0: the exit test never passes).
+! fixed number of iterations (reduc = 0: the exit test never passes), or,
+! given REDUC, up to N_ITER iterations with that relative L1 exit test.
 !
 ! File layout (little endian): int32 n_vol, n_fce; int32 ip(n_fce),
 ! in(n_fce) (1-based); float64 p(n_fce), n(n_fce), pl(n_vol), b(n_vol).
 !
-! usage: fvk_ref_cg SYSTEM_FILE N_ITER [PHI_FILE]
-! (PHI_FILE: the solution after N_ITER iterations, float64 stream)
+! usage: fvk_ref_cg SYSTEM_FILE N_ITER [PHI_FILE [REDUC]]
+! (PHI_FILE: the solution, float64 stream)
 program fvk_ref_cg
   use solv_cg_m
   implicit none
@@ -24,10 +25,15 @@ program fvk_ref_cg
   real(8),allocatable::pv(:),nv_(:),b(:)
   type(phi_t),allocatable::phi(:)
   real(wp),allocatable::res0(:,:),res0_n(:)
+  real(wp)::reduc
   integer(8)::t0,t1,rate
 
   call get_command_argument(1,sys_file)
   call get_command_argument(2,arg); read(arg,*)n_iter
+  reduc=zero
+  if(command_argument_count()>=4)then
+    call get_command_argument(4,arg); read(arg,*)reduc
+  end if
   open(10,file=trim(sys_file),access='stream',form='unformatted',status='old')
   read(10)hdr
   nv=hdr(1); nf=hdr(2)
@@ -50,7 +56,7 @@ program fvk_ref_cg
   res0(:,1)=b
 
   call system_clock(t0,rate)
-  call solv_cg(0,zero,n_iter,phi,res0,res0_n,.false.)
+  call solv_cg(0,reduc,n_iter,phi,res0,res0_n,.false.)
   call system_clock(t1)
   print '(a,i0,a,es16.8,a,es16.8)','iterations ',n_iter,' seconds ',dble(t1-t0)/dble(rate), &
     ' res0_n ',res0_n(1)

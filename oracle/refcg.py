@@ -33,9 +33,10 @@ def write_system(path, fce_ip, fce_in, ngp, ngn, pl, b):
         np.asarray(b, np.float64).tofile(f)
 
 
-def run(fce_ip, fce_in, ngp, ngn, pl, b, n_iter, want_x=False, timeout=600):
+def run(fce_ip, fce_in, ngp, ngn, pl, b, n_iter, want_x=False, timeout=600, reduc=None):
     """Runs n_iter iterations of the reference's Jacobi PCG from x = 0 (no
-    early exit).  Returns (seconds inside solv_cg, x or None)."""
+    early exit; with reduc, at most n_iter iterations and the reference's
+    relative L1 exit test).  Returns (seconds inside solv_cg, x or None)."""
     with tempfile.TemporaryDirectory() as d:
         sys_path = os.path.join(d, "system.bin")
         x_path = os.path.join(d, "x.bin")
@@ -44,11 +45,14 @@ def run(fce_ip, fce_in, ngp, ngn, pl, b, n_iter, want_x=False, timeout=600):
         # the stack): the driver runs under the hard stack limit, from a shell
         # that starts it as its own child
         cmd = ["bash", "-c", 'ulimit -s hard 2>/dev/null; "$0" "$@"', DRIVER, sys_path, str(int(n_iter))]
-        cmd += [x_path] if want_x else []
+        if want_x or reduc is not None:
+            cmd += [x_path]
+        if reduc is not None:
+            cmd += [repr(float(reduc))]
         r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
         if r.returncode != 0:
             raise RuntimeError("fvk_ref_cg failed (exit status %d): %s%s" % (r.returncode, r.stdout, r.stderr))
         words = r.stdout.split()
         seconds = float(words[words.index("seconds") + 1])
-        x = np.fromfile(x_path, np.float64) if want_x else None
+        x = np.fromfile(x_path, np.float64) if (want_x or reduc is not None) else None
         return seconds, x

@@ -535,6 +535,13 @@ int fvk_ctx_sync(fvk_ctx *c) {
   return FVK_OK;
 }
 
+int fvk_mesh_rcm(const fvk_mesh *m, int *perm) {
+  if (!m || !perm) { set_error("null argument"); return FVK_ERR_ARG; }
+  const std::vector<int> o = rcm_order(topo_from_mesh(m->m));
+  for (size_t r = 0; r < o.size(); ++r) perm[r] = o[r] + 1;
+  return FVK_OK;
+}
+
 int fvk_ctx_get_perm(const fvk_ctx *c, int *perm) {
   if (!c || !perm) return FVK_ERR_ARG;
   for (int r = 0; r < c->c.n; ++r) perm[r] = c->c.perm[r] + 1;
@@ -803,7 +810,9 @@ int fvk_sys_solve(fvk_ctx *cc, int method, int imat, double tol, int max_iter, i
     }
     status = R.converged ? FVK_OK : FVK_NOT_CONVERGED;
     if (method == FVK_SOLVE_DC32_FACTOR && !R.converged) {
-      // the plain FP64 Jacobi PCG takes over from x = 0
+      // the plain FP64 Jacobi PCG takes over from x = 0 with a budget of
+      // its own (max_iter: the learned solve was allowed what the plain
+      // solver is allowed)
       KS hp;
       GUARD(solve_pcg64(c, 0, tol, max_iter, nullptr, hp));
       info->fallback = 1;

@@ -270,7 +270,7 @@ int pcg_tol(Ctx &c, int imat, const double *b, double *x, double tol, int max_it
 
 // mixed.hip
 struct DcResult {
-  int iters = 0, outer = 0, broken = 0;
+  int iters = 0, outer = 0, broken = 0, breakdowns = 0;
   double res_l1 = 0.0, res_l2 = 0.0, b_l1 = 0.0;
   int inner[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   double res_before[8] = {0, 0, 0, 0, 0, 0, 0, 0};

@@ -856,7 +856,8 @@ static int pp_solve(Flow &F, double *x, double *b) {
       DcResult R;
       if ((rc = dc_solve(c, fac, N.b, x, F.cfg.p_tol, cap, R, nullptr))) return rc;
       if (!R.converged) {
-        // the plain FP64 Jacobi PCG from x = 0 takes over
+        // the plain FP64 Jacobi PCG from x = 0 takes over (with a cap of its
+        // own)
         KS h;
         if ((rc = pcg_tol(c, 0, N.b, x, F.cfg.p_tol, cap, chunk, nullptr, h))) return rc;
         k_ks_store<<<1, 1, 0, c.stream>>>(c.ks, R.b_l1, R.iters + h.iter, h.res_n);
@@ -871,6 +872,25 @@ static int pp_solve(Flow &F, double *x, double *b) {
       return null_end(F, x, N);
     }
   }
+}
+
+__global__ __launch_bounds__(BS) void k_add_to(int n, double *__restrict__ p, const double *__restrict__ d) {
+  const int i = blockIdx.x * BS + threadIdx.x;
+  if (i < n) p[i] = p[i] + d[i];
+}
+
+// The SIMPLER pressure solve.  The reference's call solves A dp = rhs_p
+// with phi => vol%p and not_r0, i.e. it adds its correction to the current
+// pressure (cfd_setup.f90:736-742).  The reference recurrence (p_solver 0)
+// does that in place; a solve to tolerance starts from x = 0, so it solves
+// for the correction in F.pc (free after pp_post1) and adds it.
+static int p_solve_simpler(Flow &F) {
+  if (F.cfg.p_solver == 0) return pp_solve(F, F.p, F.rhs_p);
+  int rc = pp_solve(F, F.pc, F.rhs_p);
+  if (rc) return rc;
+  k_add_to<<<grid_for(F.c->n), BS, 0, F.c->stream>>>(F.c->n, F.p, F.pc);
+  FVK_HIP(hipGetLastError());
+  return 0;
 }
 
 // after the p' solve: norms, relaxed update, p' gradient, velocity and flux
@@ -915,7 +935,7 @@ int pp_setup(Flow &F, int it) {
   if ((rc = pp_solve(F, F.pc, F.rhs))) return rc;
   if ((rc = pp_post1(F, it))) return rc;
   if (F.cfg.simpler) {
-    if ((rc = pp_solve(F, F.p, F.rhs_p))) return rc;
+    if ((rc = p_solve_simpler(F))) return rc;
     if ((rc = pp_post2(F, it))) return rc;
   }
   return pp_final(F);
@@ -947,7 +967,7 @@ static int record_segment(Flow &F, int it, int seg) {
     if ((rc = pp_post1(F, it))) return rc;
     if (F.cfg.simpler) {
       if (host) return 0;
-      if ((rc = pp_solve(F, F.p, F.rhs_p))) return rc;
+      if ((rc = p_solve_simpler(F))) return rc;
     }
   }
   if (F.cfg.simpler && (seg <= 1 || seg == 2)) {
@@ -1029,7 +1049,7 @@ int flow_run_graph(Flow &F, int it0, int n, fvk_outer_hist *h, int &replayed) {
         FVK_HIP(hipGraphLaunch(F.graphs[1], c.stream));
         c.pc_valid = false;
         if (F.cfg.simpler) {
-          if ((rc = pp_solve(F, F.p, F.rhs_p))) return rc;
+          if ((rc = p_solve_simpler(F))) return rc;
           FVK_HIP(hipGraphLaunch(F.graphs[2], c.stream));
           c.pc_valid = false;
         }

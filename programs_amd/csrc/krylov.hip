@@ -792,6 +792,10 @@ int pcg_tol(Ctx &c, int imat, const double *b, double *x, double tol, int max_it
   double tr[4];
   if ((rc = true_residual(c, x, b, c.stage, tr))) return rc;
   const double l1b = tr[2], sigma = l1b > 0 ? l1b / n : 1.0;
+  if (!std::isfinite(l1b)) {
+    set_error("non-finite right-hand side");
+    return FVK_ERR_ARG;
+  }
   double *bs = wk_slot(c, 7);
   const int g = (int)std::min<size_t>(((size_t)n + BS - 1) / BS, 4096);
   k_scale64<<<g, BS, 0, c.stream>>>(n, bs, b, 1.0 / sigma);

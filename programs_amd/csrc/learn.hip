@@ -148,6 +148,19 @@ __global__ __launch_bounds__(BS) void k_geo_feat(FaceIn F, short *g0, v4s *g1, v
   g2[q] = pack4(b);
 }
 
+__device__ v4s features4(const FaceIn &F, int q, int g);
+
+// The features as the network reads them, as float: X[12 q + k]
+__global__ __launch_bounds__(BS) void k_feat_dump(FaceIn F, float *__restrict__ X) {
+  const int q = blockIdx.x * BS + threadIdx.x;
+  if (q >= F.nfi) return;
+  for (int g = 0; g < 3; ++g) {
+    const v4s v = features4(F, q, g);
+    for (int k = 0; k < 4; ++k)
+      X[(size_t)12 * q + 4 * g + k] = __builtin_bit_cast(float, (unsigned)(unsigned short)v[k] << 16);
+  }
+}
+
 // Features 4g .. 4g+3 of interior face q (the lane's share of the B operand):
 // the coefficient ratios of group 0 from the current matrix, the rest stored
 __device__ v4s features4(const FaceIn &F, int q, int g) {
@@ -738,6 +751,31 @@ int fvk_model_set(fvk_model *m, const float *theta) {
 int fvk_factor_build(fvk_ctx *cc, const fvk_model *mc) {
   if (!cc || !mc) { set_error("null argument"); return FVK_ERR_ARG; }
   return learned_factor(cc->c, const_cast<fvk_model *>(mc));
+}
+
+int fvk_factor_features(fvk_ctx *cc, float *X) {
+  if (!cc || !X) { set_error("null argument"); return FVK_ERR_ARG; }
+  Ctx &c = cc->c;
+  GUARD_L(mix_build(c));
+  GUARD_L(geo_build(c));
+  const FaceIn F = face_in(c);
+  // the loss scratch holds 24 n doubles: 12 floats per interior face fit
+  if ((size_t)6 * F.nfi > (size_t)24 * c.n) { set_error("feature scratch too small"); return FVK_ERR_STATE; }
+  float *d = (float *)c.wk;
+  std::vector<float> h((size_t)12 * F.nfi);
+  if (F.nfi) {
+    k_feat_dump<<<grid_for(F.nfi), BS, 0, c.stream>>>(F, d);
+    FVK_HIP(hipGetLastError());
+    FVK_HIP(hipMemcpyAsync(h.data(), d, h.size() * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+  }
+  FVK_HIP(hipStreamSynchronize(c.stream));
+  int q = 0;
+  for (int f = 0; f < c.nf; ++f) {
+    float *o = X + (size_t)12 * f;
+    if (c.h_fin[f] >= 0) { for (int k = 0; k < 12; ++k) o[k] = h[(size_t)12 * q + k]; ++q; }
+    else { for (int k = 0; k < 12; ++k) o[k] = 0.f; }
+  }
+  return FVK_OK;
 }
 
 int fvk_factor_get(fvk_ctx *cc, float *l, float *sig) {

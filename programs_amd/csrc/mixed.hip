@@ -51,9 +51,11 @@ namespace fvk {
 typedef _Float16 lval_t;
 // t = Delta^2 (I + L^T) r as the update kernel hands it to the direction
 // kernel: FP16, divided by a scale both kernels derive from the same
-// committed scalars (the decision's r.z = sum Delta^2 u^2: 64 sqrt(r.z / n),
-// so that |t| / scale stays below Delta_max sqrt(n) / 64 whatever the
-// residual's size and shape).
+// committed scalars: the decision's r.z = sum Delta^2 u^2, i.e. that of the
+// residual before the update, 64 sqrt(r.z / n).  With r'.z' the new
+// residual's, |t| / scale <= Delta_max sqrt(n) / 64 * sqrt(r'.z' / r.z)
+// whatever the residual's shape; PCG does not make r.z decrease
+// monotonically, so a step where r.z grows enough can overflow FP16.
 // The first direction of a solve reads the start's t in FP32.
 typedef _Float16 t16_t;
 
@@ -81,10 +83,13 @@ __device__ __forceinline__ float t_scale(const KS &k, int n) {
   const float s = 64.f * sqrtf((float)k.s / (float)n);
   return s > 0.f && s < 1e30f ? s : 1.f;
 }
-// FP16, rounded to nearest.  The scale keeps |t| / scale below
-// Delta_max sqrt(n) / 64 < 65504 for n < 2.8e8 rows even when all of r.z sits
-// in one row; beyond, an overflow turns into a breakdown of the inner solve
-// (and the plain FP64 solver takes over).  (Rounding toward zero, which
+// FP16, rounded to nearest.  Without growth of r.z the scale keeps |t| /
+// scale below Delta_max sqrt(n) / 64 < 65504 for n < 2.8e8 rows even when
+// all of r.z sits in one row.  An overflow makes the next p.q non-finite,
+// which the update kernel's decision takes as a breakdown before applying
+// the step: the inner solve ends with a finite correction, which is kept,
+// and the defect correction restarts from that x (after repeated
+// breakdowns the caller's fallback takes over).  (Rounding toward zero, which
 // saturates, cost 13 % more iterations; a clamp before the conversion cost
 // the update kernel registers and 15 % of its time.)
 __device__ __forceinline__ t16_t to_t16(float v) { return (t16_t)v; }
@@ -505,7 +510,7 @@ __device__ __forceinline__ bool mx_alpha(KS *cur, SelLoads<1> &S, double &alpha)
   double tq[1];
   S.finish(tq);
   alpha = cur->s / (tq[0] + SMALL);
-  if (!(tq[0] > 0.0) || !isfinite(alpha)) {
+  if (!(tq[0] > 0.0) || !isfinite(tq[0]) || !isfinite(alpha)) {
     if (blockIdx.x == 0 && threadIdx.x == 0) cur->pad = 1;
     return false;
   }
@@ -701,7 +706,8 @@ __global__ __launch_bounds__(BS) void k_mx_add(int n, double *__restrict__ x, co
                                                const double *__restrict__ diag, double sigma) {
   const int i = blockIdx.x * BS + threadIdx.x;
   if (i >= n) return;
-  const float e = p ? ev[i] + alpha * p[i] : ev[i];
+  // no pending step: p is not read (after a breakdown it may hold inf)
+  const float e = p && alpha != 0.f ? ev[i] + alpha * p[i] : ev[i];
   x[i] += sigma * (double)e / sqrt(diag[i]);
 }
 
@@ -1061,6 +1067,8 @@ static int inner_solve(Ctx &c, bool fac, int n_iter, double reduc, InnerCarry &c
   return 0;
 }
 
+static const int kMaxBreakdowns = 2;
+
 int dc_solve(Ctx &c, bool use_factor, const double *b, double *x, double tol, int max_iter, DcResult &res,
              KTimer *T) {
   int rc = mix_build(c);
@@ -1089,6 +1097,10 @@ int dc_solve(Ctx &c, bool use_factor, const double *b, double *x, double tol, in
     double h[4];
     FVK_HIP(hipMemcpyAsync(h, X.tot, sizeof h, hipMemcpyDeviceToHost, c.stream));
     FVK_HIP(hipStreamSynchronize(c.stream));
+    if (outer == 0 && !std::isfinite(h[2])) {
+      set_error("non-finite right-hand side");
+      return FVK_ERR_ARG;
+    }
     res.b_l1 = h[2];
     res.res_l1 = h[2] > 0 ? h[0] / h[2] : h[0];
     res.res_l2 = h[3] > 0 ? std::sqrt(h[1] / h[3]) : std::sqrt(h[1]);
@@ -1124,7 +1136,10 @@ int dc_solve(Ctx &c, bool use_factor, const double *b, double *x, double tol, in
       res.res_before[outer] = res.res_l1;
     }
     res.iters += its;
-    res.broken |= broken;
+    // a breakdown (p.q <= 0 or non-finite) ends the inner solve with the
+    // finite correction made so far, which is kept; the next correction
+    // step restarts from that x.  After kMaxBreakdowns the solve gives up.
+    if (broken && ++res.breakdowns > kMaxBreakdowns) res.broken = 1;
     k_mx_add<<<grid_for(n), BS, 0, c.stream>>>(n, x, fv(c, V_E), use_factor ? fv(c, V_P0) : nullptr,
                                                (float)alpha_pend, c.diag, sigma);
     FVK_HIP(hipGetLastError());

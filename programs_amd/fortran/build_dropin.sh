@@ -37,4 +37,9 @@ unit "$HERE/fvk_cfd_setup.f90"
 unit "$ROOT/oracle/ref/fvk_ref_driver.f90"
 $FC $FFLAGS -o "$OUT/fvk_ref_driver" $objs -L"$ROOT/programs_amd" -lfvk \
     -Wl,-rpath,'$ORIGIN/../../../programs_amd'
+# stamp: the hash of the sources this executable was built from; the GPU test
+# refuses an executable whose stamp differs from the tree it runs in
+(cd "$ROOT" && sha256sum programs_amd/fortran/fvk_c.f90 programs_amd/fortran/fvk_solv.f90 \
+   programs_amd/fortran/fvk_grd_solv.f90 programs_amd/fortran/fvk_cfd_setup.f90 \
+   oracle/ref/fvk_ref_driver.f90 | sha256sum | cut -d' ' -f1) > "$OUT/sources.sha256"
 echo "built $OUT/fvk_ref_driver"

@@ -18,8 +18,11 @@ Errors raise FvkError carrying the library's status and message.  There is
 no fallback: without libfvk.so (or without a GPU for device calls) the calls
 fail.
 """
+import atexit
 import ctypes as C
 import os
+import sys
+import weakref
 
 import numpy as np
 
@@ -79,6 +82,29 @@ class OuterHist(C.Structure):
 
 _lib = None
 
+# Live handles by kind.  At interpreter exit they are destroyed by an atexit
+# hook, dependants first (flows, models, contexts, meshes), while the HIP
+# runtime is still up; finalizers that run later (module teardown, garbage
+# collection during shutdown) make no library call at all.
+_LIVE = {k: weakref.WeakSet() for k in ("flow", "model", "ctx", "mesh")}
+
+
+def _track(kind, obj):
+    _LIVE[kind].add(obj)
+
+
+def _release_all():
+    for kind in ("flow", "model", "ctx", "mesh"):
+        for obj in list(_LIVE[kind]):
+            try:
+                obj._release()
+            except Exception:
+                pass
+
+
+def _finalizing():
+    return sys.is_finalizing()
+
 
 def lib():
     """Load libfvk.so (built in place by programs_amd/build.py)."""
@@ -101,6 +127,7 @@ def lib():
         "fvk_ctx_create_arrays": [C.c_int] * 4 + [_ip] * 6 + [_dp] * 9 + [C.c_int, C.c_int, C.POINTER(_vp)],
         "fvk_ctx_sync": [_vp],
         "fvk_ctx_get_perm": [_vp, _ip],
+        "fvk_mesh_rcm": [_vp, _ip],
         "fvk_ctx_levels": [_vp, _ip, _ip],
         "fvk_set_matrix": [_vp, _dp, _dp, _dp],
         "fvk_solv_precond": [_vp, C.c_int, _dp, _dp],
@@ -119,6 +146,7 @@ def lib():
         "fvk_factor_set": [_vp, _fp, _fp],
         "fvk_factor_apply": [_vp, _dp, _dp],
         "fvk_factor_get": [_vp, _fp, _fp],
+        "fvk_factor_features": [_vp, _fp],
         "fvk_model_nparam": [],
         "fvk_model_create": [C.c_int, C.c_uint64, C.POINTER(_vp)],
         "fvk_model_get": [_vp, _fp],
@@ -150,6 +178,7 @@ def lib():
         getattr(L, name).argtypes = [_vp]
         getattr(L, name).restype = None
     _lib = L
+    atexit.register(_release_all)
     return L
 
 
@@ -178,6 +207,7 @@ class Mesh:
 
     def __init__(self, handle):
         self.h = handle
+        _track("mesh", self)
         info = MeshInfo()
         _check(lib().fvk_mesh_get_info(self.h, C.byref(info)))
         self.info = info
@@ -224,16 +254,23 @@ class Mesh:
                                                                          "fce_wfp")]))
         return o
 
-    def free(self):
+    def rcm(self):
+        """Reverse Cuthill-McKee cell order (1-based reference cell ids by
+        row), as a context built with rcm=True uses it; host only."""
+        p = np.zeros(self.n_vol, np.int32)
+        _check(lib().fvk_mesh_rcm(self.h, _i(p)))
+        return p
+
+    def _release(self):
         if self.h:
             lib().fvk_mesh_free(self.h)
             self.h = None
 
+    free = _release
+
     def __del__(self):
-        try:
-            self.free()
-        except Exception:
-            pass
+        if not _finalizing():
+            self._release()
 
 
 # ---------------------------------------------------------------- context
@@ -242,6 +279,8 @@ class Ctx:
 
     def __init__(self, handle, n_vol, n_fce, n_dx, n_bf):
         self.h = handle
+        self._flows = weakref.WeakSet()  # flows on this context: released first
+        _track("ctx", self)
         self.n_vol, self.n_fce, self.n_dx, self.n_bf = n_vol, n_fce, n_dx, n_bf
 
     def perm(self):
@@ -257,16 +296,18 @@ class Ctx:
     def sync(self):
         _check(lib().fvk_ctx_sync(self.h))
 
-    def destroy(self):
+    def _release(self):
         if self.h:
+            for f in list(self._flows):
+                f._release()
             lib().fvk_ctx_destroy(self.h)
             self.h = None
 
+    destroy = _release
+
     def __del__(self):
-        try:
-            self.destroy()
-        except Exception:
-            pass
+        if not _finalizing():
+            self._release()
 
 
 def grd_solv(mesh: Mesh, device=0, rcm=False):
@@ -425,12 +466,21 @@ def factor_get(ctx: Ctx):
     return l, sig
 
 
+def factor_features(ctx: Ctx):
+    """The network's twelve inputs per face as it reads them (bfloat16
+    values), (n_fce, 12), reference face order, zero rows on boundary faces."""
+    X = np.zeros((ctx.n_fce, 12), np.float32)
+    _check(lib().fvk_factor_features(ctx.h, X.ctypes.data_as(_fp)))
+    return X
+
+
 class Model:
     """The per-face MLP of the learned factor, resident on one GPU."""
 
     def __init__(self, seed=0, device=0):
         self.h = _vp()
         _check(lib().fvk_model_create(device, seed, C.byref(self.h)))
+        _track("model", self)
         self.nparam = lib().fvk_model_nparam()
 
     def get(self):
@@ -463,10 +513,16 @@ class Model:
         _check(lib().fvk_model_train(self.h, ctx.h, steps, lr, _d(losses)))
         return losses
 
-    def destroy(self):
+    def _release(self):
         if self.h:
             lib().fvk_model_destroy(self.h)
             self.h = None
+
+    destroy = _release
+
+    def __del__(self):
+        if not _finalizing():
+            self._release()
 
 
 def sys_x(ctx: Ctx):
@@ -513,6 +569,8 @@ class Flow:
         self.cfg = cfg
         self.h = _vp()
         _check(lib().fvk_flow_create(ctx.h, C.byref(cfg), C.byref(self.h)))
+        _track("flow", self)
+        ctx._flows.add(self)
 
     def set_bnd_u(self, code, u):
         u3 = np.zeros(3)
@@ -574,7 +632,13 @@ class Flow:
         _check(lib().fvk_flow_get_matrix(self.h, _d(pl), _d(gp), _d(gn)))
         return pl, gp, gn
 
-    def destroy(self):
+    def _release(self):
         if self.h:
             lib().fvk_flow_destroy(self.h)
             self.h = None
+
+    destroy = _release
+
+    def __del__(self):
+        if not _finalizing():
+            self._release()

@@ -8,8 +8,11 @@ summation-order sensitivity).
 
 The executable is built in the development container by
 programs_amd/fortran/build_dropin.sh (it links reference objects, so it is
-not part of the repository); the test is skipped where it is absent.
+not part of the repository) and travels with the tree.  The test fails when
+it is absent or was built from other sources than the tree's (the build
+writes a hash of its Fortran sources beside it).
 """
+import hashlib
 import os
 import subprocess
 
@@ -25,11 +28,29 @@ from tests.test_gpu_parity import b3_for, sens_tol, oracle_flow_fields
 pytestmark = pytest.mark.gpu
 
 DRIVER = os.path.join(ROOT, "oracle", "_ref", "dropin", "fvk_ref_driver")
+STAMP = os.path.join(ROOT, "oracle", "_ref", "dropin", "sources.sha256")
+SOURCES = ["programs_amd/fortran/fvk_c.f90", "programs_amd/fortran/fvk_solv.f90",
+           "programs_amd/fortran/fvk_grd_solv.f90", "programs_amd/fortran/fvk_cfd_setup.f90",
+           "oracle/ref/fvk_ref_driver.f90"]
 
 
-@pytest.mark.skipif(not os.path.exists(DRIVER), reason="drop-in executable not built here")
+def sources_hash():
+    """The hash build_dropin.sh writes: sha256 of `sha256sum` over SOURCES."""
+    lines = "".join("%s  %s\n" % (hashlib.sha256(open(os.path.join(ROOT, s), "rb").read()).hexdigest(), s)
+                    for s in SOURCES)
+    return hashlib.sha256(lines.encode()).hexdigest()
+
+
+def test_dropin_executable_is_current():
+    assert os.path.exists(DRIVER), "drop-in executable missing: run programs_amd/fortran/build_dropin.sh"
+    assert os.path.exists(STAMP), "drop-in executable has no source stamp: rebuild it"
+    assert open(STAMP).read().strip() == sources_hash(), \
+        "drop-in executable was built from other Fortran sources: rebuild it"
+
+
 @pytest.mark.parametrize("name", G.CASES)
 def test_fortran_dropin_matches_reference(name, tmp_path):
+    test_dropin_executable_is_current()
     rec = G.load(name)
     spec = rec["spec"]
     src = G.mesh_inputs(rec)

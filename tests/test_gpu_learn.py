@@ -176,3 +176,105 @@ def test_training_lowers_loss_and_iterations():
             ctx.destroy()
     finally:
         model.destroy()
+
+
+def bf16_ulp_bound(x):
+    """Half a bfloat16 ulp of |x| (round to nearest) plus the float32
+    rounding the GPU applies before it, as an absolute bound."""
+    ax = np.abs(np.asarray(x, np.float64))
+    e = np.floor(np.log2(np.maximum(ax, 1e-30)))
+    return 2.0 ** (e - 8) * 1.0001 + 1e-30
+
+
+@pytest.mark.parametrize("kind", ["hex", "tri"])
+def test_gpu_features_match_oracle(kind):
+    """The network's inputs on the GPU (fvk_factor_features) are the
+    oracle's features rounded to bfloat16: within one bfloat16 rounding
+    (plus the float32 cube root and logarithm the GPU takes) of the FP64
+    values, face by face.  The oracle's features are pinned by closed forms
+    on the uniform hex (tests/test_learn_oracle.py)."""
+    if kind == "hex":
+        fvk, m, ctx, S, inner = box_case(8, 0.2, 0x5EED0061)
+    else:
+        fvk, m, ctx, S, inner = delaunay_case(900, 0x5EED0062)
+    Xg = fvk.factor_features(ctx)
+    assert not np.any(Xg[~inner])
+    Xg = Xg[inner].astype(np.float64)
+    Xo = L.features(S)
+    err = np.abs(Xg - Xo)
+    bound = 2.0 * bf16_ulp_bound(Xo) + 1e-6 * np.abs(Xo)
+    bad = err > bound
+    assert not np.any(bad), (np.argwhere(bad)[:5], Xg[bad][:5], Xo[bad][:5])
+    # each of the twelve features varies on these meshes (nothing constant
+    # by mistake), except the third normal component in 2D
+    var = [k for k in range(12) if np.std(Xo[:, k]) > 0]
+    assert len(var) >= (12 if kind == "hex" else 11), var
+    ctx.destroy()
+
+
+def test_gpu_features_closed_form_on_uniform_hex():
+    """Uniform hex, a_f = s/l_pn (sys_random with sigma 0): an
+    interior-interior face reads bf16(1/6) x 3, 1, e_i, 1, 1/2, 0, 0, 0."""
+    fvk = require_lib()
+    n = 6
+    m = fvk.Mesh.box(n, n, n)
+    ctx = fvk.grd_solv(m)
+    fvk.sys_random(ctx, 1, 0.0)
+    pl, a, _ = fvk.sys_get(ctx)
+    t = m.topology()
+    inner = t["fce_in"] > 0
+    X = fvk.factor_features(ctx)[inner]
+    k = np.zeros(m.n_vol)
+    np.add.at(k, t["fce_ip"][inner] - 1, 1)
+    np.add.at(k, t["fce_in"][inner] - 1, 1)
+    full = (k[t["fce_ip"][inner] - 1] == 6) & (k[t["fce_in"][inner] - 1] == 6)
+    sixth = float(L.bf16(np.float32(1.0 / 6.0)))
+    assert full.sum() == 3 * (n - 2) ** 2 * (n - 3)
+    Xi = X[full]
+    assert np.all(Xi[:, :3] == sixth)
+    assert np.all(Xi[:, 3] == 1.0) and np.all(Xi[:, 7] == 1.0) and np.all(Xi[:, 8] == 0.5)
+    # zero up to the rounding of the mesh's own coordinates
+    assert np.allclose(np.sort(np.abs(Xi[:, 4:7]), 1), [0.0, 0.0, 1.0], rtol=0, atol=1e-6)
+    assert np.allclose(Xi[:, 9:12], 0.0, rtol=0, atol=1e-6)
+    ctx.destroy()
+
+
+def test_network_against_closed_form_fsai():
+    """What the network buys, and how close it comes to the method it
+    learns: on held-out systems (other mesh and coefficient seeds) the
+    trained network's factor needs at most 10% more FP32-inner iterations
+    than FSAI computed in closed form by the oracle on the same pattern
+    (the Kaporin minimiser), and at most 80% of Jacobi's; its Kaporin
+    measure lies between Jacobi's (0) and FSAI's."""
+    fvk = require_lib()
+    model = fvk.Model(seed=9)
+    try:
+        for seed in (0x91, 0x92, 0x93):
+            _, m, ctx, S, inner = box_case(14, 0.2, seed)
+            model.train(ctx, 150, lr=3e-3)
+            ctx.destroy()
+        for seed in (0xA1, 0xA2):
+            _, m, ctx, S, inner = box_case(16, 0.2, seed)
+            jac = fvk.sys_solve(ctx, fvk.SOLVE_DC32, 0, 1e-8)
+            model.build_factor(ctx)
+            J_net = model.loss(ctx)
+            lrn = fvk.sys_solve(ctx, fvk.SOLVE_DC32_FACTOR, 0, 1e-8)
+            lf, sf = L.fsai_closed_form(S)
+            l = np.zeros(ctx.n_fce, np.float32)
+            sig = np.zeros(ctx.n_fce, np.float32)
+            l[inner] = lf
+            sig[inner] = sf
+            J_fsai = L.kaporin(S, lf, sf, grad=False)
+            J_fsai = J_fsai[0] if isinstance(J_fsai, tuple) else J_fsai
+            fvk.factor_set(ctx, l, sig)
+            fs = fvk.sys_solve(ctx, fvk.SOLVE_DC32_FACTOR, 0, 1e-8)
+            print("held-out", hex(seed), "jacobi", jac["iters"], "network", lrn["iters"], "fsai", fs["iters"],
+                  "J network %.5f fsai %.5f" % (J_net, J_fsai))
+            for r in (lrn, fs):
+                assert r["status"] == fvk.FVK_OK and r["fallback"] == 0 and r["res_true_l1"] < 1e-8
+            assert lrn["iters"] <= 1.10 * fs["iters"]
+            assert lrn["iters"] <= 0.8 * jac["iters"]
+            assert J_fsai <= J_net < 0.0
+            ctx.destroy()
+    finally:
+        model.destroy()

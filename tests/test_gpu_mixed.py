@@ -291,3 +291,55 @@ def test_zero_rhs_and_short_budget():
             assert 1e-8 < true_res(A, x, b) < 1.0
     finally:
         om.free()
+
+
+def test_dc32_factor_breakdown_falls_back_to_plain_pcg():
+    """An indefinite system (the Laplacian shifted below its first nonzero
+    eigenvalue, so the near-constant mode is negative) with a right-hand side
+    rich in that mode: p.q < 0 in the first inner iteration, a breakdown.
+    The defect correction restarts twice from its finite x, breaks down
+    again, and the plain FP64 Jacobi PCG takes over (the reference
+    recurrence allows p.q < 0 and converges here in about 100 iterations):
+    FVK_FALLBACK with the FP64 residual met."""
+    fvk, m, ctx, om, S, A, b = setup(8, 0.2, 0x57)
+    try:
+        Ad = S.dense()
+        ev = np.linalg.eigvalsh(Ad)
+        shift = 0.3 * ev[1]
+        pl, a, _ = fvk.sys_get(ctx)
+        b2 = np.random.default_rng(3).standard_normal(len(pl)) + 1.0
+        fvk.sys_set(ctx, pl - shift, a, b2)
+        A2 = fvo.OSys(om, pl - shift, -a, -a)
+        assert np.linalg.eigvalsh(Ad - shift * np.eye(len(pl)))[0] < 0
+        fvk.factor_set(ctx, np.zeros(ctx.n_fce, np.float32), np.zeros(ctx.n_fce, np.float32))
+        out = fvk.sys_solve(ctx, fvk.SOLVE_DC32_FACTOR, 0, 1e-8, max_iter=2000)
+        x = fvk.sys_x(ctx)
+        print("indefinite:", out["status"], out["fallback"], out["iters"], out["outer"], true_res(A2, x, b2))
+        assert out["fallback"] == 1 and out["status"] == fvk.FVK_FALLBACK
+        assert out["outer"] >= 2  # the inner solve was restarted after a breakdown
+        assert true_res(A2, x, b2) < 2e-8
+    finally:
+        om.free()
+
+
+def test_non_finite_rhs_is_an_argument_error():
+    """A NaN or infinite right-hand side is rejected (FVK_ERR_ARG) by every
+    method, before any iteration."""
+    fvk, m, ctx, om, S, A, b = setup(6, 0.2, 0x58)
+    try:
+        fvk.factor_set(ctx, np.zeros(ctx.n_fce, np.float32), np.zeros(ctx.n_fce, np.float32))
+        pl, a, _ = fvk.sys_get(ctx)
+        for bad in (np.nan, np.inf):
+            b2 = b.copy()
+            b2[5] = bad
+            fvk.sys_set(ctx, pl, a, b2)
+            for method in (fvk.SOLVE_PCG64, fvk.SOLVE_DC32, fvk.SOLVE_DC32_FACTOR):
+                with pytest.raises(fvk.FvkError) as e:
+                    fvk.sys_solve(ctx, method, 0, 1e-8)
+                assert e.value.status == -1 and "non-finite" in str(e.value)
+        # the context stays usable
+        fvk.sys_set(ctx, pl, a, b)
+        out = fvk.sys_solve(ctx, fvk.SOLVE_DC32, 0, 1e-8)
+        assert out["status"] == fvk.FVK_OK and true_res(A, fvk.sys_x(ctx), b) < 1e-8
+    finally:
+        om.free()

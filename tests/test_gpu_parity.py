@@ -216,3 +216,17 @@ def test_simple_matches_oracle_box(n, jit, variant, n_outer):
         for k in ("u_cell", "p", "dpdx", "spd"):
             assert_rel(s[k].ravel(), F[k].ravel(), TOL, k)
         ctx.destroy()
+
+
+def test_context_uses_the_rcm_order():
+    """A context built with rcm=True puts the cells in the order
+    fvk_mesh_rcm gives (pinned on the host by tests/test_rcm.py); without
+    it, in the reference's order."""
+    fvk = require_lib()
+    m = fvk.Mesh.box(9, 9, 9, jitter=0.1, seed=0x5EED0073)
+    c1 = fvk.grd_solv(m, rcm=True)
+    c2 = fvk.grd_solv(m)
+    assert np.array_equal(c1.perm(), m.rcm())
+    assert np.array_equal(c2.perm(), np.arange(1, m.n_vol + 1))
+    c1.destroy()
+    c2.destroy()

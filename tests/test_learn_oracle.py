@@ -188,3 +188,82 @@ def test_bf16_rounding():
     np.testing.assert_array_equal(y[:3], np.array([1.0, 1.0, 1.0078125], np.float32))
     assert y[5] == 0.0
     assert np.all(np.abs(y[3:5] - x[3:5]) <= np.abs(x[3:5]) * 2.0 ** -8)
+
+
+def hex_system(n, L=1.0, scale=1.0):
+    """Uniform n^3 hex box of side L; a_f = scale * s/l_pn (pure Neumann)."""
+    om = fvo.OMesh.box(n, n, n, L=(L, L, L))
+    t, g = om.topology(), om.geometry()
+    om.free()
+    m = t["fce_in"] > 0
+    a = np.zeros(len(m))
+    a[m] = scale * g["fce_s"][m] / g["fce_lpn"][m]
+    pl = np.zeros(t["n_vol"])
+    np.add.at(pl, t["fce_ip"][m] - 1, a[m])
+    np.add.at(pl, t["fce_in"][m] - 1, a[m])
+    return L_.System.from_mesh(t, g, pl, -a), t, g
+
+
+L_ = L
+
+
+@pytest.mark.parametrize("n,side", [(5, 1.0), (6, 3.7)])
+def test_features_closed_form_on_uniform_hex(n, side):
+    """On a uniform hex box with a_f = s/l_pn every feature has a closed form:
+    a/sqrt(a_pp a_nn) = 1/sqrt(k_p k_n), a/a_pp = 1/k_p, a/a_nn = 1/k_n with
+    k_c the number of interior neighbours of c; s/h^2 = 1; the normal is the
+    unit vector from owner to neighbour centre; l_pn/h = 1; w_fp = 1/2;
+    |dx_p|/h = |dx_n|/h = 0 (the auxiliary poles are the cell centres);
+    log(|V_p|/|V_n|) = 0.  Interior-interior faces are
+    (1/6, 1/6, 1/6, 1, e_i, 1, 1/2, 0, 0, 0).  Independent of the box's size
+    (dimensionless: s/h instead of s/h^2 fails here) and of a scaling of
+    the matrix."""
+    S, t, g = hex_system(n, side)
+    X = L_.features(S)
+    k = np.zeros(S.N)
+    np.add.at(k, S.p, 1)
+    np.add.at(k, S.n, 1)
+    kp, kn = k[S.p], k[S.n]
+    assert np.allclose(X[:, 0], 1.0 / np.sqrt(kp * kn), rtol=1e-12, atol=0)
+    assert np.allclose(X[:, 1], 1.0 / kp, rtol=1e-12, atol=0)
+    assert np.allclose(X[:, 2], 1.0 / kn, rtol=1e-12, atol=0)
+    assert np.allclose(X[:, 3], 1.0, rtol=1e-12, atol=0)
+    c = g["vol_x"]
+    e = (c[S.n] - c[S.p]) / (side / n)
+    assert np.allclose(X[:, 4:7], e, rtol=0, atol=1e-12)
+    assert np.allclose(np.abs(X[:, 4:7]).sum(1), 1.0)
+    assert np.allclose(X[:, 7], 1.0, rtol=1e-12, atol=0)
+    assert np.allclose(X[:, 8], 0.5, rtol=1e-12, atol=0)
+    assert np.allclose(X[:, 9:11], 0.0, rtol=0, atol=1e-12)
+    assert np.allclose(X[:, 11], 0.0, rtol=0, atol=1e-12)
+    inner = (kp == 6) & (kn == 6)
+    assert inner.sum() == 3 * (n - 2) ** 2 * (n - 3)
+    assert np.allclose(X[inner, :4], [1 / 6, 1 / 6, 1 / 6, 1.0], rtol=1e-12)
+    # boundary-adjacent: an owner on a box face (5 interior neighbours) next
+    # to an interior cell
+    bnd = (kp == 5) & (kn == 6)
+    assert bnd.any()
+    assert np.allclose(X[bnd, :3], [1 / np.sqrt(30), 1 / 5, 1 / 6], rtol=1e-12)
+    # scaling the matrix changes nothing
+    S2, _, _ = hex_system(n, side, scale=7.5)
+    assert np.allclose(L_.features(S2), X, rtol=1e-12, atol=1e-15)
+
+
+def test_features_scale_invariant_on_jittered_mesh():
+    """The features are dimensionless: the same jittered mesh scaled by 10
+    (with a matrix scaled likewise) gives the same features."""
+    def case(side):
+        om = fvo.OMesh.box(5, 5, 5, jitter=0.2, seed=0x5EED0051, L=(side, side, side))
+        t, g = om.topology(), om.geometry()
+        om.free()
+        m = t["fce_in"] > 0
+        rng = np.random.default_rng(1)
+        a = np.zeros(len(m))
+        a[m] = g["fce_s"][m] / g["fce_lpn"][m] * np.exp(rng.standard_normal(int(m.sum())))
+        pl = np.zeros(t["n_vol"])
+        np.add.at(pl, t["fce_ip"][m] - 1, a[m])
+        np.add.at(pl, t["fce_in"][m] - 1, a[m])
+        return L_.features(L_.System.from_mesh(t, g, pl, -a))
+    X1, X2 = case(1.0), case(10.0)
+    assert np.allclose(X1, X2, rtol=1e-9, atol=1e-12)
+    assert np.std(X1[:, 9]) > 0 and np.std(X1[:, 11]) > 0

@@ -39,3 +39,38 @@ def test_reference_pcg_driver_matches_oracle(n, iters):
         assert np.max(np.abs(x_ref)) > 0
     finally:
         om.free()
+
+
+def seqsum(a):
+    """Sequential left-to-right sum (the order of the reference's sum())."""
+    return float(np.cumsum(np.asarray(a, np.float64))[-1])
+
+
+@pytest.mark.parametrize("singular", [False, True])
+def test_reference_pcg_to_tolerance_matches_oracle_mode(singular):
+    """The oracle's solve to a tolerance (fvo_pcg_tol) is the reference's
+    solv_cg with reduc = tol on the right-hand side scaled to a mean |b| of
+    1 (after taking out its mean when the rows sum to zero), the solution
+    scaled back (and shifted to mean zero): bit for bit."""
+    n, tol = 8, 1e-9
+    om, ip, inn, a, pl, b = lognormal_system(n, 0x5EED0031)
+    try:
+        if singular:
+            pl = pl / (1.0 + 1e-3)
+            np.add.at(pl, [0], 0.0)
+        A = fvo.OSys(om, pl, -a, -a)
+        assert A.singular() == singular
+        x_orc, its, r0, sing = A.solve_tol(b, tol)
+        assert sing == singular and 1 < its < 20000
+        nv = len(b)
+        bp = b - seqsum(b) / nv if singular else b.copy()
+        l1b = seqsum(np.abs(bp))
+        assert r0 == l1b
+        sigma = l1b / nv
+        _, x_ref = refcg.run(ip, inn, -a, -a, pl, (1.0 / sigma) * bp, 20000, reduc=tol)
+        x_ref = sigma * x_ref
+        if singular:
+            x_ref = x_ref - seqsum(x_ref) / nv
+        assert np.array_equal(x_ref, x_orc)
+    finally:
+        om.free()
