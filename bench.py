@@ -118,26 +118,77 @@ def cpu_baseline(ctx, fvk, n, jitter, gpu_iters, sample_iters):
                 per_iter_ms=per_it * 1e3)
 
 
-def cpu_simple(args):
-    """The oracle's SIMPLE outer iteration (plain C, one core) on the same
-    cavity as the SIMPLE leg: iterations 1-2 untimed, iteration 3 timed."""
+def host_info():
+    """Host cores this process may use, and the CPU model."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return dict(cpu_model=model, nproc=os.cpu_count(), usable_cores=usable)
+
+
+def cpu_baseline_all_cores(ctx, fvk, n, jitter, gpu_iters, sample_iters):
+    """The same Jacobi-PCG recurrence on every usable host core (the
+    oracle's OpenMP port: row-gather product, per-thread reductions; at most
+    16 threads, the host share of one GPU), a bounded number of iterations
+    scaled to the GPU path's count."""
+    from oracle import fvo
+    info = host_info()
+    threads = max(1, min(16, info["usable_cores"]))
+    om = fvo.OMesh.box(n, n, n, jitter=jitter, seed=0x5EED0002)
+    pl, a, b = fvk.sys_get(ctx)
+    A = fvo.OSys(om, pl, -a, -a)
+    sec, _ = A.pcg_omp(b, sample_iters, threads)
+    om.free()
+    per_it = sec / sample_iters
+    return dict(value=per_it * gpu_iters, unit="s", cores=threads, kind="port",
+                sample="%d Jacobi-PCG iterations of the same %d^3 system on %d threads (OpenMP port of the "
+                       "reference recurrence, %.2f s), scaled to the %d iterations the GPU path needs"
+                       % (sample_iters, n, threads, sec, gpu_iters),
+                per_iter_ms=per_it * 1e3, **info)
+
+
+def cpu_simple(args, gpu_states):
+    """The oracle's SIMPLE (plain C, one core, the reference's algorithm) on
+    the C2 cavity: pressure-correction solves to the FP64 relative L1
+    residual tol from x = 0 (p_solver 3: the reference's solv_cg with reduc =
+    tol), outer iterations 2 and 3 timed (iteration 1 solves nothing: the
+    first p' right-hand side is zero).  After iteration 3 each field of the
+    GPU flows is compared with the oracle's (max relative error)."""
     from oracle import fvo
     n = args.simple_n
     t0 = time.time()
     om = fvo.OMesh.box(n, n, n, jitter=0.02, seed=0x5EED0002)
-    of = fvo.OFlow(om, fvo.default_cfg(simpler=0, imat_u=0, imat_p=0, urf_p=0.3))
+    of = fvo.OFlow(om, fvo.default_cfg(simpler=0, imat_u=0, imat_p=0, urf_p=0.3, p_solver=3, p_tol=args.tol))
     of.set_bnd_u(31, [1.0, 0.0, 0.0])
     of.outer(1)
-    of.outer(2)
     t_setup = time.time() - t0
     t0 = time.time()
+    of.outer(2)
     of.outer(3)
-    dt = time.time() - t0
+    dt = (time.time() - t0) / 2
+    F = of.fields()
+    errs = {}
+    for name, st in gpu_states.items():
+        errs[name] = {k: float(np.max(np.abs(st[k] - F[k])) / np.max(np.abs(F[k])))
+                      for k in ("u_cell", "p", "dpdx", "spd")}
     of.free()
     om.free()
+    log("oracle SIMPLE %d^3: %.1f s per outer iteration; GPU fields after 3 iterations vs oracle: %s"
+        % (n, dt, json.dumps(errs)))
     return dict(value=dt, unit="s per outer iteration", cores=1, kind="port",
-                sample="outer iteration 3 of the oracle's SIMPLE on the %d^3 cavity (set-up and iterations "
-                       "1-2, %.1f s, excluded)" % (n, t_setup))
+                sample="outer iterations 2-3 of the oracle's SIMPLE on the %d^3 cavity with pressure solves "
+                       "to %g (set-up and iteration 1, %.1f s, excluded)" % (n, args.tol, t_setup),
+                field_rel_err_vs_gpu_after_3=errs)
 
 
 def train_model(fvk, args):
@@ -229,27 +280,49 @@ def simple_leg(fvk, args, model):
         out["reference_pressure_" + mode] = dict(outer_its_per_s=1.0 / dt, ms_per_outer=dt * 1e3,
                                                  mx_res_n_last=h[-1]["mx_res_n"])
         fl.destroy()
+    # C2: outer iterations 1..simple_tol_its with every pressure-correction
+    # system solved to the FP64 relative L1 residual tol from x = 0, plain
+    # FP64 PCG against the learned FP32-inner solve, replayed from HIP
+    # graphs; the fields after iteration 3 go to the oracle comparison
     legs = [("plain_pcg64", 3, None)]
     if model is not None:
         legs.append(("dc32_learned", 2, model))
+    states3, final = {}, {}
+    N = max(args.simple_tol_its, 4)
     for name, ps, mdl in legs:
         fl = flow(p_solver=ps, p_tol=args.tol)
         if mdl is not None:
             fl.set_model(mdl, train_steps=0)
-        fl.run(1, 2)
         c.sync()
         t0 = time.perf_counter()
-        h = fl.run(3, args.simple_tol_its)
+        h = fl.run_graph(1, 3)
         c.sync()
-        dt = (time.perf_counter() - t0) / args.simple_tol_its
-        out["pressure_to_tol_" + name] = dict(ms_per_outer=dt * 1e3, outer_its_per_s=1.0 / dt,
-                                              p_iters=[x["it_pp"] for x in h])
+        t3 = time.perf_counter() - t0
+        states3[name] = fl.state()
+        t0 = time.perf_counter()
+        h += fl.run_graph(4, N - 3)
+        c.sync()
+        dt = t3 + time.perf_counter() - t0
+        final[name] = fl.state()
+        out["c2_" + name] = dict(outer_iterations=N, seconds=dt, ms_per_outer=dt * 1e3 / N,
+                                 outer_its_per_s=N / dt, p_iters=[x["it_pp"] for x in h],
+                                 mx_res_n_last=h[-1]["mx_res_n"])
+        log("SIMPLE %d^3 C2 %s: %d outer iterations in %.2f s, p' iterations %s..%s"
+            % (n, name, N, dt, h[1]["it_pp"], h[-1]["it_pp"]))
         fl.destroy()
+    if len(final) == 2:
+        a, b = final["plain_pcg64"], final["dc32_learned"]
+        out["c2_field_rel_diff_learned_vs_plain"] = {
+            k: float(np.max(np.abs(a[k] - b[k])) / np.max(np.abs(a[k]))) for k in ("u_cell", "p", "dpdx", "spd")}
+        out["c2_speedup_learned_vs_plain"] = (out["c2_plain_pcg64"]["seconds"]
+                                              / out["c2_dc32_learned"]["seconds"])
     c.destroy()
     m.free()
     log("SIMPLE %d^3: %s" % (n, json.dumps(out)))
     return dict(workload="SIMPLE outer iterations, %d^3 lid-driven cavity (+-2%% jitter), Re=100, urf_u 0.7, "
-                         "urf_p 0.3, momentum BiCGStab (3 RHS batched, Jacobi, 5 its)" % n, **out)
+                         "urf_p 0.3, momentum BiCGStab (3 RHS batched, Jacobi, 5 its); c2_*: %d outer iterations "
+                         "with pressure-correction solves to FP64 relative L1 residual %g" % (n, N, args.tol),
+                **out), states3
 
 
 def channel_leg(fvk, args):
@@ -353,7 +426,8 @@ def main():
     ap.add_argument("--lr", type=float, default=1e-2, help="Adam step of the network's training")
     ap.add_argument("--simple-n", type=int, default=128)
     ap.add_argument("--simple-its", type=int, default=20)
-    ap.add_argument("--simple-tol-its", type=int, default=3)
+    ap.add_argument("--simple-tol-its", type=int, default=100,
+                    help="outer iterations of the C2 leg (pressure solves to tol)")
     ap.add_argument("--no-simple", action="store_true")
     ap.add_argument("--channel", type=int, default=0, help="triangles of the optional channel leg (0: off)")
     ap.add_argument("--channel-warm", type=int, default=6)
@@ -403,18 +477,31 @@ def main():
         r = res[-1]
         log("%s timed: %.4f s per solve (wall), %.4f s device, %d iterations, true L1 %.2e L2 %.2e"
             % (name, wall, r["seconds"], r["iters"], r["res_true_l1"], r["res_true_l2"]))
+        # every timed step must have converged on its own (no fallback) to the
+        # FP64 true residual, or the time is not the method's
+        r["steps_ok"] = all(x["status"] == fvk.FVK_OK and x["fallback"] == 0 and x["res_true_l1"] < args.tol
+                            for x in res)
+        r["step_iters"] = [x["iters"] for x in res]
+        if not r["steps_ok"]:
+            log("%s: a timed step did not converge on its own: %s" % (
+                name, [(x["status"], x["fallback"], x["res_true_l1"]) for x in res]))
         return wall, r
 
     wall, r = timed(fvk.SOLVE_PCG64, "plain_pcg64_jacobi")
     methods = {"plain_pcg64_jacobi": dict(s=wall, iters=r["iters"], res_true_l1=r["res_true_l1"],
-                                          res_true_l2=r["res_true_l2"])}
+                                          res_true_l2=r["res_true_l2"], steps_ok=r["steps_ok"])}
+    x_plain = fvk.sys_x(ctx)
     model = None
     if "learned" in args.methods.split(","):
         model, tinfo = train_model(fvk, args)
         w3, r3 = timed(fvk.SOLVE_DC32_FACTOR, "dc32_learned", model)
+        x_learned = fvk.sys_x(ctx)
         rk3 = fvk.sys_solve(ctx, fvk.SOLVE_DC32_FACTOR, 0, args.tol, 200000, time_kernel=True)
         methods["dc32_learned"] = dict(s=w3, iters=r3["iters"], res_true_l1=r3["res_true_l1"],
                                        res_true_l2=r3["res_true_l2"], fallback=r3["fallback"],
+                                       steps_ok=r3["steps_ok"], step_iters=r3["step_iters"],
+                                       x_rel_diff_vs_plain=float(np.max(np.abs(x_learned - x_plain))
+                                                                 / np.max(np.abs(x_plain))),
                                        inner_iters=r3["inner_iters"], outer_res=r3["outer_res"],
                                        speedup_vs_plain=wall / w3, k_mx_q_us=rk3["kern_seconds"] * 1e6,
                                        k_mx_q_gbs=rk3["kern_bytes"] / rk3["kern_seconds"] / 1e9,
@@ -454,11 +541,23 @@ def main():
                                     res_true_l2=head["res_true_l2"])
         method = ("dc32_learned: network factor G (per-face MLP, bf16 MFMA) + FP32-inner PCG with z = G G^T r "
                   "in FP64 defect correction; one step = factor build + solve")
-        dtype = "f32-inner/f64-outer"
+        # the factor's L and Delta^2 and the hand-off vector t are FP16, the
+        # inner Krylov vectors and the SpMV FP32, the judge (true residual,
+        # solution) FP64
+        dtype = "f16-factor/f32-inner/f64-outer"
+        # bytes per launch by two models: the compressed format k_mx_q reads
+        # (half of Ah, 16-bit columns, one-byte slot index) and the FP32 CSR
+        # SpMV model nnz (w + 4) + 4 (n + 1) + 2 w n, w = 4
+        csr32 = nnz * 8 + 4 * (mesh.n_vol + 1) + 8 * mesh.n_vol
+        us = head["k_mx_q_us"]
         roof = dict(bound="hbm", kernel="k_mx_q (FP32: q = Ah p; p.q)",
                     achieved=head["k_mx_q_gbs"], peak=PEAK_HBM / 1e9, unit="GB/s",
                     frac=head["k_mx_q_gbs"] * 1e9 / PEAK_HBM, traffic=traffic_bytes("k_mx_q"),
-                    bytes_per_launch=head["k_mx_q_bytes"], us_per_launch=head["k_mx_q_us"])
+                    bytes_per_launch=head["k_mx_q_bytes"], us_per_launch=us,
+                    bytes_model="compressed format: 8n + 4.5 nnz_off (p, q FP32; upper entries FP32 + 16-bit "
+                                "column; lower entries 16-bit column + 1-byte slot)",
+                    csr_fp32_bytes=csr32, csr_fp32_achieved=csr32 / (us * 1e-6) / 1e9,
+                    csr_fp32_frac=csr32 / (us * 1e-6) / PEAK_HBM)
     else:
         value, hr = wall, dict(iters=r["iters"], res_true_l1=r["res_true_l1"], res_true_l2=r["res_true_l2"])
         method = "plain_pcg64_jacobi"
@@ -488,17 +587,21 @@ def main():
         "copy_gbs": 2 * copy_bytes / t_copy / 1e9,
         "methods": methods,
     }
+    states3 = {}
     if not args.no_simple:
-        out["simple"] = simple_leg(fvk, args, model)
+        out["simple"], states3 = simple_leg(fvk, args, model)
     if args.channel > 0:
         out["channel"] = channel_leg(fvk, args)
     if not args.no_cpu:
         port = cpu_baseline(ctx, fvk, n, args.jitter, r["iters"], args.cpu_sample)
         ref = cpu_baseline_reference(ctx, mesh, fvk, n, r["iters"], args.cpu_ref_sample)
         out["cpu_baseline"] = ref if ref is not None else port
+        out["cpu_baseline"].update(host_info())
         out["cpu_baseline_port"] = port
+        out["cpu_baseline_all_cores"] = cpu_baseline_all_cores(ctx, fvk, n, args.jitter, r["iters"],
+                                                               args.cpu_sample)
         if not args.no_simple:
-            out["cpu_baseline_simple"] = cpu_simple(args)
+            out["cpu_baseline_simple"] = cpu_simple(args, states3)
     print(json.dumps(out), flush=True)
     ctx.destroy()
 

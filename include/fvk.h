@@ -66,6 +66,30 @@ int fvk_mesh_box(int nx, int ny, int nz, double lx, double ly, double lz, double
    bnd holds one boundary code per (cell, local face), 0 = none. */
 int fvk_mesh_from_cells(int n_dx, int n_vrt, const double *vx3, int n_vol, const int *v_ty,
                         const int *v_vptr, const int *v_vrt, const int *bnd, fvk_mesh **out);
+/* gmsh 2.2 ASCII mesh (flowlab/src/mesh.cpp:39-187 read_gmsh_mesh, serial
+   mode) + grd_conn + grd_geom: elements of dimension n_dx (n_dx <= 0: the
+   highest present) are the cells, in file order (hexahedra, prisms and
+   tetrahedra in 3D, quadrilaterals and triangles in 2D; vertex orders
+   converted to Gambit's); elements of dimension n_dx - 1 mark boundary
+   faces with their physical tag, mapped through code_map (n_map pairs
+   physical tag -> boundary code, e.g. 11 inlet, 21 outlet, 31 wall; an
+   unmapped tag is used as the code).  l_sc scales coordinates.  Each
+   cell's gmsh element number is kept (fvk_mesh_get_elm_idx). */
+int fvk_mesh_read_gmsh(const char *path, int n_dx, double l_sc, const int *code_map, int n_map,
+                       fvk_mesh **out);
+/* Where the mesh constructors (fvk_mesh_read_neu, fvk_mesh_read_gmsh,
+   fvk_mesh_box, fvk_mesh_from_cells) build connectivity and geometry: on
+   GPU `device`, or on the host (-1, the default).  Both produce the same
+   arrays bit for bit; the GPU build takes a fraction of the host's time on
+   meshes of millions of cells.  No counterpart in the reference. */
+int fvk_mesh_set_build_device(int device);
+/* Source element number of each cell (n_vol; 1..n_vol for meshes that
+   have none), e.g. for fvk_write_gmsh_field. */
+int fvk_mesh_get_elm_idx(const fvk_mesh *m, int *elm_idx);
+/* Vertex coordinates (n_vrt*3) and cell vertex lists (cell_vptr n_vol+1,
+   cell_vrt 1-based, Gambit order), as the reference's grd structure holds
+   them. */
+int fvk_mesh_get_cells(const fvk_mesh *m, double *vx3, int *v_ty, int *cell_vptr, int *cell_vrt);
 int fvk_mesh_get_info(const fvk_mesh *m, fvk_mesh_info *info);
 /* Topology as the reference holds it: fce%ip, fce%in, fce%ib, fce%bnd_ty
    (n_fce each), cell_ptr (n_vol+1, 0-based offsets) and the per-cell
@@ -353,6 +377,31 @@ int fvk_flow_run_graph(fvk_flow *f, int it0, int n, fvk_outer_hist *h);
    train_steps > 0: that many Adam steps of rate lr on every
    pressure-correction matrix before its factor is built. */
 int fvk_flow_set_model(fvk_flow *f, fvk_model *m, int train_steps, double lr);
+
+/* ------------------------------------------------------------------------
+ * Output formats.
+ * ---------------------------------------------------------------------- */
+/* Tecplot FEBLOCK file of the flow's velocity, as eqn_outp writes it
+   (spray-simulation/cfd_outp.f90:8-98): DIR/grd<igrd>%phs<iph>%eqn<ieq>.dat
+   (with @<ict, 4 digits> when n_tl > 1 and ict > 0), vertex coordinates of
+   the mesh, the n_dx velocity components per cell (cell-centred), cell
+   vertex lists in Tecplot's brick / quadrilateral order (tecplot.vrt); the
+   bytes of the flang-built reference's list-directed output.  m is the mesh
+   the flow's context was built on. */
+int fvk_eqn_outp(const fvk_mesh *m, fvk_flow *f, int igrd, int iph, int ieq, int ict, int n_tl,
+                 const char *dir);
+/* The same for the pressure, as prp_outp(..., p=.true.) writes it
+   (cfd_outp.f90:102-189): DIR/grd<igrd>%phs<iph>%prp<ipr>.dat. */
+int fvk_prp_outp_p(const fvk_mesh *m, fvk_flow *f, int igrd, int iph, int ipr, int ict, int n_tl,
+                   const char *dir);
+/* gmsh ElementData files of cell values, as flowlab writes them
+   (flowlab/src/mesh.cpp:298-412): elm_idx[i] is the element number of
+   value i; the vector form writes n_dim components padded to 3. */
+int fvk_write_gmsh_field(const char *fname, int n, const int *elm_idx, const double *phi);
+int fvk_write_gmsh_vector_field(const char *fname, int n, int n_dim, const int *elm_idx, const double *phi);
+/* flang's list-directed editing of one REAL(8) value (what the Tecplot
+   writers print), into buf (len bytes, NUL-terminated). */
+int fvk_fmt_list_real(double x, char *buf, int len);
 
 #ifdef __cplusplus
 }

@@ -14,7 +14,11 @@
 #       which is the SIMPLE/Jacobi-PCG configuration the reference cannot
 #       select without editing a literal;
 #   _ref/cg/fvk_ref_cg            the reference's solv_cg with a timing
-#       driver (oracle/ref/fvk_ref_cg.f90), the CPU baseline of bench.py.
+#       driver (oracle/ref/fvk_ref_cg.f90), the CPU baseline of bench.py;
+#   _ref/outp/fvk_ref_outp        the reference's Tecplot writers
+#       (cfd_outp.f90) behind oracle/ref/fvk_ref_outp.f90;
+#   _ref/gmsh/gmsh_ref            flowlab's gmsh reader and field writers
+#       (flowlab/src/mesh.cpp) behind oracle/ref/gmsh_ref.cpp.
 set -euo pipefail
 REF=${FVK_REFERENCE_DIR:-/root/reference/spray-simulation}
 HERE=$(cd "$(dirname "$0")" && pwd)
@@ -60,4 +64,21 @@ for m in precision solv_data solv_res solv_precond solv_direct solv_cg; do
 done
 (cd "$w" && $FC $FFLAGS -c "$HERE/ref/fvk_ref_cg.f90" -o driver.o)
 $FC $FFLAGS -o "$OUT/cg/fvk_ref_cg" "$w/driver.o" $objs
-echo "built $OUT/shipped/fvk_ref_driver $OUT/relaxed/fvk_ref_driver $OUT/cg/fvk_ref_cg"
+# the reference's Tecplot writers (cfd_outp.f90) behind a driver of this
+# project (oracle/ref/fvk_ref_outp.f90): golden output files, shipped
+# configuration
+w=$SCRATCH/outp objs=""
+mkdir -p "$w" "$OUT/outp"
+for m in $MODS cfd_outp; do
+  (cd "$w" && $FC $FFLAGS -c "$REF/$m.f90" -o "$m.o")
+  objs="$objs $w/$m.o"
+done
+(cd "$w" && $FC $FFLAGS -c "$HERE/ref/fvk_ref_outp.f90" -o driver.o)
+$FC $FFLAGS -o "$OUT/outp/fvk_ref_outp" "$w/driver.o" $objs
+# flowlab's gmsh reader and field writers (flowlab/src/mesh.cpp, compiled
+# where it lies) behind a driver of this project (oracle/ref/gmsh_ref.cpp)
+FLOWLAB=${FVK_FLOWLAB_DIR:-$REF/../flowlab/src}
+mkdir -p "$OUT/gmsh"
+g++ -O2 -I"$FLOWLAB" -o "$OUT/gmsh/gmsh_ref" "$HERE/ref/gmsh_ref.cpp" "$FLOWLAB/mesh.cpp"
+echo "built $OUT/shipped/fvk_ref_driver $OUT/relaxed/fvk_ref_driver $OUT/cg/fvk_ref_cg $OUT/outp/fvk_ref_outp" \
+     "$OUT/gmsh/gmsh_ref"

@@ -92,6 +92,9 @@ int fvo_solv_cg(const fvo_sys *A, int imat, double reduc, int n_iter, int n_vc,
 int fvo_solv_bicg(const fvo_sys *A, int imat, double reduc, int n_iter, int n_vc,
                   double *phi, int ld, double *res0, double *res0_n, int not_r0, int *iters,
                   double *hist);
+/* CPU baseline timing on all cores (not parity): n_iter Jacobi-PCG
+   iterations from x = 0, OpenMP; returns seconds. */
+double fvo_pcg_omp(const fvo_sys *A, const double *b, double *x, int n_iter, int threads);
 /* Rows summing to zero (pure Neumann)? */
 int fvo_is_singular(const fvo_sys *A);
 /* Solve to a relative L1 residual tol from x = 0 (b, x 1-based); returns

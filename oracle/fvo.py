@@ -107,6 +107,8 @@ def lib():
         L.fvo_solv_res.argtypes = [_P(Sys), C.c_int, dp, C.c_int, dp, dp, C.c_int]
         L.fvo_set_sum_order.argtypes = [C.c_int]
         L.fvo_is_singular.argtypes = [_P(Sys)]
+        L.fvo_pcg_omp.argtypes = [_P(Sys), dp, dp, C.c_int, C.c_int]
+        L.fvo_pcg_omp.restype = C.c_double
         L.fvo_pcg_tol.argtypes = [_P(Sys), dp, dp, C.c_double, C.c_int, ip, dp]
         _lib = L
     return _lib
@@ -326,6 +328,15 @@ class OSys:
         r0 = np.zeros(1)
         sing = lib().fvo_pcg_tol(C.byref(self.s), _d(bb), _d(x), tol, max_iter, _i(its), _d(r0))
         return x[1:].copy(), int(its[0]), float(r0[0]), bool(sing)
+
+    def pcg_omp(self, b, n_iter, threads=0):
+        """CPU baseline timing on all cores: seconds for n_iter Jacobi-PCG
+        iterations from x = 0 (OpenMP; not bitwise the sequential oracle).
+        Returns (seconds, x)."""
+        bb = np.concatenate([[0.0], np.asarray(b, np.float64)])
+        x = np.zeros(self.n + 1)
+        sec = lib().fvo_pcg_omp(C.byref(self.s), _d(bb), _d(x), int(n_iter), int(threads))
+        return sec, x[1:].copy()
 
     def solve(self, kind, imat, reduc, n_iter, b, x0=None, not_r0=False):
         """b: (n, n_vc).  Returns x (n, n_vc), residual (n, n_vc), res0_n, iters, hist."""

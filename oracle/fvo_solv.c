@@ -295,3 +295,65 @@ int fvo_pcg_tol(const fvo_sys *A, const double *b, double *x, double tol, int ma
   free(bs);
   return singular;
 }
+
+/* CPU baseline on all host cores (timing only, not parity): the reference's
+   Jacobi-PCG recurrence (solv_cg.f90:29-82, imat 0, no exit test) with the
+   product z = A p as a row gather over each cell's faces in ascending face
+   index (the per-row order of the reference's face scatter), OpenMP over
+   rows; dot products and the L1 norm reduce per thread, so the iterates
+   differ from the sequential oracle's in the last bits.  Runs n_iter
+   iterations from x = 0 on `threads` threads (0: the OpenMP default) and
+   returns the seconds they took; x (1-based) receives the iterate. */
+#include <omp.h>
+double fvo_pcg_omp(const fvo_sys *A, const double *b, double *x, int n_iter, int threads) {
+  const fvo_mesh *m = A->m;
+  const int n = m->n_vol;
+  if (threads > 0) omp_set_num_threads(threads);
+  double *r = V1(n), *p = V1(n), *z = V1(n), *q = V1(n), *c = V1(n);
+  const double sm = fvo_small();
+  double s_old = fvo_large();
+  double t0 = omp_get_wtime();
+#pragma omp parallel for schedule(static)
+  for (int i = 1; i <= n; ++i) {
+    x[i] = 0.0;
+    r[i] = b[i];
+    p[i] = 0.0;
+    c[i] = 1.0 / (A->pl[i] + sm);
+  }
+  for (int it = 0; it < n_iter; ++it) {
+    double s = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : s)
+    for (int i = 1; i <= n; ++i) {
+      z[i] = r[i] * c[i];
+      s += r[i] * z[i];
+    }
+    const double beta = (1.0 / s_old) * s;
+    s_old = s;
+    double pq = 0.0;
+#pragma omp parallel for schedule(static)
+    for (int i = 1; i <= n; ++i) p[i] = z[i] + beta * p[i];
+#pragma omp parallel for schedule(static) reduction(+ : pq)
+    for (int i = 1; i <= n; ++i) {
+      double y = A->pl[i] * p[i];
+      for (int k = m->v_fptr[i]; k < m->v_fptr[i + 1]; ++k) {
+        const int f = m->v_fce[k], ip = m->f_ip[f], in = m->f_in[f];
+        if (in <= 0) continue;
+        y = ip == i ? y + A->ng_p[f] * p[in] : y + A->ng_n[f] * p[ip];
+      }
+      q[i] = y;
+      pq += p[i] * y;
+    }
+    const double alpha = s / (pq + sm);
+    double l1 = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : l1)
+    for (int i = 1; i <= n; ++i) {
+      x[i] = x[i] + alpha * p[i];
+      r[i] = r[i] - alpha * q[i];
+      l1 += fabs(r[i]);
+    }
+    (void)l1;
+  }
+  double dt = omp_get_wtime() - t0;
+  free(r); free(p); free(z); free(q); free(c);
+  return dt;
+}

@@ -12,6 +12,7 @@
 
 #include "flow.h"
 #include "mesh.h"
+#include "output.h"
 
 struct fvk_mesh {
   fvk::Mesh m;
@@ -152,6 +153,22 @@ static int build_levels(Ctx &c, const Topo &t, const std::vector<int> &iperm, bo
   return upload(c, &L.rows, rows);
 }
 
+// gk of DMesh: for the neighbour side of an interior face, the slot of the
+// face in the owner row's list
+__global__ __launch_bounds__(BS) void k_slot_owner(DMesh M, signed char *gk) {
+  const size_t total = (size_t)M.W * M.n;
+  for (size_t o = blockIdx.x * (size_t)BS + threadIdx.x; o < total; o += (size_t)gridDim.x * BS) {
+    const int e = M.cf[o];
+    signed char k = -1;
+    if (e != PAD && e < 0) {
+      const int f = ~e, r = M.fip[f];
+      for (int kk = 0; kk < M.W; ++kk)
+        if (M.cf[(size_t)kk * M.n + r] == f) k = (signed char)kk;
+    }
+    gk[o] = k;
+  }
+}
+
 // Per-slot copies of the face geometry (the slot-major tables of DMesh);
 // boundary slots take the face centre minus the cell centre in gp.
 __global__ __launch_bounds__(BS) void k_slot_geom(DMesh M, double *gs, double *gnx, double *gny, double *gnz,
@@ -277,6 +294,11 @@ static int ctx_build(Ctx &c, const Topo &t, int device, int flags) {
     M.gs = g[0]; M.gnx = g[1]; M.gny = g[2]; M.gnz = g[3]; M.gw = g[4];
     M.gpx = g[5]; M.gpy = g[6]; M.gpz = g[7]; M.gqx = g[8]; M.gqy = g[9]; M.gqz = g[10];
     M.go = go;
+    signed char *gk = nullptr;
+    if ((rc = dalloc(c, &gk, ns))) return rc;
+    k_slot_owner<<<gb, BS, 0, c.stream>>>(M, gk);
+    FVK_HIP(hipGetLastError());
+    M.gk = gk;
   }
   if ((rc = build_levels(c, t, c.iperm, true, c.lf))) return rc;
   if ((rc = build_levels(c, t, c.iperm, false, c.lb))) return rc;
@@ -362,6 +384,14 @@ const char *fvk_last_error(void) { return g_err.c_str(); }
 int fvk_version(void) { return 1; }
 
 // ---------------------------------------------------------------- mesh
+// Device on which mesh constructors build connectivity and geometry (-1:
+// the host)
+static int g_mesh_device = -1;
+
+static std::string finish_sel(fvk::Mesh &m) {
+  return g_mesh_device >= 0 ? finish_device(m, g_mesh_device) : finish(m);
+}
+
 static int mesh_done(fvk_mesh *m, const std::string &e, fvk_mesh **out) {
   if (!e.empty()) {
     set_error(e);
@@ -381,7 +411,42 @@ int fvk_mesh_read_neu(const char *path, double l_sc, fvk_mesh **out) {
     delete m;
     return e.rfind("cannot open", 0) == 0 ? FVK_ERR_IO : FVK_ERR_ARG;
   }
-  return mesh_done(m, finish(m->m), out);
+  return mesh_done(m, finish_sel(m->m), out);
+}
+
+int fvk_mesh_read_gmsh(const char *path, int n_dx, double l_sc, const int *code_map, int n_map, fvk_mesh **out) {
+  if (!path || !out || n_map < 0 || (n_map > 0 && !code_map)) { set_error("bad argument"); return FVK_ERR_ARG; }
+  fvk_mesh *m = new fvk_mesh;
+  std::string e = read_gmsh(m->m, path, n_dx, l_sc, code_map, n_map);
+  if (!e.empty()) {
+    set_error(e);
+    delete m;
+    return e.rfind("cannot open", 0) == 0 ? FVK_ERR_IO : FVK_ERR_ARG;
+  }
+  return mesh_done(m, finish_sel(m->m), out);
+}
+
+int fvk_mesh_set_build_device(int device) {
+  if (device < -1) { set_error("bad device"); return FVK_ERR_ARG; }
+  g_mesh_device = device;
+  return FVK_OK;
+}
+
+int fvk_mesh_get_elm_idx(const fvk_mesh *m, int *elm_idx) {
+  if (!m || !elm_idx) { set_error("null argument"); return FVK_ERR_ARG; }
+  for (int c = 0; c < m->m.n_vol; ++c) elm_idx[c] = m->m.elm_idx.empty() ? c + 1 : m->m.elm_idx[c];
+  return FVK_OK;
+}
+
+int fvk_mesh_get_cells(const fvk_mesh *m, double *vx3, int *v_ty, int *cell_vptr, int *cell_vrt) {
+  if (!m) { set_error("null argument"); return FVK_ERR_ARG; }
+  const fvk::Mesh &M = m->m;
+  if (vx3) std::memcpy(vx3, M.vx.data(), sizeof(double) * M.vx.size());
+  if (v_ty) std::memcpy(v_ty, M.v_ty.data(), sizeof(int) * M.v_ty.size());
+  if (cell_vptr) std::memcpy(cell_vptr, M.v_vptr.data(), sizeof(int) * M.v_vptr.size());
+  if (cell_vrt)
+    for (size_t k = 0; k < M.v_vrt.size(); ++k) cell_vrt[k] = M.v_vrt[k] + 1;
+  return FVK_OK;
 }
 
 int fvk_mesh_box(int nx, int ny, int nz, double lx, double ly, double lz, double jitter, uint64_t seed, int lid_code,
@@ -389,7 +454,7 @@ int fvk_mesh_box(int nx, int ny, int nz, double lx, double ly, double lz, double
   if (!out) { set_error("null argument"); return FVK_ERR_ARG; }
   fvk_mesh *m = new fvk_mesh;
   std::string e = make_box(m->m, nx, ny, nz, lx, ly, lz, jitter, seed, lid_code, wall_code);
-  if (e.empty()) e = finish(m->m);
+  if (e.empty()) e = finish_sel(m->m);
   return mesh_done(m, e, out);
 }
 
@@ -401,7 +466,7 @@ int fvk_mesh_from_cells(int n_dx, int n_vrt, const double *vx3, int n_vol, const
   }
   fvk_mesh *m = new fvk_mesh;
   std::string e = init_cells(m->m, n_dx, n_vrt, vx3, n_vol, v_ty, v_vptr, v_vrt, bnd);
-  if (e.empty()) e = finish(m->m);
+  if (e.empty()) e = finish_sel(m->m);
   return mesh_done(m, e, out);
 }
 
@@ -1165,6 +1230,70 @@ int fvk_flow_run_graph(fvk_flow *ff, int it0, int n, fvk_outer_hist *h) {
 int fvk_flow_run(fvk_flow *ff, int it0, int n, fvk_outer_hist *h) {
   if (!ff || it0 < 1 || n < 0) return FVK_ERR_ARG;
   for (int k = 0; k < n; ++k) GUARD(fvk_flow_outer(ff, it0 + k, h ? h + k : nullptr));
+  return FVK_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- output
+
+extern "C" {
+
+int fvk_fmt_list_real(double x, char *buf, int len) {
+  if (!buf || len < 1) { set_error("null argument"); return FVK_ERR_ARG; }
+  const std::string t = fvk::fmt_list_real(x);
+  if ((int)t.size() + 1 > len) { set_error("buffer too small"); return FVK_ERR_ARG; }
+  std::memcpy(buf, t.c_str(), t.size() + 1);
+  return FVK_OK;
+}
+
+static int tecplot_common(const fvk_mesh *mm, fvk_flow *ff, bool eqn, int igrd, int iph, int ieq, int ict,
+                          int n_tl, const char *dir) {
+  if (!mm || !ff || !dir) { set_error("null argument"); return FVK_ERR_ARG; }
+  const fvk::Mesh &m = mm->m;
+  Flow &F = ff->f;
+  if (m.n_vol != F.c->n || m.n_dx != F.c->ndx) {
+    set_error("mesh and flow differ in cells or dimension");
+    return FVK_ERR_ARG;
+  }
+  const int nvc = eqn ? F.nvc : 1;
+  std::vector<double> val((size_t)m.n_vol * nvc);
+  if (eqn) GUARD(fvk_flow_get_state(ff, val.data(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr));
+  else GUARD(fvk_flow_get_state(ff, nullptr, nullptr, val.data(), nullptr, nullptr, nullptr, nullptr));
+  fvk::TecplotIn in;
+  in.kind_eqn = eqn;
+  in.igrd = igrd; in.iph = iph; in.ieq = ieq; in.ict = ict; in.n_tl = n_tl;
+  in.n_dx = m.n_dx; in.n_vrt = m.n_vrt; in.n_vol = m.n_vol; in.n_sc = 1; in.n_vc = nvc;
+  in.vx = m.vx.data(); in.v_vptr = m.v_vptr.data(); in.v_vrt = m.v_vrt.data(); in.val = val.data();
+  std::string path;
+  const std::string err = fvk::tecplot_write(in, dir, path);
+  if (!err.empty()) { set_error(err); return FVK_ERR_IO; }
+  return FVK_OK;
+}
+
+int fvk_eqn_outp(const fvk_mesh *m, fvk_flow *f, int igrd, int iph, int ieq, int ict, int n_tl, const char *dir) {
+  return tecplot_common(m, f, true, igrd, iph, ieq, ict, n_tl, dir);
+}
+
+int fvk_prp_outp_p(const fvk_mesh *m, fvk_flow *f, int igrd, int iph, int ipr, int ict, int n_tl,
+                   const char *dir) {
+  return tecplot_common(m, f, false, igrd, iph, ipr, ict, n_tl, dir);
+}
+
+int fvk_write_gmsh_field(const char *fname, int n, const int *elm_idx, const double *phi) {
+  if (!fname || n < 0 || (n > 0 && (!elm_idx || !phi))) { set_error("bad argument"); return FVK_ERR_ARG; }
+  const std::string err = fvk::gmsh_write(fname, n, 1, 1, elm_idx, phi);
+  if (!err.empty()) { set_error(err); return FVK_ERR_IO; }
+  return FVK_OK;
+}
+
+int fvk_write_gmsh_vector_field(const char *fname, int n, int n_dim, const int *elm_idx, const double *phi) {
+  if (!fname || n < 0 || n_dim < 1 || n_dim > 3 || (n > 0 && (!elm_idx || !phi))) {
+    set_error("bad argument");
+    return FVK_ERR_ARG;
+  }
+  const std::string err = fvk::gmsh_write(fname, n, 3, n_dim, elm_idx, phi);
+  if (!err.empty()) { set_error(err); return FVK_ERR_IO; }
   return FVK_OK;
 }
 

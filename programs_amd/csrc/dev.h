@@ -35,6 +35,12 @@ struct DMesh {
   // face arrays, laid out so that the rows of a wave read adjacent words.
   const double *gs, *gnx, *gny, *gnz, *gw, *gpx, *gpy, *gpz, *gqx, *gqy, *gqz;
   const int *go;
+  // W*n, slot-major: for the neighbour side of an interior face, the slot of
+  // that face in its owner row's list (the owner's entries hold the same
+  // geometry); -1 otherwise.  The gradient reads an interior face's
+  // geometry from the owner's entry, which neighbouring rows have just
+  // streamed, so each face's geometry comes from HBM about once per sweep.
+  const signed char *gk;
   const int *spc;   // Ws x ld interior slots: column row, -1 on padding
   const int *spf;   // Ws x ld: face id of the slot, -1 on padding
 };

@@ -42,15 +42,18 @@ __global__ __launch_bounds__(BS) void k_grad(DMesh M, const double *__restrict__
     const int e = M.cf[o];
     if (e == PAD) break;
     const bool own = e >= 0;
-    const double s = M.gs[o];
-    const double nv[3] = {M.gnx[o], M.gny[o], M.gnz[o]};
-    double pf[3];
     const int other = M.go[o];
+    // an interior face's geometry from its owner's entry (the same values)
+    const int ko = M.gk[o];
+    const size_t og = ko < 0 ? o : (size_t)ko * n + other;
+    const double s = M.gs[og];
+    const double nv[3] = {M.gnx[og], M.gny[og], M.gnz[og]};
+    double pf[3];
     if (other >= 0) {
       const int ip = own ? i : other, in = own ? other : i;
-      const double w = M.gw[o];
+      const double w = M.gw[og];
       if (it > 1) {
-        const double tp[3] = {M.gpx[o], M.gpy[o], M.gpz[o]}, tn[3] = {M.gqx[o], M.gqy[o], M.gqz[o]};
+        const double tp[3] = {M.gpx[og], M.gpy[og], M.gpz[og]}, tn[3] = {M.gqx[og], M.gqy[og], M.gqz[og]};
         for (int q = 0; q < nvc; ++q) {
           double gp[3] = {g0[(q * 3 + 0) * n + ip], g0[(q * 3 + 1) * n + ip], g0[(q * 3 + 2) * n + ip]};
           double gn[3] = {g0[(q * 3 + 0) * n + in], g0[(q * 3 + 1) * n + in], g0[(q * 3 + 2) * n + in]};

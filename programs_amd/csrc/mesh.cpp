@@ -14,6 +14,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <unordered_map>
+#include <algorithm>
 
 namespace fvk {
 
@@ -91,6 +93,146 @@ std::string init_cells(Mesh &m, int n_dx, int n_vrt, const double *vx3, int n_vo
   for (int s = 0; s < m.n_slot; ++s) nb += m.v_bnd[s] > 0;
   if ((m.n_slot + nb) % 2) return "face count is odd: boundary list does not close the mesh";
   m.n_fce = (m.n_slot + nb) / 2;  // grd_read.f90:94-99
+  return "";
+}
+
+// ---- gmsh 2.2 ASCII reader ----
+// The parse follows flowlab's read_gmsh_mesh (flowlab/src/mesh.cpp:39-187,
+// serial mode): whitespace-separated tokens; "$Nodes": count, then id x y z
+// per node; "$Elements": count, then id, type, tag count, the tags (when a
+// fourth tag, the partition count, is below 1 only three tags are kept) and
+// the type's vertices (0, 2, 3, 4, 4, 8, 6, 5 for types 0-7).  Elements of
+// dimension n_dx become cells, in file order; elements of dimension n_dx - 1
+// mark boundary faces, whose code is their physical tag (the first tag)
+// mapped through code_map (pairs physical -> code; unmapped tags keep their
+// value).  gmsh vertex orders are converted to the Gambit orders of the
+// element face table: the hexahedron's 1 2 3 4 5 6 7 8 become 1 2 4 3 5 6 8
+// 7 (Gambit numbers a brick lexicographically); quadrilateral, triangle,
+// prism and tetrahedron agree.  Pyramids are not supported.
+std::string read_gmsh(Mesh &m, const char *path, int n_dx, double l_sc, const int *code_map, int n_map) {
+  FILE *fp = std::fopen(path, "r");
+  if (!fp) return std::string("cannot open ") + path;
+  static const int kNvrt[8] = {0, 2, 3, 4, 4, 8, 6, 5};
+  static const int kDim[8] = {0, 1, 2, 2, 3, 3, 3, 3};
+  static const int kGambit[8] = {0, 0, 3, 2, 6, 4, 5, 0};  // gmsh type -> Gambit NTYPE
+  std::vector<int> node_id;
+  std::vector<double> node_x;
+  std::vector<int> e_idx, e_ty, e_tag0, e_vptr{0}, e_vrt;
+  char buf[256];
+  std::string err;
+  bool have_nodes = false, have_elems = false;
+  while (err.empty() && std::fscanf(fp, "%255s", buf) == 1) {
+    if (!std::strcmp(buf, "$Nodes")) {
+      int n = 0;
+      if (std::fscanf(fp, "%d", &n) != 1 || n < 0) { err = "bad $Nodes count"; break; }
+      node_id.resize(n);
+      node_x.resize(3 * (size_t)n);
+      for (int i = 0; i < n; ++i)
+        if (std::fscanf(fp, "%d %lg %lg %lg", &node_id[i], &node_x[3 * (size_t)i], &node_x[3 * (size_t)i + 1],
+                        &node_x[3 * (size_t)i + 2]) != 4) {
+          err = "bad $Nodes record";
+          break;
+        }
+      have_nodes = true;
+    } else if (!std::strcmp(buf, "$Elements")) {
+      int n = 0;
+      if (std::fscanf(fp, "%d", &n) != 1 || n < 0) { err = "bad $Elements count"; break; }
+      for (int i = 0; i < n && err.empty(); ++i) {
+        int idx, ty, nt;
+        if (std::fscanf(fp, "%d %d %d", &idx, &ty, &nt) != 3 || ty < 0 || ty > 7 || nt < 0 || nt > 11) {
+          err = "bad $Elements record";
+          break;
+        }
+        int tag[12] = {0};
+        for (int j = 0; j < nt; ++j)
+          if (std::fscanf(fp, "%d", &tag[j]) != 1) { err = "bad element tags"; break; }
+        const int nv = kNvrt[ty];
+        int v[8];
+        for (int j = 0; j < nv; ++j)
+          if (std::fscanf(fp, "%d", &v[j]) != 1) { err = "bad element vertices"; break; }
+        if (!err.empty()) break;
+        e_idx.push_back(idx);
+        e_ty.push_back(ty);
+        e_tag0.push_back(nt > 0 ? tag[0] : 0);
+        for (int j = 0; j < nv; ++j) e_vrt.push_back(v[j]);
+        e_vptr.push_back((int)e_vrt.size());
+      }
+      have_elems = true;
+    }
+  }
+  std::fclose(fp);
+  if (!err.empty()) return err;
+  if (!have_nodes || !have_elems) return "no $Nodes or $Elements section";
+  // node labels -> positions
+  std::unordered_map<int, int> pos;
+  pos.reserve(node_id.size() * 2);
+  for (size_t i = 0; i < node_id.size(); ++i)
+    if (!pos.emplace(node_id[i], (int)i).second) return "duplicate node id " + std::to_string(node_id[i]);
+  if (n_dx <= 0) {
+    n_dx = 2;
+    for (int t : e_ty) if (kDim[t] == 3) n_dx = 3;
+  }
+  if (n_dx != 2 && n_dx != 3) return "n_dx must be 2 or 3";
+  // boundary faces: sorted vertex positions -> code
+  std::unordered_map<uint64_t, std::vector<std::pair<std::vector<int>, int>>> bmap;
+  auto key_of = [](std::vector<int> &k) {
+    std::sort(k.begin(), k.end());
+    uint64_t h = 0x9E3779B97F4A7C15ULL;
+    for (int a : k) { h ^= (uint64_t)(uint32_t)a + 0x9E3779B97F4A7C15ULL + (h << 6) + (h >> 2); h *= 0xBF58476D1CE4E5B9ULL; }
+    return h;
+  };
+  std::vector<int> cty, cvptr{0}, cvrt, cidx;
+  for (size_t e = 0; e < e_ty.size(); ++e) {
+    const int t = e_ty[e], a = e_vptr[e], b = e_vptr[e + 1];
+    std::vector<int> vv;
+    for (int k = a; k < b; ++k) {
+      auto it = pos.find(e_vrt[k]);
+      if (it == pos.end()) return "element " + std::to_string(e_idx[e]) + " names an unknown node";
+      vv.push_back(it->second);
+    }
+    if (kDim[t] == n_dx) {
+      if (!kGambit[t]) return "unsupported cell type " + std::to_string(t) + " (element " + std::to_string(e_idx[e]) + ")";
+      static const int kHex[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+      cty.push_back(kGambit[t]);
+      for (int k = 0; k < (int)vv.size(); ++k) cvrt.push_back(1 + (t == 5 ? vv[kHex[k]] : vv[k]));
+      cvptr.push_back((int)cvrt.size());
+      cidx.push_back(e_idx[e]);
+    } else if (kDim[t] == n_dx - 1) {
+      int code = e_tag0[e];
+      for (int q = 0; q < n_map; ++q)
+        if (code_map[2 * q] == code) { code = code_map[2 * q + 1]; break; }
+      std::vector<int> k = vv;
+      const uint64_t h = key_of(k);
+      bmap[h].push_back({k, code});
+    }
+  }
+  const int n_vol = (int)cty.size();
+  if (!n_vol) return "no cells of dimension " + std::to_string(n_dx);
+  // boundary code of each (cell, local face)
+  std::vector<int> bnd;
+  for (int c = 0; c < n_vol; ++c) {
+    const int ty = cty[c];
+    for (int k = 0; k < kElem[ty].n_fce; ++k) {
+      int loc[5];
+      const int nv = elem_face(ty, k, loc);
+      std::vector<int> key(nv);
+      for (int a = 0; a < nv; ++a) key[a] = cvrt[cvptr[c] + loc[a] - 1] - 1;
+      const uint64_t h = key_of(key);
+      int code = 0;
+      auto it = bmap.find(h);
+      if (it != bmap.end())
+        for (auto &pr : it->second)
+          if (pr.first == key) { code = pr.second; break; }
+      bnd.push_back(code);
+    }
+  }
+  std::vector<double> vx3(3 * node_id.size(), 0.0);
+  for (size_t i = 0; i < node_id.size(); ++i)
+    for (int d = 0; d < n_dx; ++d) vx3[3 * i + d] = node_x[3 * i + d] * l_sc;
+  std::string e = init_cells(m, n_dx, (int)node_id.size(), vx3.data(), n_vol, cty.data(), cvptr.data(),
+                             cvrt.data(), bnd.data());
+  if (!e.empty()) return e;
+  m.elm_idx = cidx;
   return "";
 }
 

@@ -16,16 +16,22 @@ struct Mesh {
   std::vector<int> f_ip, f_in, f_ib, f_bty, f_vptr, f_vrt;  // f_in = -1 and f_ib >= 0 on boundary
   std::vector<double> vol_x, vol_v, vol_vr;                 // vol_x stride 3
   std::vector<double> f_x, f_n, f_dxp, f_dxn, f_s, f_lpn, f_wfp;  // vectors stride 3
+  std::vector<int> elm_idx;  // element number of each cell in its source file (empty: 1..n_vol)
 };
 
 int elem_nfce(int ty);
 std::string init_cells(Mesh &m, int n_dx, int n_vrt, const double *vx3, int n_vol, const int *v_ty,
                        const int *v_vptr, const int *v_vrt1, const int *bnd);
 std::string read_neu(Mesh &m, const char *path, double l_sc);
+// gmsh 2.2 ASCII mesh; n_dx <= 0: from the highest element dimension;
+// code_map: n_map pairs (physical tag, boundary code)
+std::string read_gmsh(Mesh &m, const char *path, int n_dx, double l_sc, const int *code_map, int n_map);
 std::string make_box(Mesh &m, int nx, int ny, int nz, double lx, double ly, double lz, double jitter,
                      uint64_t seed, int lid, int wall);
 std::string connect(Mesh &m);
 std::string geometry(Mesh &m);
 std::string finish(Mesh &m);  // connect + geometry
+// the same on a GPU (meshdev.hip): bit-identical arrays
+std::string finish_device(Mesh &m, int device);
 
 }  // namespace fvk

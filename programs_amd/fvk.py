@@ -121,6 +121,10 @@ def lib():
                          C.c_int, C.c_int, C.POINTER(_vp)],
         "fvk_mesh_from_cells": [C.c_int, C.c_int, _dp, C.c_int, _ip, _ip, _ip, _ip, C.POINTER(_vp)],
         "fvk_mesh_get_info": [_vp, C.POINTER(MeshInfo)],
+        "fvk_mesh_read_gmsh": [C.c_char_p, C.c_int, C.c_double, _ip, C.c_int, C.POINTER(_vp)],
+        "fvk_mesh_get_elm_idx": [_vp, _ip],
+        "fvk_mesh_set_build_device": [C.c_int],
+        "fvk_mesh_get_cells": [_vp, _dp, _ip, _ip, _ip],
         "fvk_mesh_get_topology": [_vp] + [_ip] * 7,
         "fvk_mesh_get_geometry": [_vp] + [_dp] * 9,
         "fvk_ctx_create": [_vp, C.c_int, C.c_int, C.POINTER(_vp)],
@@ -169,9 +173,18 @@ def lib():
         "fvk_flow_run_graph": [_vp, C.c_int, C.c_int, C.POINTER(OuterHist)],
         "fvk_flow_set_model": [_vp, _vp, C.c_int, C.c_double],
         "fvk_flow_get_rhs": [_vp, _dp],
+        "fvk_eqn_outp": [_vp, _vp] + [C.c_int] * 5 + [C.c_char_p],
+        "fvk_prp_outp_p": [_vp, _vp] + [C.c_int] * 5 + [C.c_char_p],
+        "fvk_write_gmsh_field": [C.c_char_p, C.c_int, _ip, _dp],
+        "fvk_write_gmsh_vector_field": [C.c_char_p, C.c_int, C.c_int, _ip, _dp],
+        "fvk_fmt_list_real": [C.c_double, C.c_char_p, C.c_int],
     }
     for name, args in sig.items():
-        f = getattr(L, name)
+        # a library older than this binding lacks some entry points: those
+        # fail when called, the rest keep working
+        f = getattr(L, name, None)
+        if f is None:
+            continue
         f.argtypes = args
         f.restype = C.c_int
     for name in ("fvk_mesh_free", "fvk_ctx_destroy", "fvk_flow_destroy", "fvk_model_destroy"):
@@ -202,6 +215,12 @@ def _f64(a, shape=None):
 
 
 # ---------------------------------------------------------------- mesh
+def mesh_build_device(device):
+    """Mesh constructors build connectivity and geometry on this GPU (-1:
+    on the host, the default); the arrays are the same bit for bit."""
+    _check(lib().fvk_mesh_set_build_device(int(device)))
+
+
 class Mesh:
     """The grd structure after grd_read / grd_conn / grd_geom."""
 
@@ -219,6 +238,31 @@ class Mesh:
         h = _vp()
         _check(lib().fvk_mesh_read_neu(os.fsencode(path), l_sc, C.byref(h)))
         return cls(h)
+
+    @classmethod
+    def read_gmsh(cls, path, n_dx=0, l_sc=1.0, code_map=None):
+        """gmsh 2.2 ASCII mesh; code_map {physical tag: boundary code}."""
+        pairs = np.array([[k, v] for k, v in (code_map or {}).items()], np.int32).reshape(-1)
+        h = _vp()
+        _check(lib().fvk_mesh_read_gmsh(os.fsencode(path), n_dx, l_sc, _i(pairs) if len(pairs) else None,
+                                        len(pairs) // 2, C.byref(h)))
+        return cls(h)
+
+    def elm_idx(self):
+        e = np.zeros(self.n_vol, np.int32)
+        _check(lib().fvk_mesh_get_elm_idx(self.h, _i(e)))
+        return e
+
+    def cells(self):
+        """Vertex coordinates (n_vrt, 3), element types, cell_vptr and the
+        1-based cell vertex lists (Gambit order)."""
+        vx = np.zeros((self.n_vrt, 3))
+        ty = np.zeros(self.n_vol, np.int32)
+        vp = np.zeros(self.n_vol + 1, np.int32)
+        _check(lib().fvk_mesh_get_cells(self.h, None, None, _i(vp), None))
+        vr = np.zeros(int(vp[-1]), np.int32)
+        _check(lib().fvk_mesh_get_cells(self.h, _d(vx), _i(ty), _i(vp), _i(vr)))
+        return vx, ty, vp, vr
 
     @classmethod
     def box(cls, nx, ny, nz, jitter=0.0, seed=0, lid=31, wall=32, L=(1.0, 1.0, 1.0)):
@@ -642,3 +686,49 @@ class Flow:
     def __del__(self):
         if not _finalizing():
             self._release()
+
+
+# ---------------------------------------------------------------- output
+def fmt_list_real(x):
+    """flang's list-directed editing of one REAL(8) value (no leading blank)."""
+    buf = C.create_string_buffer(64)
+    _check(lib().fvk_fmt_list_real(float(x), buf, 64))
+    return buf.value.decode()
+
+
+def _title(kind, igrd, iph, k, ict, n_tl):
+    one = lambda v: str(v) if 0 <= v <= 9 else "*"  # noqa: E731
+    t = "grd%s%%phs%s%%%s%s" % (one(igrd), one(iph), kind, one(k))
+    if n_tl > 1 and ict > 0:
+        t += "@" + ("%04d" % ict if ict <= 9999 else "****")
+    return t
+
+
+def _dir(d):
+    d = os.fspath(d)
+    return d if d.endswith("/") or d == "" else d + "/"
+
+
+def eqn_outp(mesh: Mesh, flow: Flow, dir, igrd=1, iph=1, ieq=1, ict=0, n_tl=1):
+    """Tecplot file of the velocity (cfd_outp.f90 eqn_outp); returns its path."""
+    d = _dir(dir)
+    _check(lib().fvk_eqn_outp(mesh.h, flow.h, igrd, iph, ieq, ict, n_tl, os.fsencode(d)))
+    return d + _title("eqn", igrd, iph, ieq, ict, n_tl) + ".dat"
+
+
+def prp_outp_p(mesh: Mesh, flow: Flow, dir, igrd=1, iph=1, ipr=1, ict=0, n_tl=1):
+    """Tecplot file of the pressure (cfd_outp.f90 prp_outp with p=.true.)."""
+    d = _dir(dir)
+    _check(lib().fvk_prp_outp_p(mesh.h, flow.h, igrd, iph, ipr, ict, n_tl, os.fsencode(d)))
+    return d + _title("prp", igrd, iph, ipr, ict, n_tl) + ".dat"
+
+
+def write_gmsh_field(fname, elm_idx, phi):
+    """gmsh ElementData file of one value per element (flowlab's
+    write_gmsh_field_); phi (n,) or (n, n_dim) for the vector form."""
+    idx = np.ascontiguousarray(elm_idx, np.int32)
+    ph = np.ascontiguousarray(phi, np.float64)
+    if ph.ndim == 1:
+        _check(lib().fvk_write_gmsh_field(os.fsencode(fname), len(idx), _i(idx), _d(ph)))
+    else:
+        _check(lib().fvk_write_gmsh_vector_field(os.fsencode(fname), len(idx), ph.shape[1], _i(idx), _d(ph)))

@@ -123,3 +123,21 @@ def test_simple_p_solver3_solves_the_assembled_system(simpler):
         F.free()
     finally:
         om.free()
+
+
+def test_all_cores_baseline_pcg_is_the_reference_recurrence():
+    """The all-cores CPU baseline's PCG (fvo_pcg_omp: row-gather product,
+    OpenMP) on one thread is the oracle's solv_cg bit for bit (the gather
+    adds each row's terms in the face scatter's order); on several threads
+    only its reductions' rounding differs."""
+    om, pl, a, b = laplacian(9, 0x5EED0045)
+    try:
+        b = b - b.mean()
+        S = fvo.OSys(om, pl, -a, -a)
+        x1 = S.solve("cg", 0, 0.0, 40, b)[0][:, 0]
+        _, x2 = S.pcg_omp(b, 40, 1)
+        assert np.array_equal(x1, x2)
+        _, x4 = S.pcg_omp(b, 40, 4)
+        assert np.max(np.abs(x4 - x1)) <= 1e-10 * np.max(np.abs(x1))
+    finally:
+        om.free()
