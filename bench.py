@@ -280,6 +280,23 @@ def simple_leg(fvk, args, model):
         out["reference_pressure_" + mode] = dict(outer_its_per_s=1.0 / dt, ms_per_outer=dt * 1e3,
                                                  mx_res_n_last=h[-1]["mx_res_n"])
         fl.destroy()
+    # the reference's shipped configuration (cfd_setup.f90:528,581,662,742):
+    # SIMPLER pressure, DILU momentum and DIC pressure preconditioners (level-
+    # scheduled triangular sweeps), urf_p 0.1
+    for mode in ("eager", "graph"):
+        cfg = fvk.flow_cfg()
+        fl = fvk.Flow(c, cfg)
+        fl.set_bnd_u(31, [1.0, 0.0, 0.0])
+        run = fl.run if mode == "eager" else fl.run_graph
+        run(1, 2)
+        c.sync()
+        t0 = time.perf_counter()
+        h = run(3, args.simple_its)
+        c.sync()
+        dt = (time.perf_counter() - t0) / args.simple_its
+        out["shipped_" + mode] = dict(outer_its_per_s=1.0 / dt, ms_per_outer=dt * 1e3,
+                                      mx_res_n_last=h[-1]["mx_res_n"], levels=c.levels())
+        fl.destroy()
     # C2: outer iterations 1..simple_tol_its with every pressure-correction
     # system solved to the FP64 relative L1 residual tol from x = 0, plain
     # FP64 PCG against the learned FP32-inner solve, replayed from HIP
@@ -408,6 +425,56 @@ def channel_leg(fvk, args):
                          "%g" % (warm + 1, warm + its, args.channel, args.tol), **out)
 
 
+def large_leg(fvk, args, model):
+    """The SpMV kernels on a mesh whose working set exceeds the 256 MB
+    Infinity Cache (args.large^3 cells, +-20% jitter, built on the GPU): the
+    plain FP64 SpMV and the learned path's k_mx_q, per launch."""
+    n = args.large
+    fvk.mesh_build_device(0)
+    try:
+        m = fvk.Mesh.box(n, n, n, jitter=args.jitter, seed=0x5EED00E1)
+    finally:
+        fvk.mesh_build_device(-1)
+    c = fvk.grd_solv(m)
+    fvk.sys_random(c, 0x5EED00E2, args.sigma)
+    nnz = m.n_vol + 2 * (m.n_fce - m.n_bf)
+    t = fvk.bench_spmv(c, 100)
+    b = nnz * 12 + 4 * (m.n_vol + 1) + 16 * m.n_vol
+    out = dict(cells=m.n_vol, spmv_fp64=dict(us_per_launch=t * 1e6, bytes=b, gbs=b / t / 1e9, frac=b / t / PEAK_HBM))
+    if model is not None:
+        model.build_factor(c)
+        r = fvk.sys_solve(c, fvk.SOLVE_DC32_FACTOR, 0, args.tol, 200000, time_kernel=True)
+        out["k_mx_q"] = dict(us_per_launch=r["kern_seconds"] * 1e6, bytes=r["kern_bytes"],
+                             gbs=r["kern_bytes"] / r["kern_seconds"] / 1e9,
+                             frac=r["kern_bytes"] / r["kern_seconds"] / PEAK_HBM, iters=r["iters"],
+                             status=r["status"])
+    log("large %d^3: %s" % (n, json.dumps(out)))
+    c.destroy()
+    m.free()
+    return out
+
+
+def setup_leg(fvk, n, jitter):
+    """Mesh connectivity and geometry of the n^3 box: on the host against on
+    the GPU (same arrays bit for bit), and the device context build."""
+    out = {}
+    for dev in (-1, 0):
+        fvk.mesh_build_device(dev)
+        t0 = time.perf_counter()
+        m = fvk.Mesh.box(n, n, n, jitter=jitter, seed=0x5EED0002)
+        out["mesh_" + ("host" if dev < 0 else "gpu") + "_s"] = time.perf_counter() - t0
+        if dev == 0:
+            t0 = time.perf_counter()
+            c = fvk.grd_solv(m)
+            c.sync()
+            out["context_s"] = time.perf_counter() - t0
+            c.destroy()
+        m.free()
+    fvk.mesh_build_device(-1)
+    log("set-up %d^3: %s" % (n, json.dumps(out)))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -420,7 +487,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=300, help="iterations of the oracle's PCG timed")
     ap.add_argument("--cpu-ref-sample", type=int, default=500, help="iterations of the reference's PCG timed")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--methods", default="dc32,learned", help="methods timed beside the plain solve")
+    ap.add_argument("--methods", default="dc32,learned,fsai", help="methods timed beside the plain solve")
     ap.add_argument("--train-lognormal", type=int, default=1600)
     ap.add_argument("--train-simple", type=int, default=400)
     ap.add_argument("--lr", type=float, default=1e-2, help="Adam step of the network's training")
@@ -429,6 +496,8 @@ def main():
     ap.add_argument("--simple-tol-its", type=int, default=100,
                     help="outer iterations of the C2 leg (pressure solves to tol)")
     ap.add_argument("--no-simple", action="store_true")
+    ap.add_argument("--large", type=int, default=160, help="cells per side of the beyond-cache SpMV leg (0: off)")
+    ap.add_argument("--tight", type=float, default=1e-12, help="tolerance of the solution-agreement leg (0: off)")
     ap.add_argument("--channel", type=int, default=0, help="triangles of the optional channel leg (0: off)")
     ap.add_argument("--channel-warm", type=int, default=6)
     ap.add_argument("--channel-its", type=int, default=3)
@@ -438,6 +507,7 @@ def main():
     ap.add_argument("--channel-pretrain", type=int, default=300, help="SIMPLE outer iterations of the training run")
     ap.add_argument("--channel-pretrain-lognormal", type=int, default=3, help="lognormal systems per outer")
     ap.add_argument("--channel-only", action="store_true", help="run the channel leg alone")
+    ap.add_argument("--simple-only", action="store_true", help="run the SIMPLE leg alone (no network)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is "
@@ -449,6 +519,13 @@ def main():
         if args.channel <= 0:
             sys.exit("--channel-only needs --channel N (triangles)")
         print(json.dumps({"channel": channel_leg(fvk, args)}), flush=True)
+        return
+    if args.simple_only:
+        # the SIMPLE leg alone (for kernel profiles), plain pressure solves
+        simple, _ = simple_leg(fvk, args, None)
+        # a 1 GiB device copy: the byte counters' calibration in the same run
+        simple["copy_gbs"] = 2 * (1 << 30) / fvk.bench_copy(1 << 30, 5) / 1e9
+        print(json.dumps({"simple": simple}), flush=True)
         return
     n = args.n
     t0 = time.time()
@@ -507,6 +584,47 @@ def main():
                                        k_mx_q_gbs=rk3["kern_bytes"] / rk3["kern_seconds"] / 1e9,
                                        k_mx_q_bytes=rk3["kern_bytes"],
                                        training=tinfo)
+    if model is not None and args.tight > 0:
+        # the solutions of the plain and the learned solve agree to what the
+        # tolerance fixes (cond x tol); at a tighter tolerance, both timed
+        tight = {}
+        xs = {}
+        for name, meth in (("plain_pcg64_jacobi", fvk.SOLVE_PCG64), ("dc32_learned", fvk.SOLVE_DC32_FACTOR)):
+            if meth == fvk.SOLVE_DC32_FACTOR:
+                model.build_factor(ctx)
+            ctx.sync()
+            t0 = time.perf_counter()
+            rt = fvk.sys_solve(ctx, meth, 0, args.tight, 400000)
+            ctx.sync()
+            tight[name] = dict(s=time.perf_counter() - t0, iters=rt["iters"], res_true_l1=rt["res_true_l1"],
+                               status=rt["status"], fallback=rt["fallback"])
+            xs[name] = fvk.sys_x(ctx)
+        a, b = xs["plain_pcg64_jacobi"], xs["dc32_learned"]
+        tight["tol"] = args.tight
+        tight["x_rel_diff"] = float(np.max(np.abs(a - b)) / np.max(np.abs(a)))
+        tight["speedup"] = tight["plain_pcg64_jacobi"]["s"] / tight["dc32_learned"]["s"]
+        methods["tight"] = tight
+        log("tight %g: %s" % (args.tight, json.dumps(tight)))
+    if "fsai" in args.methods.split(","):
+        # the closed-form FSAI factor on the same pattern: the method the
+        # network learns, built without it (factor build inside the step)
+        def fsai_step():
+            fvk.factor_fsai(ctx)
+            return fvk.sys_solve(ctx, fvk.SOLVE_DC32_FACTOR, 0, args.tol, 200000)
+        for _ in range(args.warmup):
+            fsai_step()
+        ctx.sync()
+        t0 = time.perf_counter()
+        rf = [fsai_step() for _ in range(args.steps)]
+        ctx.sync()
+        wf = (time.perf_counter() - t0) / args.steps
+        methods["dc32_fsai"] = dict(s=wf, iters=rf[-1]["iters"], res_true_l1=rf[-1]["res_true_l1"],
+                                    speedup_vs_plain=wall / wf,
+                                    steps_ok=all(x["status"] == fvk.FVK_OK and x["fallback"] == 0 and
+                                                 x["res_true_l1"] < args.tol for x in rf))
+        log("dc32_fsai timed: %.4f s per solve, %d iterations" % (wf, rf[-1]["iters"]))
+        if "dc32_learned" in methods:
+            methods["dc32_learned"]["iters_vs_fsai"] = methods["dc32_learned"]["iters"] / rf[-1]["iters"]
     if "dc32" in args.methods.split(","):
         w2, r2 = timed(fvk.SOLVE_DC32, "dc32_jacobi")
         rk2 = fvk.sys_solve(ctx, fvk.SOLVE_DC32, 0, args.tol, 200000, time_kernel=True)
@@ -530,6 +648,14 @@ def main():
 
     copy_bytes = 1 << 30
     t_copy = fvk.bench_copy(copy_bytes, 20)
+    # the bandwidth ceiling: streaming probes at 4 GiB (beyond the 256 MB
+    # Infinity Cache), four 16-byte loads in flight per thread
+    sb = 4 << 30
+    ceiling = dict(bytes=sb, copy_gbs=2 * sb / fvk.bench_stream("copy", sb, 10) / 1e9,
+                   read_gbs=sb / fvk.bench_stream("read", sb, 10) / 1e9,
+                   write_gbs=sb / fvk.bench_stream("write", sb, 10) / 1e9)
+    log("streaming probes at 4 GiB: copy %.0f, read %.0f, write %.0f GB/s"
+        % (ceiling["copy_gbs"], ceiling["read_gbs"], ceiling["write_gbs"]))
 
     # headline: the learned-preconditioned mixed-precision solve when timed
     head = methods.get("dc32_learned")
@@ -585,8 +711,12 @@ def main():
         "spmv": dict(us_per_launch=t_spmv * 1e6, bytes=b_spmv, gbs=b_spmv / t_spmv / 1e9,
                      frac=b_spmv / t_spmv / PEAK_HBM),
         "copy_gbs": 2 * copy_bytes / t_copy / 1e9,
+        "ceiling": ceiling,
         "methods": methods,
     }
+    if args.large > 0:
+        out["large"] = large_leg(fvk, args, model)
+    out["setup"] = setup_leg(fvk, n, args.jitter)
     states3 = {}
     if not args.no_simple:
         out["simple"], states3 = simple_leg(fvk, args, model)

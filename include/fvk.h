@@ -195,6 +195,7 @@ typedef struct fvk_solve_info {
   int outer;             /* defect-correction steps (DC methods) */
   int inner_iters[8];    /* iterations of the first 8 inner solves */
   double outer_res[8];   /* FP64 relative L1 residual before each of them */
+  int breakdowns;        /* inner solves that broke down (DC methods) */
 } fvk_solve_info;
 
 #define FVK_SOLVE_PCG64 0 /* the reference PCG recurrence in FP64 (solv_cg.f90) */
@@ -205,10 +206,11 @@ typedef struct fvk_solve_info {
                                    x = 0, with a budget of max_iter of its
                                    own: a call does at most 2 max_iter
                                    iterations) when it does not converge
-                                   within max_iter or breaks down more than
-                                   twice (p.q <= 0 or non-finite; a breakdown
-                                   keeps the finite correction made so far
-                                   and restarts the inner solve):
+                                   within max_iter or breaks down again after
+                                   "dc_max_breakdowns" restarts (default 1;
+                                   p.q <= 0 or non-finite; a breakdown keeps
+                                   the finite correction made so far and
+                                   restarts the inner solve):
                                    FVK_FALLBACK */
 
 /* a_f = s/l_pn * exp(sigma*z_f), z_f ~ N(0,1) from SplitMix64(seed) by face
@@ -230,7 +232,9 @@ int fvk_sys_get_x(fvk_ctx *ctx, double *x);
    direction; each inner solve then aims at max(sqrt(1e-12/n),
    dc_eta_margin tol / current residual); 0: no replacement, and the floor
    is "dc_inner_reduc", default 1e-5), "dc_eta_margin" (default 0.8),
-   "dc_max_outer" (defect-correction steps, default 30), "dc_carry" (1: a
+   "dc_max_outer" (defect-correction steps, default 30),
+   "dc_max_breakdowns" (inner breakdowns followed by a restart from the
+   current x before the solve gives up, default 1), "dc_carry" (1: a
    correction step continues the previous step's search direction; 0:
    restart, default). */
 int fvk_ctx_set_param(fvk_ctx *ctx, const char *name, double value);
@@ -242,6 +246,13 @@ int fvk_ctx_set_param(fvk_ctx *ctx, const char *name, double value);
    counterpart in the reference.  l, sig: n_fce entries in reference face
    order (boundary faces ignored). */
 int fvk_factor_set(fvk_ctx *ctx, const float *l, const float *sig);
+/* The factor of the current matrix by FSAI in closed form (Kolotilina and
+   Yeremin's factorised sparse approximate inverse on A's own pattern:
+   l_f = -Ah_f, Delta_c = (1 - sum of Ah_f^2 over the faces c owns)^-1/2,
+   Ah = D^-1/2 A D^-1/2), exact when no two higher neighbours of a cell are
+   adjacent (hexahedral and quadrilateral meshes).  The method the network
+   learns, built without it; no counterpart in the reference. */
+int fvk_factor_fsai(fvk_ctx *ctx);
 /* z = G G^T r (FP32 inside; host arrays in reference order). */
 int fvk_factor_apply(fvk_ctx *ctx, const double *r, double *z);
 /* Per-face l_f, sig_f of the current factor (n_fce entries, 0 on boundary
@@ -297,6 +308,11 @@ int fvk_bench_spmv(fvk_ctx *ctx, int reps, double *sec_per_launch);
    loads and stores): the achievable-bandwidth reference and the calibration
    kernel for hardware byte counters.  Allocates two scratch buffers. */
 int fvk_bench_copy(int device, long nbytes, int reps, double *sec_per_launch);
+/* Seconds per pass of a streaming probe over nbytes (kind 0: copy, nbytes
+   read and nbytes written; 1: read only; 2: write only), every thread
+   keeping four independent 16-byte loads in flight: the bandwidth ceiling
+   the kernels are measured against.  No counterpart in the reference. */
+int fvk_bench_stream(int device, long nbytes, int reps, int kind, double *sec);
 
 /* ------------------------------------------------------------------------
  * SIMPLE: momentum assembly + BiCGStab (cfd_setup_eqn,

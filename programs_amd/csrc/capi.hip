@@ -869,6 +869,7 @@ int fvk_sys_solve(fvk_ctx *cc, int method, int imat, double tol, int max_iter, i
     info->iters = R.iters;
     info->res_rec = R.res_l1;
     info->outer = R.outer;
+    info->breakdowns = R.breakdowns;
     for (int k = 0; k < 8; ++k) {
       info->inner_iters[k] = R.inner[k];
       info->outer_res[k] = R.res_before[k];
@@ -925,6 +926,8 @@ int fvk_ctx_set_param(fvk_ctx *cc, const char *name, double value) {
   if (!std::strcmp(name, "dc_inner_reduc") && value > 0 && value < 1) X.inner_reduc = value;
   else if (!std::strcmp(name, "dc_max_outer") && value >= 1 && value <= 1000) X.max_outer = (int)value;
   else if (!std::strcmp(name, "dc_carry") && (value == 0 || value == 1)) X.carry = value != 0;
+  else if (!std::strcmp(name, "dc_max_breakdowns") && value >= 0 && value <= 100 && value == (int)value)
+    X.max_breakdowns = (int)value;
   else if (!std::strcmp(name, "dc_eta_margin") && value > 0 && value <= 1) X.eta_margin = value;
   else if (!std::strcmp(name, "dc_replace_every") && value >= 0 && value <= 1e6 && value == (int)value)
     X.replace_every = (int)value;
@@ -993,6 +996,74 @@ int fvk_bench_copy(int device, long nbytes, int reps, double *sec) {
   for (int k = 0; k < 3; ++k) k_copy<<<G, BS>>>(n4, (const double4 *)a, (double4 *)b);
   hipEventRecord(e0, 0);
   for (int k = 0; k < reps; ++k) k_copy<<<G, BS>>>(n4, (const double4 *)a, (double4 *)b);
+  hipEventRecord(e1, 0);
+  hipEventSynchronize(e1);
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipError_t err = hipGetLastError();
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(a);
+  hipFree(b);
+  if (err != hipSuccess) { set_error(hipGetErrorString(err)); return FVK_ERR_HIP; }
+  *sec = ms * 1e-3 / reps;
+  return FVK_OK;
+}
+
+// Streaming probes of the bandwidth ceiling: every thread keeps U
+// independent 16-byte loads in flight (grid-stride over float4 blocks of U).
+constexpr int kStreamU = 4;
+__global__ __launch_bounds__(BS) void k_stream_copy(long n16, const float4 *__restrict__ a, float4 *__restrict__ b) {
+  constexpr int U = kStreamU;
+  const long stride = (long)gridDim.x * BS;
+  for (long i = blockIdx.x * (long)BS + threadIdx.x; i < n16; i += stride * U) {
+    float4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) if (i + u * stride < n16) v[u] = a[i + u * stride];
+#pragma unroll
+    for (int u = 0; u < U; ++u) if (i + u * stride < n16) b[i + u * stride] = v[u];
+  }
+}
+
+__global__ __launch_bounds__(BS) void k_stream_read(long n16, const float4 *__restrict__ a, float *__restrict__ sink) {
+  constexpr int U = kStreamU;
+  const long stride = (long)gridDim.x * BS;
+  float acc = 0.f;
+  for (long i = blockIdx.x * (long)BS + threadIdx.x; i < n16; i += stride * U) {
+    float4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = i + u * stride < n16 ? a[i + u * stride] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc += v[u].x + v[u].y + v[u].z + v[u].w;
+  }
+  if (acc == 1234.5f) sink[0] = acc;  // keeps the loads; never true for the zero-filled input
+}
+
+__global__ __launch_bounds__(BS) void k_stream_fill(long n16, float4 *__restrict__ b) {
+  for (long i = blockIdx.x * (long)BS + threadIdx.x; i < n16; i += (long)gridDim.x * BS)
+    b[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+int fvk_bench_stream(int device, long nbytes, int reps, int kind, double *sec) {
+  if (!sec || reps < 1 || nbytes < 4096 || kind < 0 || kind > 2) { set_error("bad argument"); return FVK_ERR_ARG; }
+  FVK_HIP(hipSetDevice(device));
+  const long n16 = nbytes / 16;
+  void *a = nullptr, *b = nullptr;
+  FVK_HIP(hipMalloc(&a, n16 * 16));
+  if (hipMalloc(&b, n16 * 16) != hipSuccess) { hipFree(a); set_error("hipMalloc failed"); return FVK_ERR_HIP; }
+  hipMemset(a, 0, n16 * 16);
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  const int G = 256 * 16;
+  auto launch = [&]() {
+    if (kind == 0) k_stream_copy<<<G, BS>>>(n16, (const float4 *)a, (float4 *)b);
+    else if (kind == 1) k_stream_read<<<G, BS>>>(n16, (const float4 *)a, (float *)b);
+    else k_stream_fill<<<G, BS>>>(n16, (float4 *)b);
+  };
+  for (int k = 0; k < 3; ++k) launch();
+  hipEventRecord(e0, 0);
+  for (int k = 0; k < reps; ++k) launch();
   hipEventRecord(e1, 0);
   hipEventSynchronize(e1);
   float ms = 0.f;

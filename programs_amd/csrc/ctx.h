@@ -73,6 +73,7 @@ struct Mix {
   int max_outer = 30;
   bool carry = false;                  // continue the Krylov recurrence across correction steps
   int replace_every = 160;             // >0: FP64 residual replacement every that many inner iterations
+  int max_breakdowns = 1;              // inner breakdowns followed by a restart before the solve gives up
 };
 
 struct Ctx {

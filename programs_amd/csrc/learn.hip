@@ -432,6 +432,34 @@ __device__ __forceinline__ double ahat(const FaceIn &F, int q) {
   return F.ngp[f] / sqrt(F.diag[F.fi_p[q]] * F.diag[F.fi_n[q]]);
 }
 
+// FSAI factor in closed form, per column c over the faces q it owns:
+// l_q = -Ah_q, sig_q = -log(1 - sum_q Ah_q^2) / (2 cnt_c).  It is the
+// Kaporin minimiser (FSAI) when no two higher neighbours of a cell are
+// adjacent (hexahedral and quadrilateral meshes); elsewhere an approximation
+// of it.  A column whose sum reaches 1 keeps Jacobi (l = sig = 0).
+__global__ __launch_bounds__(BS) void k_fsai(FaceIn F, int WU, const int *__restrict__ ufi, float *__restrict__ fl,
+                                             float *__restrict__ fsig) {
+  const int c = blockIdx.x * BS + threadIdx.x, n = F.M.n;
+  if (c >= n) return;
+  double q2 = 0.0;
+  int cnt = 0;
+  for (int k = 0; k < WU; ++k) {
+    const int q = ufi[(size_t)k * n + c];
+    if (q < 0) continue;
+    const double a = ahat(F, q);
+    q2 += a * a;
+    ++cnt;
+  }
+  const bool ok = cnt > 0 && q2 < 1.0;
+  const double sg = ok ? -0.5 * log1p(-q2) / cnt : 0.0;
+  for (int k = 0; k < WU; ++k) {
+    const int q = ufi[(size_t)k * n + c];
+    if (q < 0) continue;
+    fl[q] = ok ? (float)(-ahat(F, q)) : 0.f;
+    fsig[q] = (float)sg;
+  }
+}
+
 // Pf_q = l_q + Ah_q + sum over the other faces q2 owned by the same cell c
 // whose neighbour j is adjacent to q's neighbour i of Ah_ij l_q2
 __global__ __launch_bounds__(BS) void k_loss_face(FaceIn F, int WU, const int *__restrict__ ufi,
@@ -751,6 +779,16 @@ int fvk_model_set(fvk_model *m, const float *theta) {
 int fvk_factor_build(fvk_ctx *cc, const fvk_model *mc) {
   if (!cc || !mc) { set_error("null argument"); return FVK_ERR_ARG; }
   return learned_factor(cc->c, const_cast<fvk_model *>(mc));
+}
+
+int fvk_factor_fsai(fvk_ctx *cc) {
+  if (!cc) { set_error("null argument"); return FVK_ERR_ARG; }
+  Ctx &c = cc->c;
+  GUARD_L(mix_build(c));
+  const FaceIn F = face_in(c);
+  k_fsai<<<grid_for(c.n), BS, 0, c.stream>>>(F, c.mx.WU, c.mx.ufi, c.mx.fl, c.mx.fsig);
+  FVK_HIP(hipGetLastError());
+  return factor_assemble(c);
 }
 
 int fvk_factor_features(fvk_ctx *cc, float *X) {

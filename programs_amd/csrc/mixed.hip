@@ -1067,8 +1067,6 @@ static int inner_solve(Ctx &c, bool fac, int n_iter, double reduc, InnerCarry &c
   return 0;
 }
 
-static const int kMaxBreakdowns = 2;
-
 int dc_solve(Ctx &c, bool use_factor, const double *b, double *x, double tol, int max_iter, DcResult &res,
              KTimer *T) {
   int rc = mix_build(c);
@@ -1138,8 +1136,10 @@ int dc_solve(Ctx &c, bool use_factor, const double *b, double *x, double tol, in
     res.iters += its;
     // a breakdown (p.q <= 0 or non-finite) ends the inner solve with the
     // finite correction made so far, which is kept; the next correction
-    // step restarts from that x.  After kMaxBreakdowns the solve gives up.
-    if (broken && ++res.breakdowns > kMaxBreakdowns) res.broken = 1;
+    // step restarts from that x (a fresh inner solve re-derives its scales
+    // from the new residual).  After X.max_breakdowns restarts the solve
+    // gives up and the caller's fallback takes over.
+    if (broken && ++res.breakdowns > X.max_breakdowns) res.broken = 1;
     k_mx_add<<<grid_for(n), BS, 0, c.stream>>>(n, x, fv(c, V_E), use_factor ? fv(c, V_P0) : nullptr,
                                                (float)alpha_pend, c.diag, sigma);
     FVK_HIP(hipGetLastError());

@@ -57,7 +57,7 @@ class SolveInfo(C.Structure):
     _fields_ = [("iters", C.c_int), ("fallback", C.c_int), ("seconds", C.c_double), ("res_rec", C.c_double),
                 ("res_true_l1", C.c_double), ("res_true_l2", C.c_double), ("kern_seconds", C.c_double),
                 ("kern_bytes", C.c_double), ("kern_launches", C.c_long), ("outer", C.c_int),
-                ("inner_iters", C.c_int * 8), ("outer_res", C.c_double * 8)]
+                ("inner_iters", C.c_int * 8), ("outer_res", C.c_double * 8), ("breakdowns", C.c_int)]
 
 
 SOLVE_PCG64 = 0         # reference PCG recurrence, FP64
@@ -151,6 +151,7 @@ def lib():
         "fvk_factor_apply": [_vp, _dp, _dp],
         "fvk_factor_get": [_vp, _fp, _fp],
         "fvk_factor_features": [_vp, _fp],
+        "fvk_factor_fsai": [_vp],
         "fvk_model_nparam": [],
         "fvk_model_create": [C.c_int, C.c_uint64, C.POINTER(_vp)],
         "fvk_model_get": [_vp, _fp],
@@ -161,6 +162,7 @@ def lib():
         "fvk_sys_get": [_vp, _dp, _dp, _dp],
         "fvk_bench_spmv": [_vp, C.c_int, _dp],
         "fvk_bench_copy": [C.c_int, C.c_long, C.c_int, _dp],
+        "fvk_bench_stream": [C.c_int, C.c_long, C.c_int, C.c_int, _dp],
         "fvk_flow_create": [_vp, C.POINTER(FlowCfg), C.POINTER(_vp)],
         "fvk_flow_set_bnd_u": [_vp, C.c_int, _dp],
         "fvk_flow_set_state": [_vp] + [_dp] * 5,
@@ -510,6 +512,11 @@ def factor_get(ctx: Ctx):
     return l, sig
 
 
+def factor_fsai(ctx: Ctx):
+    """Factor of the current matrix by FSAI in closed form (on the GPU)."""
+    _check(lib().fvk_factor_fsai(ctx.h))
+
+
 def factor_features(ctx: Ctx):
     """The network's twelve inputs per face as it reads them (bfloat16
     values), (n_fce, 12), reference face order, zero rows on boundary faces."""
@@ -585,6 +592,14 @@ def bench_copy(nbytes=1 << 30, reps=20, device=0):
     """Seconds per device copy of nbytes (read nbytes + write nbytes)."""
     s = C.c_double()
     _check(lib().fvk_bench_copy(device, int(nbytes), reps, C.byref(s)))
+    return s.value
+
+
+def bench_stream(kind, nbytes=1 << 30, reps=20, device=0):
+    """Seconds per pass of a streaming probe: kind "copy" (nbytes read and
+    written), "read" or "write"; four 16-byte loads in flight per thread."""
+    s = C.c_double()
+    _check(lib().fvk_bench_stream(device, int(nbytes), reps, {"copy": 0, "read": 1, "write": 2}[kind], C.byref(s)))
     return s.value
 
 

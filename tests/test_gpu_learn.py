@@ -202,7 +202,10 @@ def test_gpu_features_match_oracle(kind):
     Xg = Xg[inner].astype(np.float64)
     Xo = L.features(S)
     err = np.abs(Xg - Xo)
+    # log(|V_p|/|V_n|) is taken in FP32 of the rounded ratio: an absolute
+    # error up to a few FP32 ulps of 1 where the ratio is near 1
     bound = 2.0 * bf16_ulp_bound(Xo) + 1e-6 * np.abs(Xo)
+    bound[:, 11] += 4 * 2.0 ** -23
     bad = err > bound
     assert not np.any(bad), (np.argwhere(bad)[:5], Xg[bad][:5], Xo[bad][:5])
     # each of the twelve features varies on these meshes (nothing constant
@@ -228,7 +231,7 @@ def test_gpu_features_closed_form_on_uniform_hex():
     np.add.at(k, t["fce_ip"][inner] - 1, 1)
     np.add.at(k, t["fce_in"][inner] - 1, 1)
     full = (k[t["fce_ip"][inner] - 1] == 6) & (k[t["fce_in"][inner] - 1] == 6)
-    sixth = float(L.bf16(np.float32(1.0 / 6.0)))
+    sixth = float(L.bf16(np.array([1.0 / 6.0], np.float32))[0])
     assert full.sum() == 3 * (n - 2) ** 2 * (n - 3)
     Xi = X[full]
     assert np.all(Xi[:, :3] == sixth)
@@ -249,10 +252,14 @@ def test_network_against_closed_form_fsai():
     fvk = require_lib()
     model = fvk.Model(seed=9)
     try:
-        for seed in (0x91, 0x92, 0x93):
-            _, m, ctx, S, inner = box_case(14, 0.2, seed)
-            model.train(ctx, 150, lr=3e-3)
-            ctx.destroy()
+        # two passes over six training systems (a network trained on three
+        # systems for 150 steps each can extrapolate to large l on a
+        # held-out system and take thousands of iterations there)
+        for _ in range(2):
+            for seed in (0x91, 0x92, 0x93, 0x94, 0x95, 0x96):
+                _, m, ctx, S, inner = box_case(14, 0.2, seed)
+                model.train(ctx, 100, lr=3e-3)
+                ctx.destroy()
         for seed in (0xA1, 0xA2):
             _, m, ctx, S, inner = box_case(16, 0.2, seed)
             jac = fvk.sys_solve(ctx, fvk.SOLVE_DC32, 0, 1e-8)

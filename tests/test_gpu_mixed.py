@@ -343,3 +343,30 @@ def test_non_finite_rhs_is_an_argument_error():
         assert out["status"] == fvk.FVK_OK and true_res(A, fvk.sys_x(ctx), b) < 1e-8
     finally:
         om.free()
+
+
+def test_gpu_fsai_closed_form_matches_oracle():
+    """fvk_factor_fsai builds on the GPU the factor the oracle's
+    fsai_closed_form gives (l, sig to float32 rounding), and the solve with
+    it takes the iterations of the oracle's factor loaded by fvk_factor_set."""
+    fvk, m, ctx, om, S, A, b = setup(14, 0.2, 0x59)
+    try:
+        fvk.factor_fsai(ctx)
+        l, sig = fvk.factor_get(ctx)
+        lf, sf = L.fsai_closed_form(S)
+        inner = om.topology()["fce_in"] > 0
+        assert np.allclose(l[inner], lf, rtol=2e-7, atol=0)
+        assert np.allclose(sig[inner], sf, rtol=2e-6, atol=1e-12)
+        assert not np.any(l[~inner]) and not np.any(sig[~inner])
+        a = fvk.sys_solve(ctx, fvk.SOLVE_DC32_FACTOR, 0, 1e-8)
+        ll = np.zeros(ctx.n_fce, np.float32)
+        ss = np.zeros(ctx.n_fce, np.float32)
+        ll[inner] = lf
+        ss[inner] = sf
+        fvk.factor_set(ctx, ll, ss)
+        b2 = fvk.sys_solve(ctx, fvk.SOLVE_DC32_FACTOR, 0, 1e-8)
+        print("fsai on the GPU", a["iters"], "from the oracle", b2["iters"])
+        assert a["status"] == fvk.FVK_OK and a["fallback"] == 0 and a["res_true_l1"] < 1e-8
+        assert abs(a["iters"] - b2["iters"]) <= 2
+    finally:
+        om.free()

@@ -94,6 +94,27 @@ def test_cavity_32cube_200_outer_iterations():
     assert hist[-1]["mx_res_n"] < 1e-2
 
 
+@pytest.mark.timeout(300)
+def test_cavity_32cube_200_outer_iterations_pressure_to_tolerance():
+    """The same 200 iterations with every pressure correction solved to a
+    relative L1 residual of 1e-10 (p_solver 3): without the reference's
+    20-iteration, reduc-0.1 exits (whose discrete stops carry rounding
+    differences into the trajectory), GPU and oracle agree to 1e-10."""
+    fvk = require_lib()
+    n, n_outer = 32, 200
+    kw = dict(RELAXED, p_solver=3, p_tol=1e-10)
+    om = fvo.OMesh.box(n, n, n)
+    try:
+        t0 = time.time()
+        H, F = oracle_run(om, kw, n_outer)
+        print("oracle: %.1f s" % (time.time() - t0), flush=True)
+    finally:
+        om.free()
+    m = fvk.Mesh.box(n, n, n)
+    hist, s = gpu_run(fvk, m, {k: v for k, v in kw.items()}, n_outer, graph=True)
+    compare(hist, s, H, F, TOL, "32^3 x 200 p_tol 1e-10")
+
+
 @pytest.mark.timeout(600)
 def test_cavity_128cube_pressure_to_tolerance_3_outer():
     fvk = require_lib()
