@@ -154,6 +154,20 @@ int fvk_solv_direct(fvk_ctx *ctx, int imat, const double *c, const double *c_r, 
    not_r0, res0_n = per-component L1 norm. */
 int fvk_solv_res(fvk_ctx *ctx, int n_vc, const double *phi, double *res0, double *res0_n,
                  int not_r0);
+/* fvk_solv_res on device memory, allocating nothing.  No counterpart in the
+   reference: for callers that cannot allocate device memory (code under
+   HIP graph capture, tight serving loops).  phi_dev and res0_dev are device
+   arrays of n_vc components, component-major in the context's row order
+   (element (row, q) at [q*n_vol + row]; the reference's order unless the
+   context was built with FVK_CTX_RCM); res0_n_dev is a device array of n_vc
+   doubles.  The work is queued on the context's stream and nothing waits
+   for it (fvk_ctx_sync does).  Its reduction partials live in a fixed
+   48 KiB static device array per GPU (every block's partials are written
+   before they are read: nothing carries over between calls); n_vc is at
+   most 3.  Calls that overlap on one GPU (contexts on other streams) race
+   on that array. */
+int fvk_solv_res_dev(fvk_ctx *ctx, int n_vc, const double *phi_dev, double *res0_dev, double *res0_n_dev,
+                     int not_r0);
 
 typedef struct fvk_stats {
   int iters[3];      /* iterations per component */

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -200,6 +201,34 @@ __global__ __launch_bounds__(BS) void k_slot_geom(DMesh M, double *gs, double *g
   }
 }
 
+// Scratch of fvk_solv_res_dev, the entry point that allocates nothing: the
+// per-block partial sums of the residual kernel (6 per block, at most RG
+// blocks: 48 KiB).  Each GPU holds its own copy until the process ends.
+// Host code never uses the array's name as an address: the device address
+// of each GPU's copy is taken once per device, with that device current
+// (scratch_init, from the context build), and kept in g_scratch.
+static __device__ __align__(256) double scratch[6 * RG];
+static std::mutex g_scratch_mu;
+static const int kMaxDevices = 64;
+static double *g_scratch[kMaxDevices] = {};
+
+static int scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) {
+    set_error("device index beyond the scratch table");
+    return FVK_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[device]) return 0;
+  double *ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    return FVK_ERR_HIP;
+  }
+  g_scratch[device] = ptr;
+  return 0;
+}
+
 static int ctx_build(Ctx &c, const Topo &t, int device, int flags) {
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device) {
@@ -208,6 +237,10 @@ static int ctx_build(Ctx &c, const Topo &t, int device, int flags) {
   }
   c.device = device;
   FVK_HIP(hipSetDevice(device));
+  {
+    int rc = scratch_init(device);
+    if (rc) return rc;
+  }
   FVK_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
   FVK_HIP(hipEventCreate(&c.ev0));
   FVK_HIP(hipEventCreate(&c.ev1));
@@ -691,6 +724,17 @@ int fvk_solv_res(fvk_ctx *cc, int n_vc, const double *phi, double *res0, double 
   FVK_HIP(hipMemcpy(h, c.ks, sizeof h, hipMemcpyDeviceToHost));
   for (int q = 0; q < n_vc; ++q) res0_n[q] = h[q].res0_n;
   return FVK_OK;
+}
+
+int fvk_solv_res_dev(fvk_ctx *cc, int n_vc, const double *phi_dev, double *res0_dev, double *res0_n_dev,
+                     int not_r0) {
+  if (!cc || !res0_dev || !res0_n_dev || (!not_r0 && !phi_dev)) { set_error("null argument"); return FVK_ERR_ARG; }
+  if (n_vc < 1 || n_vc > 3) { set_error("n_vc must be 1, 2 or 3 (the scratch holds 3 components)"); return FVK_ERR_ARG; }
+  Ctx &c = cc->c;
+  double *part = (c.device >= 0 && c.device < kMaxDevices) ? g_scratch[c.device] : nullptr;
+  if (!part) { set_error("scratch of this device not initialised"); return FVK_ERR_STATE; }
+  FVK_HIP(hipSetDevice(c.device));
+  return residual_to(c, n_vc, phi_dev, res0_dev, not_r0 != 0, part, res0_n_dev);
 }
 
 static int solve_common(fvk_ctx *cc, int kind, int imat, double reduc, int n_iter, int n_vc, double *phi,

@@ -266,6 +266,7 @@ int true_residual(Ctx &c, const double *x, const double *b, double *dev_out4, do
 // a mean |b| of 1 first: the recurrence's alpha = s/(p.q + small)
 // (solv_cg.f90:69-70) has an absolute small = 1e-20 that stalls solves whose
 // p.q gets that small.  out: the final scalars, res0_n = |b|_1.
+int residual_to(Ctx &c, int nvc, const double *x, double *r, bool not_r0, double *part, double *out_dev);
 int pcg_tol(Ctx &c, int imat, const double *b, double *x, double tol, int max_iter, int chunk, KTimer *T,
             KS &out);
 

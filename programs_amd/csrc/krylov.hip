@@ -641,6 +641,32 @@ int residual(Ctx &c, int nvc, const double *x, double *r, bool not_r0) {
   return 0;
 }
 
+__global__ __launch_bounds__(RB64) void k_res_final_to(int nvc, const double *partU, int PU, double *out) {
+  double t[6];
+  load_totals<6, RB64>(partU, PU, t);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < nvc; ++q) out[q] = t[q];
+}
+
+// residual() with the per-block partials in `part` (6 RG doubles) and the
+// L1 norms written to the device array out_dev (nvc doubles): nothing
+// outside the inputs, the outputs and `part` is touched, and nothing waits
+// for the device.
+int residual_to(Ctx &c, int nvc, const double *x, double *r, bool not_r0, double *part, double *out_dev) {
+  const int G = red_grid64(c.n);
+  if (G > RG) {
+    set_error("partials exceed the scratch");
+    return FVK_ERR_ARG;
+  }
+#define L(W) \
+  k_residual<W><<<G, RB64, 0, c.stream>>>(c.M, nvc, c.diag, c.spv, x, r, not_r0 ? 1 : 0, 0, nullptr, nullptr, part)
+  FVK_WS(c.M.Ws, L);
+#undef L
+  k_res_final_to<<<1, RB64, 0, c.stream>>>(nvc, part, G, out_dev);
+  FVK_HIP(hipGetLastError());
+  return 0;
+}
+
 static int all_done_host(Ctx &c, int nvc, bool &done) {
   KS h[3];
   FVK_HIP(hipMemcpyAsync(h, c.ks, sizeof(KS) * nvc, hipMemcpyDeviceToHost, c.stream));

@@ -139,6 +139,7 @@ def lib():
         "fvk_flow_configure": [_vp, C.POINTER(FlowCfg)],
         "fvk_flow_scalars": [_vp, C.c_int, _dp, _dp],
         "fvk_solv_res": [_vp, C.c_int, _dp, _dp, _dp, C.c_int],
+        "fvk_solv_res_dev": [_vp, C.c_int, _vp, _vp, _vp, C.c_int],
         "fvk_solv_cg": [_vp, C.c_int, C.c_double, C.c_int, C.c_int, _dp, _dp, _dp, C.c_int, C.POINTER(Stats)],
         "fvk_solv_bicg": [_vp, C.c_int, C.c_double, C.c_int, C.c_int, _dp, _dp, _dp, C.c_int, C.POINTER(Stats)],
         "fvk_solv_grad": [_vp, C.c_int, _dp, _dp, C.c_int, C.c_int],
@@ -418,6 +419,12 @@ def solv_res(ctx: Ctx, phi, res0, not_r0=False):
     _check(lib().fvk_solv_res(ctx.h, nvc, _d(ph), _d(col), _d(rn), int(bool(not_r0))))
     res0[...] = col.reshape(nvc, ctx.n_vol).T
     return rn
+
+
+def solv_res_dev(ctx: Ctx, n_vc, phi_ptr, res0_ptr, res0_n_ptr, not_r0=False):
+    """fvk_solv_res_dev on device pointers (integers): component-major
+    arrays in the context's row order; queued on the context's stream."""
+    _check(lib().fvk_solv_res_dev(ctx.h, n_vc, phi_ptr, res0_ptr, res0_n_ptr, int(bool(not_r0))))
 
 
 def _solve(fn, ctx, imat, reduc, n_iter, phi, res0, not_r0):
