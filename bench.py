@@ -268,7 +268,10 @@ def simple_leg(fvk, args, model):
         fl.set_bnd_u(31, [1.0, 0.0, 0.0])
         return fl
 
-    for mode in ("eager", "graph"):
+    modes = ("eager",) if args.no_graph else ("eager", "graph")
+    legs_on = set(args.simple_legs.split(","))
+    for mode in modes if "reference" in legs_on else ():
+        log("SIMPLE %d^3: reference pressure solves, %s" % (n, mode))
         fl = flow()
         run = fl.run if mode == "eager" else fl.run_graph
         run(1, 2)
@@ -281,28 +284,52 @@ def simple_leg(fvk, args, model):
                                                  mx_res_n_last=h[-1]["mx_res_n"])
         fl.destroy()
     # the reference's shipped configuration (cfd_setup.f90:528,581,662,742):
-    # SIMPLER pressure, DILU momentum and DIC pressure preconditioners (level-
-    # scheduled triangular sweeps), urf_p 0.1
-    for mode in ("eager", "graph"):
+    # SIMPLER pressure, DILU momentum and DIC pressure preconditioners, urf_p
+    # 0.1; factor and triangular sweeps in one launch each (rows wait for the
+    # values they need), and for comparison one launch per level
+    # one application of the DIC / DILU factor (both sweeps), per-level
+    # launches against one launch per sweep at 1, 2 and 4 resident blocks per CU
+    if "shipped" in legs_on:
+        fvk.sys_random(c, 0x5EED0003, 1.0)
+        sw = {"levels": c.levels()}
+        for imat, nvc in ((1, 1), (2, 3)):
+            key = "imat%d_nvc%d" % (imat, nvc)
+            fvk.set_param(c, "sweep_one_launch", 0)
+            sw[key] = {"per_level_launches_us": fvk.bench_sweep(c, imat, nvc, 5) * 1e6}
+            fvk.set_param(c, "sweep_one_launch", 1)
+            for bpc in (1, 2, 4):
+                fvk.set_param(c, "sweep_blocks_per_cu", bpc)
+                sw[key]["one_launch_bpc%d_us" % bpc] = fvk.bench_sweep(c, imat, nvc, 20) * 1e6
+        fvk.set_param(c, "sweep_blocks_per_cu", 2)
+        out["sweep_apply"] = sw
+        log("SIMPLE %d^3: factor application %s" % (n, json.dumps(sw)))
+    shipped = [(m_, 1) for m_ in modes] + [("eager", 0)] if "shipped" in legs_on else []
+    for mode, one in shipped:
+        log("SIMPLE %d^3: shipped configuration, %s, %s" % (n, mode, "one launch per sweep" if one
+                                                            else "one launch per level"))
+        fvk.set_param(c, "sweep_one_launch", one)
         cfg = fvk.flow_cfg()
         fl = fvk.Flow(c, cfg)
         fl.set_bnd_u(31, [1.0, 0.0, 0.0])
         run = fl.run if mode == "eager" else fl.run_graph
         run(1, 2)
         c.sync()
+        its = args.simple_its if one else max(2, args.simple_its // 4)
         t0 = time.perf_counter()
-        h = run(3, args.simple_its)
+        h = run(3, its)
         c.sync()
-        dt = (time.perf_counter() - t0) / args.simple_its
-        out["shipped_" + mode] = dict(outer_its_per_s=1.0 / dt, ms_per_outer=dt * 1e3,
-                                      mx_res_n_last=h[-1]["mx_res_n"], levels=c.levels())
+        dt = (time.perf_counter() - t0) / its
+        key = "shipped_" + mode + ("" if one else "_per_level_launches")
+        out[key] = dict(outer_its_per_s=1.0 / dt, ms_per_outer=dt * 1e3, outer_iterations=its,
+                        mx_res_n_last=h[-1]["mx_res_n"], levels=c.levels())
         fl.destroy()
+    fvk.set_param(c, "sweep_one_launch", 1)
     # C2: outer iterations 1..simple_tol_its with every pressure-correction
     # system solved to the FP64 relative L1 residual tol from x = 0, plain
     # FP64 PCG against the learned FP32-inner solve, replayed from HIP
     # graphs; the fields after iteration 3 go to the oracle comparison
-    legs = [("plain_pcg64", 3, None)]
-    if model is not None:
+    legs = [("plain_pcg64", 3, None)] if "c2" in legs_on else []
+    if model is not None and legs:
         legs.append(("dc32_learned", 2, model))
     states3, final = {}, {}
     N = max(args.simple_tol_its, 4)
@@ -310,14 +337,15 @@ def simple_leg(fvk, args, model):
         fl = flow(p_solver=ps, p_tol=args.tol)
         if mdl is not None:
             fl.set_model(mdl, train_steps=0)
+        run = fl.run if args.no_graph else fl.run_graph
         c.sync()
         t0 = time.perf_counter()
-        h = fl.run_graph(1, 3)
+        h = run(1, 3)
         c.sync()
         t3 = time.perf_counter() - t0
         states3[name] = fl.state()
         t0 = time.perf_counter()
-        h += fl.run_graph(4, N - 3)
+        h += run(4, N - 3)
         c.sync()
         dt = t3 + time.perf_counter() - t0
         final[name] = fl.state()
@@ -496,6 +524,12 @@ def main():
     ap.add_argument("--simple-tol-its", type=int, default=100,
                     help="outer iterations of the C2 leg (pressure solves to tol)")
     ap.add_argument("--no-simple", action="store_true")
+    ap.add_argument("--simple-legs", default="reference,shipped,c2",
+                    help="SIMPLE legs run: reference (reference pressure solves), shipped (SIMPLER, DILU/DIC), "
+                         "c2 (pressure solves to tolerance)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="SIMPLE legs eager only (profiling: graph replays under rocprofv3's kernel trace "
+                         "ended in a host segmentation fault inside the HIP runtime)")
     ap.add_argument("--large", type=int, default=160, help="cells per side of the beyond-cache SpMV leg (0: off)")
     ap.add_argument("--tight", type=float, default=1e-12, help="tolerance of the solution-agreement leg (0: off)")
     ap.add_argument("--channel", type=int, default=0, help="triangles of the optional channel leg (0: off)")
@@ -649,13 +683,19 @@ def main():
     copy_bytes = 1 << 30
     t_copy = fvk.bench_copy(copy_bytes, 20)
     # the bandwidth ceiling: streaming probes at 4 GiB (beyond the 256 MB
-    # Infinity Cache), four 16-byte loads in flight per thread
+    # Infinity Cache), four 16-byte loads in flight per thread, and eight
+    # non-temporal ones from a grid that is resident at once
     sb = 4 << 30
     ceiling = dict(bytes=sb, copy_gbs=2 * sb / fvk.bench_stream("copy", sb, 10) / 1e9,
                    read_gbs=sb / fvk.bench_stream("read", sb, 10) / 1e9,
-                   write_gbs=sb / fvk.bench_stream("write", sb, 10) / 1e9)
-    log("streaming probes at 4 GiB: copy %.0f, read %.0f, write %.0f GB/s"
-        % (ceiling["copy_gbs"], ceiling["read_gbs"], ceiling["write_gbs"]))
+                   write_gbs=sb / fvk.bench_stream("write", sb, 10) / 1e9,
+                   copy_nt_gbs=2 * sb / fvk.bench_stream("copy_nt", sb, 10) / 1e9,
+                   read_nt_gbs=sb / fvk.bench_stream("read_nt", sb, 10) / 1e9)
+    ceiling["best_read_gbs"] = max(ceiling["read_gbs"], ceiling["read_nt_gbs"])
+    ceiling["best_copy_gbs"] = max(ceiling["copy_gbs"], ceiling["copy_nt_gbs"])
+    log("streaming probes at 4 GiB: copy %.0f / %.0f (nt), read %.0f / %.0f (nt), write %.0f GB/s"
+        % (ceiling["copy_gbs"], ceiling["copy_nt_gbs"], ceiling["read_gbs"], ceiling["read_nt_gbs"],
+           ceiling["write_gbs"]))
 
     # headline: the learned-preconditioned mixed-precision solve when timed
     head = methods.get("dc32_learned")
@@ -730,7 +770,7 @@ def main():
         out["cpu_baseline_port"] = port
         out["cpu_baseline_all_cores"] = cpu_baseline_all_cores(ctx, fvk, n, args.jitter, r["iters"],
                                                                args.cpu_sample)
-        if not args.no_simple:
+        if not args.no_simple and states3:
             out["cpu_baseline_simple"] = cpu_simple(args, states3)
     print(json.dumps(out), flush=True)
     ctx.destroy()

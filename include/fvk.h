@@ -318,14 +318,21 @@ int fvk_model_train(fvk_model *m, fvk_ctx *ctx, int steps, double lr, double *lo
 int fvk_sys_get(fvk_ctx *ctx, double *pl, double *a, double *b);
 /* Mean duration of one FP64 SpMV launch over reps launches (HIP events). */
 int fvk_bench_spmv(fvk_ctx *ctx, int reps, double *sec_per_launch);
+/* Seconds per application of the DIC (imat 1) / DILU (imat 2) factor of
+   the context's current matrix (forward and backward sweep) to nvc
+   right-hand sides, as the solvers apply it.  No counterpart in the
+   reference. */
+int fvk_bench_sweep(fvk_ctx *ctx, int imat, int nvc, int reps, double *sec_per_apply);
 /* Mean duration of a device copy of nbytes (kernel k_copy, 16-byte vector
    loads and stores): the achievable-bandwidth reference and the calibration
    kernel for hardware byte counters.  Allocates two scratch buffers. */
 int fvk_bench_copy(int device, long nbytes, int reps, double *sec_per_launch);
 /* Seconds per pass of a streaming probe over nbytes (kind 0: copy, nbytes
-   read and nbytes written; 1: read only; 2: write only), every thread
-   keeping four independent 16-byte loads in flight: the bandwidth ceiling
-   the kernels are measured against.  No counterpart in the reference. */
+   read and nbytes written; 1: read only; 2: write only; every thread
+   keeping four independent 16-byte loads in flight; 3: copy and 4: read
+   with eight non-temporal 16-byte loads in flight and one resident grid):
+   the bandwidth ceiling the kernels are measured against.  No counterpart
+   in the reference. */
 int fvk_bench_stream(int device, long nbytes, int reps, int kind, double *sec);
 
 /* ------------------------------------------------------------------------

@@ -341,6 +341,13 @@ static int ctx_build(Ctx &c, const Topo &t, int device, int flags) {
   if ((rc = dalloc(c, &c.ngn, nf))) return rc;
   if ((rc = dalloc(c, &c.pc_c, n))) return rc;
   if ((rc = dalloc(c, &c.pc_cr, n))) return rc;
+  if ((rc = dalloc(c, &c.sweep_t, (size_t)3 * n))) return rc;
+  if ((rc = dalloc(c, &c.sweep_int, 4))) return rc;
+  if ((rc = dalloc(c, &c.sweep_lmask, n))) return rc;
+  if ((rc = dalloc(c, &c.sweep_umask, n))) return rc;
+  FVK_HIP(hipDeviceGetAttribute(&c.n_cu, hipDeviceAttributeMultiprocessorCount, c.device));
+  if (Ws > 8) c.sweep_one_launch = false;  // slot masks hold 8 slots
+  if ((rc = sweep_masks_build(c))) return rc;
   if ((rc = dalloc(c, &c.wk, (size_t)8 * wk_ld(n)))) return rc;
   if ((rc = dalloc(c, &c.ks, 3))) return rc;
   if ((rc = dalloc(c, &c.ksr, 6))) return rc;
@@ -630,7 +637,7 @@ void fvk_ctx_destroy(fvk_ctx *c) {
 int fvk_ctx_sync(fvk_ctx *c) {
   if (!c) return FVK_ERR_ARG;
   FVK_HIP(hipStreamSynchronize(c->c.stream));
-  return FVK_OK;
+  return sweep_status(c->c);
 }
 
 int fvk_mesh_rcm(const fvk_mesh *m, int *perm) {
@@ -671,6 +678,7 @@ int fvk_solv_precond(fvk_ctx *cc, int imat, double *cv, double *c_r) {
   if (!cc || imat < 0 || imat > 2) { set_error("bad argument"); return FVK_ERR_ARG; }
   Ctx &c = cc->c;
   GUARD(precond_build(c, imat));
+  GUARD(sweep_status(c));
   if (cv) GUARD(to_host(c, c.pc_c, 1, 1, c.n, &c.perm, cv));
   if (c_r) GUARD(to_host(c, c.pc_cr, 1, 1, c.n, &c.perm, c_r));
   return FVK_OK;
@@ -690,6 +698,7 @@ int fvk_solv_direct(fvk_ctx *cc, int imat, const double *cv, const double *c_r, 
   double *db = wk_slot(c, 5), *dx = wk_slot(c, 6);
   GUARD(to_dev(c, b, 1, 1, c.n, &c.perm, db));
   GUARD(precond_apply(c, imat, 1, db, dx, nullptr));
+  GUARD(sweep_status(c));
   GUARD(to_host(c, dx, 1, 1, c.n, &c.perm, x));
   return FVK_OK;
 }
@@ -767,11 +776,13 @@ static int solve_common(fvk_ctx *cc, int kind, int imat, double reduc, int n_ite
 
 int fvk_solv_cg(fvk_ctx *c, int imat, double reduc, int n_iter, int n_vc, double *phi, double *res0, double *res0_n,
                 int not_r0, fvk_stats *st) {
-  return solve_common(c, 0, imat, reduc, n_iter, n_vc, phi, res0, res0_n, not_r0, st);
+  const int rc = solve_common(c, 0, imat, reduc, n_iter, n_vc, phi, res0, res0_n, not_r0, st);
+  return rc ? rc : sweep_status(c->c);
 }
 int fvk_solv_bicg(fvk_ctx *c, int imat, double reduc, int n_iter, int n_vc, double *phi, double *res0,
                   double *res0_n, int not_r0, fvk_stats *st) {
-  return solve_common(c, 1, imat, reduc, n_iter, n_vc, phi, res0, res0_n, not_r0, st);
+  const int rc = solve_common(c, 1, imat, reduc, n_iter, n_vc, phi, res0, res0_n, not_r0, st);
+  return rc ? rc : sweep_status(c->c);
 }
 
 int fvk_solv_grad(fvk_ctx *cc, int n_vc, const double *phi, double *gout, int n_iter, int neu) {
@@ -975,6 +986,10 @@ int fvk_ctx_set_param(fvk_ctx *cc, const char *name, double value) {
   else if (!std::strcmp(name, "dc_eta_margin") && value > 0 && value <= 1) X.eta_margin = value;
   else if (!std::strcmp(name, "dc_replace_every") && value >= 0 && value <= 1e6 && value == (int)value)
     X.replace_every = (int)value;
+  else if (!std::strcmp(name, "sweep_one_launch") && (value == 0 || value == 1))
+    cc->c.sweep_one_launch = value != 0 && cc->c.Ws <= 8;
+  else if (!std::strcmp(name, "sweep_blocks_per_cu") && value >= 1 && value <= 8 && value == (int)value)
+    cc->c.sweep_blocks_per_cu = (int)value;
   else { set_error(std::string("unknown parameter or value out of range: ") + name); return FVK_ERR_ARG; }
   return FVK_OK;
 }
@@ -1019,6 +1034,25 @@ int fvk_bench_spmv(fvk_ctx *cc, int reps, double *sec) {
   FVK_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
   *sec = ms * 1e-3 / reps;
   return FVK_OK;
+}
+
+int fvk_bench_sweep(fvk_ctx *cc, int imat, int nvc, int reps, double *sec) {
+  if (!cc || !sec || reps < 1 || imat < 1 || imat > 2 || nvc < 1 || nvc > 3) {
+    set_error("bad argument");
+    return FVK_ERR_ARG;
+  }
+  Ctx &c = cc->c;
+  double *b = wk_slot(c, 0), *x = wk_slot(c, 1);
+  GUARD(precond_build(c, imat));
+  for (int k = 0; k < 2; ++k) GUARD(precond_apply(c, imat, nvc, b, x, nullptr));
+  FVK_HIP(hipEventRecord(c.ev0, c.stream));
+  for (int k = 0; k < reps; ++k) GUARD(precond_apply(c, imat, nvc, b, x, nullptr));
+  FVK_HIP(hipEventRecord(c.ev1, c.stream));
+  FVK_HIP(hipEventSynchronize(c.ev1));
+  float ms = 0.f;
+  FVK_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+  *sec = ms * 1e-3 / reps;
+  return sweep_status(c);
 }
 
 __global__ __launch_bounds__(BS) void k_copy(long n4, const double4 *__restrict__ a, double4 *__restrict__ b) {
@@ -1088,8 +1122,42 @@ __global__ __launch_bounds__(BS) void k_stream_fill(long n16, float4 *__restrict
     b[i] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+// Eight 16-byte non-temporal loads in flight per thread, one grid of 8
+// blocks per CU (full occupancy, no tail wave).
+typedef float nt4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(BS) void k_stream_copy_nt(long n16, const nt4 *__restrict__ a, nt4 *__restrict__ b) {
+  constexpr int U = 8;
+  const long stride = (long)gridDim.x * BS;
+  for (long i = blockIdx.x * (long)BS + threadIdx.x; i < n16; i += stride * U) {
+    nt4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (i + u * stride < n16) v[u] = __builtin_nontemporal_load(a + i + u * stride);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (i + u * stride < n16) __builtin_nontemporal_store(v[u], b + i + u * stride);
+  }
+}
+
+__global__ __launch_bounds__(BS) void k_stream_read_nt(long n16, const nt4 *__restrict__ a, float *__restrict__ sink) {
+  constexpr int U = 8;
+  const long stride = (long)gridDim.x * BS;
+  float acc = 0.f;
+  for (long i = blockIdx.x * (long)BS + threadIdx.x; i < n16; i += stride * U) {
+    nt4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      v[u] = nt4{0.f, 0.f, 0.f, 0.f};
+      if (i + u * stride < n16) v[u] = __builtin_nontemporal_load(a + i + u * stride);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc += v[u].x + v[u].y + v[u].z + v[u].w;
+  }
+  if (acc == 1234.5f) sink[0] = acc;  // keeps the loads; never true for the zero-filled input
+}
+
 int fvk_bench_stream(int device, long nbytes, int reps, int kind, double *sec) {
-  if (!sec || reps < 1 || nbytes < 4096 || kind < 0 || kind > 2) { set_error("bad argument"); return FVK_ERR_ARG; }
+  if (!sec || reps < 1 || nbytes < 4096 || kind < 0 || kind > 4) { set_error("bad argument"); return FVK_ERR_ARG; }
   FVK_HIP(hipSetDevice(device));
   const long n16 = nbytes / 16;
   void *a = nullptr, *b = nullptr;
@@ -1099,11 +1167,15 @@ int fvk_bench_stream(int device, long nbytes, int reps, int kind, double *sec) {
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
-  const int G = 256 * 16;
+  int ncu = 256;
+  hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
+  const int G = 256 * 16, Gnt = ncu * 8;
   auto launch = [&]() {
     if (kind == 0) k_stream_copy<<<G, BS>>>(n16, (const float4 *)a, (float4 *)b);
     else if (kind == 1) k_stream_read<<<G, BS>>>(n16, (const float4 *)a, (float *)b);
-    else k_stream_fill<<<G, BS>>>(n16, (float4 *)b);
+    else if (kind == 2) k_stream_fill<<<G, BS>>>(n16, (float4 *)b);
+    else if (kind == 3) k_stream_copy_nt<<<Gnt, BS>>>(n16, (const nt4 *)a, (nt4 *)b);
+    else k_stream_read_nt<<<Gnt, BS>>>(n16, (const nt4 *)a, (float *)b);
   };
   for (int k = 0; k < 3; ++k) launch();
   hipEventRecord(e0, 0);
@@ -1302,7 +1374,7 @@ static int read_hist(Flow &F, fvk_outer_hist *h) {
     h->res0_n_p = s.res0_n_p;
   }
   if (s.diverged) { set_error("momentum residual exceeded large"); return FVK_ERR_DIVERGED; }
-  return FVK_OK;
+  return sweep_status(*F.c);
 }
 
 int fvk_flow_setup_eqn(fvk_flow *ff, int it, fvk_outer_hist *h) {

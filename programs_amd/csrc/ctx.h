@@ -92,6 +92,13 @@ struct Ctx {
   int pc_imat = -1;
   bool pc_valid = false;
   Levels lf, lb;                  // forward / backward level sets
+  // one-launch factorisation and sweeps (else one launch per level)
+  bool sweep_one_launch = true;
+  int sweep_blocks_per_cu = 1;    // resident blocks per CU of the one-launch kernels (1: measured fastest)
+  int n_cu = 256;
+  double *sweep_t = nullptr;      // forward sweep output (3 x n)
+  int *sweep_int = nullptr;       // tickets: forward, backward, factor; [3] = wait-bound error
+  unsigned char *sweep_lmask = nullptr, *sweep_umask = nullptr;  // lower / upper slots of each row
   // Krylov workspace (3 components)
   double *wk = nullptr;           // 8 vectors x 3 components x n
   KS *ks = nullptr;               // 3 components: state after the last kernel / solve
@@ -215,6 +222,9 @@ int dalloc(Ctx &c, T **p, size_t n) {
 // krylov.hip
 int precond_build(Ctx &c, int imat);
 int precond_apply(Ctx &c, int imat, int nvc, const double *b, double *x, const KS *gate);
+// FVK_ERR_HIP if a one-launch factorisation or sweep exceeded its wait bound
+int sweep_status(Ctx &c);
+int sweep_masks_build(Ctx &c);
 int spmv(Ctx &c, int nvc, const double *x, double *y);
 // a[0..n) = v and b = a as kernels (capturable in a HIP graph)
 int fill(Ctx &c, double *a, size_t n, double v);

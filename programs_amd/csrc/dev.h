@@ -217,6 +217,18 @@ inline int red_grid(int n) {
 }
 inline int grid_for(int n) { return n > 0 ? (n + BS - 1) / BS : 1; }
 
+// MI355X dispatches the workgroups of a launch to its 8 XCDs round robin,
+// and each XCD has its own L2.  xcd_block() renumbers blocks so that XCD x
+// runs the contiguous range [x G/8, (x+1) G/8) of a grid of G blocks
+// (blocks past the last full round keep their number): rows that are
+// neighbours in the mesh then meet in one L2.  A bijection on block
+// numbers, so a kernel covers the same rows.
+constexpr int kXcd = 8;
+__device__ __forceinline__ int xcd_block() {
+  const int G = gridDim.x, b = blockIdx.x, per = G / kXcd;
+  return b < per * kXcd ? (b % kXcd) * per + b / kXcd : b;
+}
+
 // The FP64 Krylov kernels (krylov.hip) run 1024-thread blocks, at most 512
 // of them: measured 2.6 % faster than 512 x 512 on the plain FP64 solve, and
 // 4.5 % slower on the FP32 inner solve, which keeps RB.

@@ -26,7 +26,7 @@ template <int nvc, int nd>
 __global__ __launch_bounds__(BS) void k_grad(DMesh M, const double *__restrict__ phi, const double *__restrict__ phib,
                                              const double *__restrict__ g0, double *__restrict__ g, int it, int neu,
                                              const double *__restrict__ on_flag) {
-  const int i = blockIdx.x * BS + threadIdx.x, n = M.n;
+  const int i = xcd_block() * BS + threadIdx.x, n = M.n;
   if (i >= n) return;
   double acc[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
   if (on_flag && !(*on_flag > SMALL)) {
@@ -213,7 +213,7 @@ __global__ __launch_bounds__(BS) void k_mom_face(DMesh M, MomPar P, const double
                            double *__restrict__ cfl, double *__restrict__ cinout, const int *cell_oc,
                            const int *oc_face, const double *oc_u) {
   constexpr int nvc = nd;
-  const int f = blockIdx.x * BS + threadIdx.x;
+  const int f = xcd_block() * BS + threadIdx.x;
   if (f >= M.nf) return;
   const int n = M.n, nf = M.nf;
   const int ip = M.fip[f], in = M.fin[f];
@@ -395,7 +395,7 @@ __global__ __launch_bounds__(BS) void k_mom_cell(DMesh M, MomPar P, int nvc, dou
                            const double *__restrict__ cor, const double *__restrict__ cfl,
                            const double *__restrict__ dpdx, double *__restrict__ diag, double *__restrict__ spv,
                            double *__restrict__ rhs, double *__restrict__ cs, double *__restrict__ frc_vol) {
-  const int i = blockIdx.x * BS + threadIdx.x;
+  const int i = xcd_block() * BS + threadIdx.x;
   if (i >= M.n) return;
   const int n = M.n, nf = M.nf;
   double pl = 0.0, c_s = 0.0, r[3] = {0.0, 0.0, 0.0};
@@ -496,7 +496,7 @@ __global__ __launch_bounds__(BS) void k_pp_face(DMesh M, const double *__restric
                           const double *__restrict__ pl_r, const double *__restrict__ frc_dns,
                           const double *__restrict__ spd, double *__restrict__ ngp, double *__restrict__ ngn,
                           double *__restrict__ cpp, const FlowSc *sc) {
-  const int f = blockIdx.x * BS + threadIdx.x;
+  const int f = xcd_block() * BS + threadIdx.x;
   if (f >= M.nf) return;
   const int n = M.n, ip = M.fip[f], in = M.fin[f];
   if (in >= 0) {
@@ -536,7 +536,7 @@ __global__ __launch_bounds__(BS) void k_pp_face(DMesh M, const double *__restric
 __global__ __launch_bounds__(BS) void k_pp_cell(DMesh M, const double *__restrict__ ngp, const double *__restrict__ ngn,
                           const double *__restrict__ cpp, const double *__restrict__ cs, double *__restrict__ diag,
                           double *__restrict__ spv, double *__restrict__ rhs) {
-  const int i = blockIdx.x * BS + threadIdx.x;
+  const int i = xcd_block() * BS + threadIdx.x;
   if (i >= M.n) return;
   const int n = M.n;
   double pl = 0.0, r = -cs[i];
@@ -600,7 +600,7 @@ template <int nd>
 __global__ __launch_bounds__(BS) void k_flux_face(DMesh M, const double *__restrict__ pc, const double *__restrict__ ddxp,
                             const double *__restrict__ ngp, const double *__restrict__ frc_dns,
                             double *__restrict__ spd, double *__restrict__ fc1) {
-  const int f = blockIdx.x * BS + threadIdx.x;
+  const int f = xcd_block() * BS + threadIdx.x;
   if (f >= M.nf) return;
   const int in = M.fin[f];
   if (in < 0) return;
@@ -621,7 +621,7 @@ __global__ __launch_bounds__(BS) void k_flux_face(DMesh M, const double *__restr
 // cell part: rhs_p = rhs + (accumulated correction), SIMPLER only
 __global__ __launch_bounds__(BS) void k_flux_cell(DMesh M, const double *__restrict__ fc1, const double *__restrict__ rhs,
                             double *__restrict__ rhs_p) {
-  const int i = blockIdx.x * BS + threadIdx.x;
+  const int i = xcd_block() * BS + threadIdx.x;
   if (i >= M.n) return;
   const int n = M.n;
   double r = 0.0;
@@ -1077,7 +1077,7 @@ int flow_run_graph(Flow &F, int it0, int n, fvk_outer_hist *h, int &replayed) {
     }
     if (s.diverged) { set_error("momentum residual exceeded large"); return FVK_ERR_DIVERGED; }
   }
-  return 0;
+  return sweep_status(c);
 }
 
 }  // namespace fvk

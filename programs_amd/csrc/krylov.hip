@@ -116,6 +116,261 @@ __global__ __launch_bounds__(BS) void k_sweep_bwd(DMesh M, const int *__restrict
   for (int q = 0; q < nvc; ++q) x[q * n + r] = acc[q] * cc;
 }
 
+// ---------------------------------------------------------------- one-launch sweeps
+// The factorisation and the triangular sweeps in one launch each.  Rows are
+// taken in level order (every row after the rows it depends on) in chunks
+// of BS positions; a resident grid of a few blocks per CU takes the chunks
+// from a ticket counter, so the rows in flight stay near the front of the
+// order and every row a chunk waits for belongs to a chunk taken earlier,
+// by a block that is running or has finished.  An output starts as
+// kSweepMark (a signalling NaN: no arithmetic result has this pattern) and
+// a row waits until each value it needs has been replaced, taking the values
+// in slot order, so each row performs the same operations in the same order
+// as the level-by-level kernels above.  Waiting lanes never block their
+// wave: the wave loops until all its lanes are finished, and lanes whose
+// values have arrived finish in the same pass.  Every wait is bounded
+// (kSweepSpin passes per chunk); past the bound the kernel records an error
+// and its blocks stop taking chunks.  lmask / umask: bit k of a row is set
+// when slot k holds a lower / upper neighbour (rank order).
+constexpr unsigned long long kSweepMark = 0x7FF4DEADBEEF0001ull;
+constexpr unsigned kSweepSpin = 1u << 20;
+
+__device__ __forceinline__ unsigned long long ld_agent(const double *p) {
+  return __hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(double *p, double v) {
+  __hip_atomic_store((unsigned long long *)p, (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// out[0 .. cnt) = kSweepMark and *ticket = 0, unless the gate is closed
+__global__ __launch_bounds__(BS) void k_sweep_mark(size_t cnt, double *__restrict__ out, int *__restrict__ ticket,
+                                                   const KS *ks, int nvc) {
+  if (all_done(ks, nvc)) return;
+  unsigned long long *o = (unsigned long long *)out;
+  for (size_t i = blockIdx.x * (size_t)BS + threadIdx.x; i < cnt; i += (size_t)gridDim.x * BS) o[i] = kSweepMark;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *ticket = 0;
+}
+
+__global__ __launch_bounds__(BS) void k_sweep_masks(DMesh M, unsigned char *__restrict__ lmask,
+                                                    unsigned char *__restrict__ umask) {
+  const int r = blockIdx.x * BS + threadIdx.x;
+  if (r >= M.n) return;
+  const int rr = M.rank[r];
+  unsigned lo = 0, up = 0;
+  for (int k = 0; k < M.Ws; ++k) {
+    const int j = M.spc[(size_t)k * M.ld + r];
+    if (j < 0) break;
+    if (M.rank[j] < rr) lo |= 1u << k;
+    else if (M.rank[j] > rr) up |= 1u << k;
+  }
+  lmask[r] = (unsigned char)lo;
+  umask[r] = (unsigned char)up;
+}
+
+// The chunk loop shared by the three kernels: row(pos, fin, k) advances one
+// row as far as its values allow and sets fin when the row is written.
+template <class Row>
+__device__ __forceinline__ void sweep_chunks(int n, int *ticket, int *err, Row &row) {
+  __shared__ int s_chunk, s_abort;
+  if (threadIdx.x == 0) s_abort = 0;
+  for (;;) {
+    if (threadIdx.x == 0) s_chunk = atomicAdd(ticket, 1);
+    __syncthreads();
+    const int chunk = s_chunk;
+    const int abort = s_abort;
+    __syncthreads();
+    if (abort || (long)chunk * BS >= (long)n) return;
+    const int pos = chunk * BS + threadIdx.x;
+    bool fin = pos >= n;
+    if (!fin) row.start(pos);
+    unsigned spin = 0, idle = 0;
+    while (!__all(fin)) {
+      bool moved = false;
+      if (!fin) {
+        fin = row.advance();
+        moved = fin;
+      }
+      if (++spin > kSweepSpin) {
+        if (!fin) { *err = 1; s_abort = 1; }
+        break;
+      }
+      idle = __any(moved) ? 0 : idle + 1;
+      if (!__all(fin)) {
+        if (idle < 8) __builtin_amdgcn_s_sleep(1);
+        else __builtin_amdgcn_s_sleep(8);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// A row keeps its slots' columns and coefficients in registers (read once,
+// all slots at a time) and each pass issues the loads of all values it still
+// waits for at once, then takes them in slot order up to the first missing
+// one: one memory round trip per pass.  At most kSweepSlots slots.
+constexpr int kSweepSlots = 8;
+
+// DIC (imat 1) / DILU (imat 2) factor (solv_precond.f90:10-29)
+struct PrecondRow {
+  const int *spc, *spf, *order;
+  size_t ld;
+  int Ws;
+  const unsigned char *lmask;
+  int imat;
+  const double *diag, *ngp, *ngn;
+  double *c, *cr;
+  int r, k;
+  unsigned m;
+  double d;
+  int j[kSweepSlots];
+  double a1[kSweepSlots], a2[kSweepSlots];
+  __device__ void start(int pos) {
+    r = order[pos];
+    m = lmask[r];
+    d = diag[r];
+    k = 0;
+#pragma unroll
+    for (int s = 0; s < kSweepSlots; ++s) {
+      j[s] = 0;
+      a1[s] = a2[s] = 0.0;
+      if (s < Ws && ((m >> s) & 1u)) {
+        j[s] = spc[(size_t)s * ld + r];
+        const int f = spf[(size_t)s * ld + r];
+        a1[s] = imat == 1 ? ngn[f] : ngp[f];
+        a2[s] = ngn[f];
+      }
+    }
+  }
+  __device__ bool advance() {
+    unsigned long long v[kSweepSlots];
+#pragma unroll
+    for (int s = 0; s < kSweepSlots; ++s)
+      if (s >= k && s < Ws && ((m >> s) & 1u)) v[s] = ld_agent(c + j[s]);
+#pragma unroll
+    for (int s = 0; s < kSweepSlots; ++s) {
+      if (s < k || s >= Ws || !((m >> s) & 1u)) continue;
+      if (v[s] == kSweepMark) break;
+      const double cj = __longlong_as_double((long long)v[s]);
+      if (imat == 1) d = d - a1[s] * a2[s] * cj;
+      else d = d - a1[s] * cj * a2[s];
+      k = s + 1;
+    }
+    // k: the slot after the last one taken; done when no lower slot remains
+    if ((m >> k) != 0u) return false;
+    cr[r] = d;
+    st_agent(c + r, 1.0 / (d + SMALL));
+    return true;
+  }
+};
+
+__global__ __launch_bounds__(BS) void k_precond_sf(DMesh M, const int *__restrict__ order,
+                                                   const unsigned char *__restrict__ lmask, int imat,
+                                                   const double *__restrict__ diag, const double *__restrict__ ngp,
+                                                   const double *__restrict__ ngn, double *c,
+                                                   double *__restrict__ cr, int *ticket, int *err) {
+  PrecondRow row;
+  row.spc = M.spc; row.spf = M.spf; row.order = order; row.ld = M.ld; row.Ws = M.Ws;
+  row.lmask = lmask; row.imat = imat; row.diag = diag; row.ngp = ngp; row.ngn = ngn; row.c = c; row.cr = cr;
+  row.r = 0; row.k = 0; row.m = 0; row.d = 0.0;
+  sweep_chunks(M.n, ticket, err, row);
+}
+
+// Forward sweep t = (b - L t) * c (solv_direct.f90:12-24) and backward
+// sweep x = (t*c_r - U x) * c (:26-38) for NVC right-hand sides: out is
+// written, src read (b or t), the waits are on out.
+template <bool FWD, int NVC>
+struct SweepRow {
+  const int *spc, *order;
+  size_t ld;
+  int Ws, n;
+  const unsigned char *mask;
+  const double *spv, *c, *cr, *src;
+  double *out;
+  int r, k;
+  unsigned m;
+  double acc[NVC];
+  int j[kSweepSlots];
+  double a[kSweepSlots];
+  __device__ void start(int pos) {
+    r = order[pos];
+    m = mask[r];
+    k = 0;
+#pragma unroll
+    for (int s = 0; s < kSweepSlots; ++s) {
+      j[s] = 0;
+      a[s] = 0.0;
+      if (s < Ws && ((m >> s) & 1u)) {
+        j[s] = spc[(size_t)s * ld + r];
+        a[s] = spv[(size_t)s * ld + r];
+      }
+    }
+    if (FWD) {
+#pragma unroll
+      for (int q = 0; q < NVC; ++q) acc[q] = src[q * n + r];
+    } else {
+      const double crr = cr[r];
+#pragma unroll
+      for (int q = 0; q < NVC; ++q) acc[q] = src[q * n + r] * crr;
+    }
+  }
+  __device__ bool advance() {
+    unsigned long long v[kSweepSlots][NVC];
+#pragma unroll
+    for (int s = 0; s < kSweepSlots; ++s)
+      if (s >= k && s < Ws && ((m >> s) & 1u))
+#pragma unroll
+        for (int q = 0; q < NVC; ++q) v[s][q] = ld_agent(out + q * n + j[s]);
+#pragma unroll
+    for (int s = 0; s < kSweepSlots; ++s) {
+      if (s < k || s >= Ws || !((m >> s) & 1u)) continue;
+      bool ready = true;
+#pragma unroll
+      for (int q = 0; q < NVC; ++q) ready = ready && v[s][q] != kSweepMark;
+      if (!ready) break;
+#pragma unroll
+      for (int q = 0; q < NVC; ++q) acc[q] = acc[q] - a[s] * __longlong_as_double((long long)v[s][q]);
+      k = s + 1;
+    }
+    if ((m >> k) != 0u) return false;
+    const double cc = c[r];
+#pragma unroll
+    for (int q = 0; q < NVC; ++q) st_agent(out + q * n + r, acc[q] * cc);
+    return true;
+  }
+};
+
+template <bool FWD, int NVC>
+__device__ __forceinline__ void sweep_run(const DMesh &M, const int *order, const unsigned char *mask,
+                                          const double *spv, const double *c, const double *cr,
+                                          const double *src, double *out, int *ticket, int *err) {
+  SweepRow<FWD, NVC> row;
+  row.spc = M.spc; row.order = order; row.ld = M.ld; row.Ws = M.Ws; row.n = M.n; row.mask = mask;
+  row.spv = spv; row.c = c; row.cr = cr; row.src = src; row.out = out;
+  row.r = 0; row.k = 0; row.m = 0;
+  sweep_chunks(M.n, ticket, err, row);
+}
+
+template <int NVC>
+__global__ __launch_bounds__(BS) void k_sweep_fwd_sf(DMesh M, const int *__restrict__ order,
+                                                     const unsigned char *__restrict__ lmask,
+                                                     const double *__restrict__ spv, const double *__restrict__ c,
+                                                     const double *__restrict__ b, double *t, int *ticket, int *err,
+                                                     const KS *ks) {
+  if (all_done(ks, NVC)) return;
+  sweep_run<true, NVC>(M, order, lmask, spv, c, nullptr, b, t, ticket, err);
+}
+
+template <int NVC>
+__global__ __launch_bounds__(BS) void k_sweep_bwd_sf(DMesh M, const int *__restrict__ order,
+                                                     const unsigned char *__restrict__ umask,
+                                                     const double *__restrict__ spv, const double *__restrict__ c,
+                                                     const double *__restrict__ cr, const double *__restrict__ t,
+                                                     double *x, int *ticket, int *err, const KS *ks) {
+  if (all_done(ks, NVC)) return;
+  sweep_run<false, NVC>(M, order, umask, spv, c, cr, t, x, ticket, err);
+}
+
 template <int WS>
 __global__ __launch_bounds__(BS) void k_spmv(DMesh M, int nvc, const double *__restrict__ diag,
                                              const double *__restrict__ spv, const double *__restrict__ x,
@@ -566,10 +821,28 @@ int copy_d(Ctx &c, const double *a, double *b, size_t n) {
 
 // ------------------------------------------------------------ host side
 
+static int mark_grid(size_t cnt) { return (int)std::min<size_t>((cnt + BS - 1) / BS, 4096); }
+// resident grid of the one-launch kernels: sweep_blocks_per_cu blocks per CU,
+// no more than there are chunks
+static int sweep_grid(const Ctx &c) {
+  const int chunks = std::max(1, (c.n + BS - 1) / BS);
+  return std::max(1, std::min(chunks, c.n_cu * c.sweep_blocks_per_cu));
+}
+
+int sweep_masks_build(Ctx &c) {
+  k_sweep_masks<<<grid_for(c.n), BS, 0, c.stream>>>(c.M, c.sweep_lmask, c.sweep_umask);
+  FVK_HIP(hipGetLastError());
+  return 0;
+}
+
 int precond_build(Ctx &c, int imat) {
   const int n = c.n;
   if (imat == 0) {
     k_precond_jacobi<<<grid_for(n), BS, 0, c.stream>>>(n, c.diag, c.pc_c, c.pc_cr);
+  } else if (c.sweep_one_launch) {
+    k_sweep_mark<<<mark_grid(n), BS, 0, c.stream>>>(n, c.pc_c, c.sweep_int + 2, nullptr, 1);
+    k_precond_sf<<<sweep_grid(c), BS, 0, c.stream>>>(c.M, c.lf.rows, c.sweep_lmask, imat, c.diag, c.ngp, c.ngn,
+                                                       c.pc_c, c.pc_cr, c.sweep_int + 2, c.sweep_int + 3);
   } else {
     for (int l = 0; l < c.lf.count(); ++l) {
       int cnt = c.lf.ptr[l + 1] - c.lf.ptr[l];
@@ -587,6 +860,19 @@ int precond_apply(Ctx &c, int imat, int nvc, const double *b, double *x, const K
   const int n = c.n;
   if (imat == 0) {
     k_jacobi_apply<<<grid_for(n), BS, 0, c.stream>>>(n, nvc, c.pc_c, b, x, gate);
+  } else if (c.sweep_one_launch) {
+    const size_t cnt = (size_t)nvc * n;
+    k_sweep_mark<<<mark_grid(cnt), BS, 0, c.stream>>>(cnt, c.sweep_t, c.sweep_int, gate, nvc);
+    const int G = sweep_grid(c);
+#define FWD(V) k_sweep_fwd_sf<V><<<G, BS, 0, c.stream>>>(c.M, c.lf.rows, c.sweep_lmask, c.spv, c.pc_c, b, \
+                                                         c.sweep_t, c.sweep_int, c.sweep_int + 3, gate)
+#define BWD(V) k_sweep_bwd_sf<V><<<G, BS, 0, c.stream>>>(c.M, c.lb.rows, c.sweep_umask, c.spv, c.pc_c, c.pc_cr, \
+                                                         c.sweep_t, x, c.sweep_int + 1, c.sweep_int + 3, gate)
+    if (nvc == 1) FWD(1); else if (nvc == 2) FWD(2); else FWD(3);
+    k_sweep_mark<<<mark_grid(cnt), BS, 0, c.stream>>>(cnt, x, c.sweep_int + 1, gate, nvc);
+    if (nvc == 1) BWD(1); else if (nvc == 2) BWD(2); else BWD(3);
+#undef FWD
+#undef BWD
   } else {
     for (int l = 0; l < c.lf.count(); ++l) {
       int cnt = c.lf.ptr[l + 1] - c.lf.ptr[l];
@@ -664,6 +950,18 @@ int residual_to(Ctx &c, int nvc, const double *x, double *r, bool not_r0, double
 #undef L
   k_res_final_to<<<1, RB64, 0, c.stream>>>(nvc, part, G, out_dev);
   FVK_HIP(hipGetLastError());
+  return 0;
+}
+
+int sweep_status(Ctx &c) {
+  if (!c.sweep_int) return 0;
+  int e = 0;
+  FVK_HIP(hipMemcpyAsync(&e, c.sweep_int + 3, sizeof e, hipMemcpyDeviceToHost, c.stream));
+  FVK_HIP(hipStreamSynchronize(c.stream));
+  if (e) {
+    set_error("triangular sweep: a wait exceeded its bound; the preconditioner's results are invalid");
+    return FVK_ERR_HIP;
+  }
   return 0;
 }
 

@@ -109,15 +109,14 @@ std::string init_cells(Mesh &m, int n_dx, int n_vrt, const double *vx3, int n_vo
 // element face table: the hexahedron's 1 2 3 4 5 6 7 8 become 1 2 4 3 5 6 8
 // 7 (Gambit numbers a brick lexicographically); quadrilateral, triangle,
 // prism and tetrahedron agree.  Pyramids are not supported.
-std::string read_gmsh(Mesh &m, const char *path, int n_dx, double l_sc, const int *code_map, int n_map) {
+std::string parse_gmsh(GmshRaw &g, const char *path) {
   FILE *fp = std::fopen(path, "r");
   if (!fp) return std::string("cannot open ") + path;
   static const int kNvrt[8] = {0, 2, 3, 4, 4, 8, 6, 5};
-  static const int kDim[8] = {0, 1, 2, 2, 3, 3, 3, 3};
-  static const int kGambit[8] = {0, 0, 3, 2, 6, 4, 5, 0};  // gmsh type -> Gambit NTYPE
-  std::vector<int> node_id;
-  std::vector<double> node_x;
-  std::vector<int> e_idx, e_ty, e_tag0, e_vptr{0}, e_vrt;
+  std::vector<int> &node_id = g.node_id, &e_idx = g.e_idx, &e_ty = g.e_ty, &e_tag0 = g.e_tag0,
+                   &e_vptr = g.e_vptr, &e_vrt = g.e_vrt;
+  std::vector<double> &node_x = g.node_x;
+  e_vptr.assign(1, 0);
   char buf[256];
   std::string err;
   bool have_nodes = false, have_elems = false;
@@ -163,6 +162,20 @@ std::string read_gmsh(Mesh &m, const char *path, int n_dx, double l_sc, const in
   std::fclose(fp);
   if (!err.empty()) return err;
   if (!have_nodes || !have_elems) return "no $Nodes or $Elements section";
+  return "";
+}
+
+std::string read_gmsh(Mesh &m, const char *path, int n_dx, double l_sc, const int *code_map, int n_map) {
+  static const int kDim[8] = {0, 1, 2, 2, 3, 3, 3, 3};
+  static const int kGambit[8] = {0, 0, 3, 2, 6, 4, 5, 0};  // gmsh type -> Gambit NTYPE
+  GmshRaw g;
+  {
+    std::string err = parse_gmsh(g, path);
+    if (!err.empty()) return err;
+  }
+  const std::vector<int> &node_id = g.node_id, &e_idx = g.e_idx, &e_ty = g.e_ty, &e_tag0 = g.e_tag0,
+                         &e_vptr = g.e_vptr, &e_vrt = g.e_vrt;
+  const std::vector<double> &node_x = g.node_x;
   // node labels -> positions
   std::unordered_map<int, int> pos;
   pos.reserve(node_id.size() * 2);

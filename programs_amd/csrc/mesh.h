@@ -23,6 +23,15 @@ int elem_nfce(int ty);
 std::string init_cells(Mesh &m, int n_dx, int n_vrt, const double *vx3, int n_vol, const int *v_ty,
                        const int *v_vptr, const int *v_vrt1, const int *bnd);
 std::string read_neu(Mesh &m, const char *path, double l_sc);
+// The records of a gmsh 2.2 ASCII file, as flowlab's read_gmsh_mesh takes
+// them (flowlab/src/mesh.cpp:39-187): node ids and coordinates, and per
+// element its number, type, first tag (the physical tag) and node ids.
+struct GmshRaw {
+  std::vector<int> node_id;
+  std::vector<double> node_x;  // 3 per node
+  std::vector<int> e_idx, e_ty, e_tag0, e_vptr, e_vrt;
+};
+std::string parse_gmsh(GmshRaw &g, const char *path);
 // gmsh 2.2 ASCII mesh; n_dx <= 0: from the highest element dimension;
 // code_map: n_map pairs (physical tag, boundary code)
 std::string read_gmsh(Mesh &m, const char *path, int n_dx, double l_sc, const int *code_map, int n_map);

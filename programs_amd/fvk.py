@@ -162,6 +162,7 @@ def lib():
         "fvk_model_train": [_vp, _vp, C.c_int, C.c_double, _dp],
         "fvk_sys_get": [_vp, _dp, _dp, _dp],
         "fvk_bench_spmv": [_vp, C.c_int, _dp],
+        "fvk_bench_sweep": [_vp, C.c_int, C.c_int, C.c_int, _dp],
         "fvk_bench_copy": [C.c_int, C.c_long, C.c_int, _dp],
         "fvk_bench_stream": [C.c_int, C.c_long, C.c_int, C.c_int, _dp],
         "fvk_flow_create": [_vp, C.POINTER(FlowCfg), C.POINTER(_vp)],
@@ -595,6 +596,13 @@ def bench_spmv(ctx: Ctx, reps=100):
     return s.value
 
 
+def bench_sweep(ctx: Ctx, imat, nvc=1, reps=20):
+    """Seconds per application of the DIC / DILU factor (both sweeps)."""
+    s = C.c_double()
+    _check(lib().fvk_bench_sweep(ctx.h, imat, nvc, reps, C.byref(s)))
+    return s.value
+
+
 def bench_copy(nbytes=1 << 30, reps=20, device=0):
     """Seconds per device copy of nbytes (read nbytes + write nbytes)."""
     s = C.c_double()
@@ -604,9 +612,11 @@ def bench_copy(nbytes=1 << 30, reps=20, device=0):
 
 def bench_stream(kind, nbytes=1 << 30, reps=20, device=0):
     """Seconds per pass of a streaming probe: kind "copy" (nbytes read and
-    written), "read" or "write"; four 16-byte loads in flight per thread."""
+    written), "read" or "write" with four 16-byte loads in flight per thread,
+    "copy_nt" or "read_nt" with eight non-temporal ones and a resident grid."""
     s = C.c_double()
-    _check(lib().fvk_bench_stream(device, int(nbytes), reps, {"copy": 0, "read": 1, "write": 2}[kind], C.byref(s)))
+    kinds = {"copy": 0, "read": 1, "write": 2, "copy_nt": 3, "read_nt": 4}
+    _check(lib().fvk_bench_stream(device, int(nbytes), reps, kinds[kind], C.byref(s)))
     return s.value
 
 
