@@ -440,6 +440,86 @@ int fvk_write_gmsh_vector_field(const char *fname, int n, int n_dim, const int *
    writers print), into buf (len bytes, NUL-terminated). */
 int fvk_fmt_list_real(double x, char *buf, int len);
 
+/* ------------------------------------------------------------------------
+ * flowlab's serial CSR SIMPLE (flowlab/src/navierstokes.f90:515-641,
+ * geometry_m.f90, transport_m.f90, solv_m.f90).  Arrays as flowlab's
+ * Fortran receives them from its C++ mesh pipeline (navierstokes.f90:1-300),
+ * one block: boundary elements 1..n_bnd, domain elements n_bnd+1 ..
+ * n_bnd+n_dmn; all indices 1-based.  Fields are column-major (n_cmp, m) as
+ * Fortran declares them.
+ * ---------------------------------------------------------------------- */
+typedef struct fvk_fl_mesh fvk_fl_mesh;
+typedef struct fvk_fl_ctx fvk_fl_ctx;
+typedef struct fvk_fl_flow fvk_fl_flow;
+
+/* gmsh 2.2 file -> flowlab-layout arrays (the job of flowlab's read_gmsh_mesh,
+   connectivity and mesh_to_dual, flowlab/src/main.cpp:146-480): elements of
+   dimension n_dim are the domain, of n_dim - 1 the boundary (physical tag =
+   boundary type, e.g. 11 inlet, 21 outlet, 31 wall); boundary faces first,
+   interior faces in reverse order of discovery, each interior face with the
+   vertex order of the element that met it first (its owner); the cell
+   matrix pattern with its diagonal, columns ascending. */
+int fvk_fl_mesh_read_gmsh(const char *path, int n_dim, double l_sc, fvk_fl_mesh **out);
+/* info: n_dim, n_vrt, n_bnd, n_dmn, n_fce, nnz, element vertex entries,
+   face vertex entries */
+int fvk_fl_mesh_get_info(const fvk_fl_mesh *m, int *info);
+/* x3 (3 x n_vrt), elm_vptr (n_dmn+1), elm_vrt, bnd_tag (n_bnd), fce_elm
+   (2 x n_fce), fce_vptr (n_fce+1), fce_vrt, ng_ptr (n_dmn+1), ng_col (nnz),
+   elm_idx (gmsh element number of each domain element); NULL skips */
+int fvk_fl_mesh_get_arrays(const fvk_fl_mesh *m, double *x3, int *elm_vptr, int *elm_vrt, int *bnd_tag,
+                           int *fce_elm, int *fce_vptr, int *fce_vrt, int *ng_ptr, int *ng_col, int *elm_idx);
+void fvk_fl_mesh_free(fvk_fl_mesh *m);
+
+/* Device context from flowlab's arrays: topology in HBM, then geometry
+   (geometry, geometry_m.f90:97-330) and the inverted least-squares matrices
+   (ls_grad_matrix, :592-660) on the GPU, bit for bit the reference's. */
+int fvk_fl_ctx_create(int n_dim, int n_vrt, const double *x3, int n_bnd, const int *bnd_tag, int n_dmn,
+                      const int *elm_vptr, const int *elm_vrt, int n_fce, const int *fce_elm, const int *fce_vptr,
+                      const int *fce_vrt, const int *ng_ptr, const int *ng_col, int device, fvk_fl_ctx **out);
+void fvk_fl_ctx_destroy(fvk_fl_ctx *ctx);
+/* xdmn%x_vc (n_dim x n_dmn), vol, x_fc, norm, area, x_pc, x_nc, l_pn, w_fp
+   (per face), ls (n_dim x n_dim x n_dmn); NULL skips */
+int fvk_fl_geometry(fvk_fl_ctx *ctx, double *x_vc, double *vol, double *x_fc, double *norm, double *area,
+                    double *x_pc, double *x_nc, double *l_pn, double *w_fp, double *ls);
+/* ls_gradient (geometry_m.f90:665-770): phi (n_cmp x n_dmn), phi_bnd
+   (n_cmp x n_bnd; extrapolated in place when neu_bnd), grad (n_dim x n_cmp
+   x n_dmn) */
+int fvk_fl_ls_gradient(fvk_fl_ctx *ctx, int n_cmp, const double *phi, double *phi_bnd, double *grad, int neu_bnd);
+/* lin_eqns_solver (solv_m.f90:10-30) on the context's pattern (its ia, ja):
+   isol 1 cg_solver, 2 bicg_solver, with solv_m's preconditioner and
+   direct_solve; rhs returns the initial residual as the reference leaves
+   it, norm the final L1 residual */
+int fvk_fl_lin_eqns_solver(fvk_fl_ctx *ctx, const double *a_ij, double *phi, double *rhs, double *norm, int n_iter,
+                           double reduc, int isol);
+
+/* SIMPLE state (velocity, pressure, pressure correction) on a context.
+   params: density, viscosity, velocity relaxation, pressure relaxation,
+   convection blending, momentum iterations and reduction, pressure
+   iterations and reduction (navierstokes.f90:161-175, 566, 598: 25, 0.01). */
+int fvk_fl_flow_create(fvk_fl_ctx *ctx, const double *params, fvk_fl_flow **out);
+void fvk_fl_flow_destroy(fvk_fl_flow *f);
+/* vel (n_dim x n_dmn), vel_bnd (n_dim x n_bnd; the values
+   set_boundary_conditions sets), pres, pres_bnd; NULL keeps */
+int fvk_fl_flow_set(fvk_fl_flow *f, const double *vel, const double *vel_bnd, const double *pres,
+                    const double *pres_bnd);
+int fvk_fl_flow_get(fvk_fl_flow *f, double *vel, double *vel_bnd, double *pres, double *pres_bnd, double *pp,
+                    double *spd);
+/* the current matrix values (nnz), right-hand side (n_dim x n_dmn), 1/a_P,
+   velocity (n_dim x n_dim x n_dmn), pressure and pressure-correction
+   gradients, pressure-correction boundary values */
+int fvk_fl_flow_get_system(fvk_fl_flow *f, double *val, double *rhs, double *diag, double *vel_grad,
+                           double *pres_grad, double *pp_grad, double *pp_bnd);
+/* One step of the outer loop by the routine it replaces: 0 zero rhs and
+   matrix, 1 ls_gradient(pres, 1), 2 ls_gradient(vel, 0), 3 ns_source
+   (transport_m.f90:726-752), 4 construct_pde (:58-348), 5 momentum
+   lin_eqns_solver calls (res0[0..n_dim)), 6 construct_pp (:430-629),
+   7 pressure-correction lin_eqns_solver (res0[0]), 8 ls_gradient(pp, 1),
+   9 ns_correction (:757-800) */
+int fvk_fl_flow_step(fvk_fl_flow *f, int step, double *res0);
+/* Outer iterations with fp_iter_ctrl's exit rule (transport_m.f90:806-845):
+   res0 (4 per iteration), norm_gl (per iteration), *n_done */
+int fvk_fl_flow_run(fvk_fl_flow *f, int n_iter, double res_drop, double *res0, double *norm_gl, int *n_done);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,7 +18,9 @@
 #   _ref/outp/fvk_ref_outp        the reference's Tecplot writers
 #       (cfd_outp.f90) behind oracle/ref/fvk_ref_outp.f90;
 #   _ref/gmsh/gmsh_ref            flowlab's gmsh reader and field writers
-#       (flowlab/src/mesh.cpp) behind oracle/ref/gmsh_ref.cpp.
+#       (flowlab/src/mesh.cpp) behind oracle/ref/gmsh_ref.cpp;
+#   _ref/flowlab/flowlab_ref_driver  flowlab's serial CSR SIMPLE (its
+#       Fortran modules) behind oracle/ref/flowlab_ref_driver.f90.
 set -euo pipefail
 REF=${FVK_REFERENCE_DIR:-/root/reference/spray-simulation}
 HERE=$(cd "$(dirname "$0")" && pwd)
@@ -80,5 +82,16 @@ $FC $FFLAGS -o "$OUT/outp/fvk_ref_outp" "$w/driver.o" $objs
 FLOWLAB=${FVK_FLOWLAB_DIR:-$REF/../flowlab/src}
 mkdir -p "$OUT/gmsh"
 g++ -O2 -I"$FLOWLAB" -o "$OUT/gmsh/gmsh_ref" "$HERE/ref/gmsh_ref.cpp" "$FLOWLAB/mesh.cpp"
+# flowlab's serial CSR SIMPLE: its Fortran modules compiled where they lie
+# (-cpp, no USE_MPI: the serial build) behind a driver of this project
+# (oracle/ref/flowlab_ref_driver.f90)
+w=$SCRATCH/flowlab objs=""
+mkdir -p "$w" "$OUT/flowlab"
+for m in prec_m constants_m vector_utils_m matrix_utils_m graph_m hash_m solv_m geometry_m transport_m; do
+  (cd "$w" && $FC $FFLAGS -cpp -c "$FLOWLAB/$m.f90" -o "$m.o")
+  objs="$objs $w/$m.o"
+done
+(cd "$w" && $FC $FFLAGS -c "$HERE/ref/flowlab_ref_driver.f90" -o driver.o)
+$FC $FFLAGS -o "$OUT/flowlab/flowlab_ref_driver" "$w/driver.o" $objs
 echo "built $OUT/shipped/fvk_ref_driver $OUT/relaxed/fvk_ref_driver $OUT/cg/fvk_ref_cg $OUT/outp/fvk_ref_outp" \
-     "$OUT/gmsh/gmsh_ref"
+     "$OUT/gmsh/gmsh_ref $OUT/flowlab/flowlab_ref_driver"

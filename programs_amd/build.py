@@ -16,8 +16,8 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libfvk.so")
 BUILD = os.path.join(HERE, "_build")
 
-HIP_SOURCES = ["krylov.hip", "mixed.hip", "learn.hip", "flow.hip", "meshdev.hip", "capi.hip"]
-CPP_SOURCES = ["mesh.cpp", "output.cpp"]
+HIP_SOURCES = ["krylov.hip", "mixed.hip", "learn.hip", "flow.hip", "meshdev.hip", "flowlab.hip", "capi.hip"]
+CPP_SOURCES = ["mesh.cpp", "output.cpp", "flmesh.cpp"]
 ARCH = os.environ.get("FVK_OFFLOAD_ARCH", "gfx950")
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-function",

@@ -182,6 +182,20 @@ def lib():
         "fvk_write_gmsh_field": [C.c_char_p, C.c_int, _ip, _dp],
         "fvk_write_gmsh_vector_field": [C.c_char_p, C.c_int, C.c_int, _ip, _dp],
         "fvk_fmt_list_real": [C.c_double, C.c_char_p, C.c_int],
+        "fvk_fl_mesh_read_gmsh": [C.c_char_p, C.c_int, C.c_double, C.POINTER(_vp)],
+        "fvk_fl_mesh_get_info": [_vp, _ip],
+        "fvk_fl_mesh_get_arrays": [_vp, _dp] + [_ip] * 9,
+        "fvk_fl_ctx_create": [C.c_int, C.c_int, _dp, C.c_int, _ip, C.c_int, _ip, _ip, C.c_int, _ip, _ip, _ip,
+                              _ip, _ip, C.c_int, C.POINTER(_vp)],
+        "fvk_fl_geometry": [_vp] + [_dp] * 10,
+        "fvk_fl_ls_gradient": [_vp, C.c_int, _dp, _dp, _dp, C.c_int],
+        "fvk_fl_lin_eqns_solver": [_vp, _dp, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int],
+        "fvk_fl_flow_create": [_vp, _dp, C.POINTER(_vp)],
+        "fvk_fl_flow_set": [_vp, _dp, _dp, _dp, _dp],
+        "fvk_fl_flow_get": [_vp] + [_dp] * 6,
+        "fvk_fl_flow_get_system": [_vp] + [_dp] * 7,
+        "fvk_fl_flow_step": [_vp, C.c_int, _dp],
+        "fvk_fl_flow_run": [_vp, C.c_int, C.c_double, _dp, _dp, _ip],
     }
     for name, args in sig.items():
         # a library older than this binding lacks some entry points: those
@@ -191,9 +205,13 @@ def lib():
             continue
         f.argtypes = args
         f.restype = C.c_int
-    for name in ("fvk_mesh_free", "fvk_ctx_destroy", "fvk_flow_destroy", "fvk_model_destroy"):
-        getattr(L, name).argtypes = [_vp]
-        getattr(L, name).restype = None
+    for name in ("fvk_mesh_free", "fvk_ctx_destroy", "fvk_flow_destroy", "fvk_model_destroy", "fvk_fl_mesh_free",
+                 "fvk_fl_ctx_destroy", "fvk_fl_flow_destroy"):
+        f = getattr(L, name, None)
+        if f is None:
+            continue
+        f.argtypes = [_vp]
+        f.restype = None
     _lib = L
     atexit.register(_release_all)
     return L
