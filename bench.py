@@ -482,6 +482,60 @@ def large_leg(fvk, args, model):
     return out
 
 
+def flowlab_leg(fvk, args):
+    """flowlab's serial CSR SIMPLE (flowlab/src/navierstokes.f90:515-641) on
+    an n^3 lid-driven cavity (+-10% jitter, Re 100, urf 0.7 / 0.3, blending
+    0.5): outer iterations per second with the factor and sweeps in one
+    launch each and with one launch per level, beside one outer iteration of
+    flowlab's own Fortran (flang -O2, one core) on the same mesh."""
+    from programs_amd import flowlab as fl
+    from programs_amd import meshgen
+    n = args.flowlab
+    t0 = time.perf_counter()
+    m = fl.FlMesh.from_records(3, *meshgen.box_records(n, jitter=0.1))
+    t_mesh = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ctx = fl.FlCtx(m)
+    t_ctx = time.perf_counter() - t0
+    ub = fl.boundary_values(m, ctx.geometry()["norm"], "Wall", 1, (1, 0, 0, 1))
+    par = (1.0, 0.01, 0.7, 0.3, 0.5)
+    out = dict(workload="flowlab CSR SIMPLE, %d^3 lid-driven cavity (+-10%% jitter), Re 100, momentum BiCGStab and "
+                        "pressure CG (25 iterations, reduction 0.01) with flowlab's factor" % n,
+               cells=m.n_dmn, faces=m.n_fce, nnz=m.nnz, mesh_build_s=t_mesh, context_s=t_ctx)
+    for one in (1, 0):
+        ctx.set_param("sweep_one_launch", one)
+        fw = fl.FlFlow(ctx, *par)
+        fw.set(vel_bnd=ub)
+        fw.run(2, 0.0)
+        K = args.flowlab_its if one else max(1, args.flowlab_its // 2)
+        t0 = time.perf_counter()
+        res0, gl = fw.run(K, 0.0)
+        dt = time.perf_counter() - t0
+        key = "one_launch" if one else "per_level_launches"
+        out[key] = dict(outer_iterations=K, ms_per_outer=dt * 1e3 / K, outer_its_per_s=K / dt,
+                        norm_gl_last=float(gl[-1]))
+        log("flowlab %d^3 %s: %.1f ms per outer iteration" % (n, key, dt * 1e3 / K))
+        fw.destroy()
+    ctx.set_param("sweep_one_launch", 1)
+    if not args.no_cpu:
+        from oracle import flowlab_ref
+        if flowlab_ref.available():
+            import tempfile
+            with tempfile.TemporaryDirectory() as tmp:
+                K = args.flowlab_its
+                dt, setup = flowlab_ref.time_outer_iterations(m, list(par) + [0.0], ub, K, tmp)
+                dt /= K
+            out["cpu_baseline"] = dict(value=dt, unit="s per outer iteration", cores=1, kind="reference",
+                                       sample="outer iterations 3..%d of flowlab's own Fortran modules (flang -O2, "
+                                              "one core) driven by oracle/ref/flowlab_ref_driver.f90 on the same "
+                                              "mesh, the iterations the GPU legs time: a %d-iteration run minus a "
+                                              "2-iteration run (%.1f s)" % (K + 2, K + 2, setup))
+            out["speedup_vs_cpu_reference"] = dt / (out["one_launch"]["ms_per_outer"] * 1e-3)
+            log("flowlab %d^3 reference: %.1f s per outer iteration" % (n, dt))
+    ctx.destroy()
+    return out
+
+
 def setup_leg(fvk, n, jitter):
     """Mesh connectivity and geometry of the n^3 box: on the host against on
     the GPU (same arrays bit for bit), and the device context build."""
@@ -542,6 +596,9 @@ def main():
     ap.add_argument("--channel-pretrain-lognormal", type=int, default=3, help="lognormal systems per outer")
     ap.add_argument("--channel-only", action="store_true", help="run the channel leg alone")
     ap.add_argument("--simple-only", action="store_true", help="run the SIMPLE leg alone (no network)")
+    ap.add_argument("--flowlab", type=int, default=96, help="cells per side of the flowlab CSR SIMPLE leg (0: off)")
+    ap.add_argument("--flowlab-its", type=int, default=6)
+    ap.add_argument("--flowlab-only", action="store_true", help="run the flowlab leg alone")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is "
@@ -553,6 +610,9 @@ def main():
         if args.channel <= 0:
             sys.exit("--channel-only needs --channel N (triangles)")
         print(json.dumps({"channel": channel_leg(fvk, args)}), flush=True)
+        return
+    if args.flowlab_only:
+        print(json.dumps({"flowlab": flowlab_leg(fvk, args)}), flush=True)
         return
     if args.simple_only:
         # the SIMPLE leg alone (for kernel profiles), plain pressure solves
@@ -762,6 +822,8 @@ def main():
         out["simple"], states3 = simple_leg(fvk, args, model)
     if args.channel > 0:
         out["channel"] = channel_leg(fvk, args)
+    if args.flowlab > 0:
+        out["flowlab"] = flowlab_leg(fvk, args)
     if not args.no_cpu:
         port = cpu_baseline(ctx, fvk, n, args.jitter, r["iters"], args.cpu_sample)
         ref = cpu_baseline_reference(ctx, mesh, fvk, n, r["iters"], args.cpu_ref_sample)

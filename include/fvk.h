@@ -460,6 +460,11 @@ typedef struct fvk_fl_flow fvk_fl_flow;
    vertex order of the element that met it first (its owner); the cell
    matrix pattern with its diagonal, columns ascending. */
 int fvk_fl_mesh_read_gmsh(const char *path, int n_dim, double l_sc, fvk_fl_mesh **out);
+/* The same from element records in memory: node coordinates x3 (3 x
+   n_node), per element its gmsh type, physical tag and nodes (e_vptr 0-based
+   offsets, n_elm+1; e_vrt node numbers 1..n_node, gmsh vertex order). */
+int fvk_fl_mesh_from_records(int n_dim, int n_node, const double *x3, int n_elm, const int *e_ty, const int *e_tag,
+                             const int *e_vptr, const int *e_vrt, fvk_fl_mesh **out);
 /* info: n_dim, n_vrt, n_bnd, n_dmn, n_fce, nnz, element vertex entries,
    face vertex entries */
 int fvk_fl_mesh_get_info(const fvk_fl_mesh *m, int *info);
@@ -477,6 +482,11 @@ int fvk_fl_ctx_create(int n_dim, int n_vrt, const double *x3, int n_bnd, const i
                       const int *elm_vptr, const int *elm_vrt, int n_fce, const int *fce_elm, const int *fce_vptr,
                       const int *fce_vrt, const int *ng_ptr, const int *ng_col, int device, fvk_fl_ctx **out);
 void fvk_fl_ctx_destroy(fvk_fl_ctx *ctx);
+/* "sweep_one_launch" 1 (default when every row has at most 8 lower and 8
+   upper entries): the factor and direct_solve's sweeps in one launch each,
+   rows waiting for their values; 0: one launch per level.  Both give the
+   same bits.  No counterpart in the reference. */
+int fvk_fl_ctx_set_param(fvk_fl_ctx *ctx, const char *name, double value);
 /* xdmn%x_vc (n_dim x n_dmn), vol, x_fc, norm, area, x_pc, x_nc, l_pn, w_fp
    (per face), ls (n_dim x n_dim x n_dmn); NULL skips */
 int fvk_fl_geometry(fvk_fl_ctx *ctx, double *x_vc, double *vol, double *x_fc, double *norm, double *area,

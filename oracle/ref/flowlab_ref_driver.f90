@@ -23,9 +23,10 @@
 ! Elements are numbered as flowlab numbers them: boundary elements 1..n_bnd,
 ! domain elements n_bnd+1 .. n_bnd+n_dmn.
 !
-! Output (stream file): geometry, then per outer iteration the four
-! residual norms and norm_gl (fp_iter_ctrl), and for the first n_dump
-! iterations the intermediate fields; the final fields at the end.
+! Output (stream file): geometry (unless n_dump < 0), then per outer
+! iteration the four residual norms and norm_gl (fp_iter_ctrl), and for the
+! first n_dump iterations the intermediate fields; the final fields at the
+! end.
 program flowlab_ref_driver
   use constants_m
   use hash_m
@@ -181,8 +182,9 @@ program flowlab_ref_driver
   close(u)
 
   open(newunit=v, file=trim(fout), access='stream', form='unformatted', status='replace')
-  write(v) xdmn%x_vc%dmn, xdmn%vol%dmn, xdmn%x_fc, xdmn%norm, xdmn%area, xdmn%x_pc, xdmn%x_nc, &
-           xdmn%l_pn, xdmn%w_fp, xdmn%ls%dmn
+  ! n_dump < 0: timing runs, no geometry written
+  if (n_dump >= 0) write(v) xdmn%x_vc%dmn, xdmn%vol%dmn, xdmn%x_fc, xdmn%norm, xdmn%area, xdmn%x_pc, &
+                            xdmn%x_nc, xdmn%l_pn, xdmn%w_fp, xdmn%ls%dmn
 
   ! the outer loop (navierstokes.f90:515-641); pARMS calls are no-ops in a
   ! serial build and are left out; res0 starts defined (the reference

@@ -21,6 +21,7 @@
 
 #include "ctx.h"
 #include "flmesh.h"
+#include "sweep.h"
 
 namespace fvk {
 
@@ -48,6 +49,11 @@ struct FlCtx {
   // triangular sweeps by levels (lower / upper dependencies)
   std::vector<int> lf_ptr, lb_ptr;
   int *lf_rows = nullptr, *lb_rows = nullptr;
+  // one launch per factorisation / sweep (rows wait for their values) when
+  // no row has more than kSweepSlots lower or upper entries
+  bool one_launch = true;
+  int n_cu = 256;
+  int *sweep_int = nullptr;  // tickets: forward, backward, factor; [3] wait-bound error
   // Krylov work: 6 vectors x 3 components, partials, scalars
   double *wk = nullptr, *pc = nullptr, *part = nullptr;
   FlKS *ks = nullptr;
@@ -542,6 +548,158 @@ __global__ void k_fl_ks_open(FlKS *ks, int ncmp) {
 
 static int fl_pgrid(int n) { return red_grid(n); }
 
+// ---------------------------------------------------------------- one-launch sweeps
+__device__ __forceinline__ bool fl_all_done(const FlKS *ks, int ncmp) {
+  if (!ks) return false;
+  for (int c = 0; c < ncmp; ++c)
+    if (!ks[c].done) return false;
+  return true;
+}
+
+__global__ __launch_bounds__(BS) void k_fl_mark(size_t cnt, double *out, int *ticket, const FlKS *ks, int ncmp) {
+  if (fl_all_done(ks, ncmp)) return;
+  unsigned long long *o = (unsigned long long *)out;
+  for (size_t i = blockIdx.x * (size_t)BS + threadIdx.x; i < cnt; i += (size_t)gridDim.x * BS) o[i] = kSweepMark;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *ticket = 0;
+}
+
+// preconditioner (solv_m.f90:224-250) for one row: the lower entries in
+// column order
+struct FlPcRow {
+  const int *ia, *ja, *kii, *tpos, *order;
+  const double *a;
+  double *pc;
+  int r, k, nl;
+  double s;
+  int j[kSweepSlots];
+  double a1[kSweepSlots], a2[kSweepSlots];
+  __device__ void start(int pos) {
+    r = order[pos];
+    const int lo = ia[r];
+    nl = kii[r] - lo;
+    s = a[kii[r]];
+    k = 0;
+#pragma unroll
+    for (int t = 0; t < kSweepSlots; ++t) {
+      j[t] = 0;
+      a1[t] = a2[t] = 0.0;
+      if (t < nl) {
+        j[t] = ja[lo + t];
+        a1[t] = a[lo + t];
+        a2[t] = a[tpos[lo + t]];
+      }
+    }
+  }
+  __device__ bool advance() {
+    unsigned long long v[kSweepSlots];
+#pragma unroll
+    for (int t = 0; t < kSweepSlots; ++t)
+      if (t >= k && t < nl) v[t] = ld_agent(pc + j[t]);
+#pragma unroll
+    for (int t = 0; t < kSweepSlots; ++t) {
+      if (t < k || t >= nl) continue;
+      if (v[t] == kSweepMark) break;
+      s = s - a1[t] * __longlong_as_double((long long)v[t]) * a2[t];
+      k = t + 1;
+    }
+    if (k < nl) return false;
+    st_agent(pc + r, 1.0 / (s + SMALL));
+    return true;
+  }
+};
+
+// direct_solve for NC right-hand sides: forward (solv_m.f90:255-268) writes
+// out from src = b; backward (:270-286) scales src = the forward result and
+// writes out from the upper entries, last column first
+template <bool FWD, int NC>
+struct FlSweepRow {
+  const int *ia, *ja, *kii, *order;
+  const double *a, *pc, *src;
+  double *out;
+  int n, r, k, nl;
+  double s[NC];
+  int j[kSweepSlots];
+  double av[kSweepSlots];
+  __device__ void start(int pos) {
+    r = order[pos];
+    int first, step;
+    if (FWD) {
+      first = ia[r];
+      nl = kii[r] - first;
+      step = 1;
+    } else {
+      first = ia[r + 1] - 1;
+      nl = first - kii[r];
+      step = -1;
+    }
+    k = 0;
+#pragma unroll
+    for (int t = 0; t < kSweepSlots; ++t) {
+      j[t] = 0;
+      av[t] = 0.0;
+      if (t < nl) {
+        j[t] = ja[first + step * t];
+        av[t] = a[first + step * t];
+      }
+    }
+    if (FWD) {
+#pragma unroll
+      for (int q = 0; q < NC; ++q) s[q] = src[(size_t)q * n + r];
+    } else {
+      const double cof = 1.0 / (pc[r] + SMALL);
+#pragma unroll
+      for (int q = 0; q < NC; ++q) s[q] = cof * src[(size_t)q * n + r];
+    }
+  }
+  __device__ bool advance() {
+    unsigned long long v[kSweepSlots][NC];
+#pragma unroll
+    for (int t = 0; t < kSweepSlots; ++t)
+      if (t >= k && t < nl)
+#pragma unroll
+        for (int q = 0; q < NC; ++q) v[t][q] = ld_agent(out + (size_t)q * n + j[t]);
+#pragma unroll
+    for (int t = 0; t < kSweepSlots; ++t) {
+      if (t < k || t >= nl) continue;
+      bool ready = true;
+#pragma unroll
+      for (int q = 0; q < NC; ++q) ready = ready && v[t][q] != kSweepMark;
+      if (!ready) break;
+#pragma unroll
+      for (int q = 0; q < NC; ++q) s[q] = s[q] - av[t] * __longlong_as_double((long long)v[t][q]);
+      k = t + 1;
+    }
+    if (k < nl) return false;
+    const double cc = pc[r];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) st_agent(out + (size_t)q * n + r, s[q] * cc);
+    return true;
+  }
+};
+
+__global__ __launch_bounds__(BS) void k_fl_pc_sf(int n, const int *__restrict__ order, const int *__restrict__ ia,
+                                                 const int *__restrict__ ja, const int *__restrict__ kii,
+                                                 const int *__restrict__ tpos, const double *__restrict__ a,
+                                                 double *pc, int *ticket, int *err) {
+  FlPcRow row;
+  row.ia = ia; row.ja = ja; row.kii = kii; row.tpos = tpos; row.order = order; row.a = a; row.pc = pc;
+  row.r = 0; row.k = 0; row.nl = 0; row.s = 0.0;
+  sweep_chunks(n, ticket, err, row);
+}
+
+template <bool FWD, int NC>
+__global__ __launch_bounds__(BS) void k_fl_sweep_sf(int n, const int *__restrict__ order, const int *__restrict__ ia,
+                                                    const int *__restrict__ ja, const int *__restrict__ kii,
+                                                    const double *__restrict__ a, const double *__restrict__ pc,
+                                                    const double *__restrict__ src, double *out, int *ticket,
+                                                    int *err, const FlKS *ks) {
+  if (fl_all_done(ks, NC)) return;
+  FlSweepRow<FWD, NC> row;
+  row.ia = ia; row.ja = ja; row.kii = kii; row.order = order; row.a = a; row.pc = pc; row.src = src;
+  row.out = out; row.n = n; row.r = 0; row.k = 0; row.nl = 0;
+  sweep_chunks(n, ticket, err, row);
+}
+
 // ---------------------------------------------------------------- host
 static int fl_levels(const std::vector<int> &ia, const std::vector<int> &ja, int n, bool fwd,
                      std::vector<int> &ptr, std::vector<int> &rows) {
@@ -566,7 +724,17 @@ static int fl_levels(const std::vector<int> &ia, const std::vector<int> &ja, int
   return 0;
 }
 
+static int fl_sweep_grid(const FlCtx &c) { return std::max(1, std::min((c.n + BS - 1) / BS, c.n_cu)); }
+static int fl_mark_grid(size_t cnt) { return (int)std::max<size_t>(1, std::min<size_t>((cnt + BS - 1) / BS, 4096)); }
+
 static int fl_precond(FlCtx &c, const double *a) {
+  if (c.one_launch) {
+    k_fl_mark<<<fl_mark_grid(c.n), BS, 0, c.stream>>>(c.n, c.pc, c.sweep_int + 2, nullptr, 1);
+    k_fl_pc_sf<<<fl_sweep_grid(c), BS, 0, c.stream>>>(c.n, c.lf_rows, c.ia, c.ja, c.kii, c.tpos, a, c.pc,
+                                                       c.sweep_int + 2, c.sweep_int + 3);
+    FVK_HIP(hipGetLastError());
+    return 0;
+  }
   for (size_t l = 0; l + 1 < c.lf_ptr.size(); ++l) {
     const int cnt = c.lf_ptr[l + 1] - c.lf_ptr[l];
     k_fl_pc_level<<<grid_for(cnt), BS, 0, c.stream>>>(c.lf_rows + c.lf_ptr[l], cnt, c.ia, c.ja, c.kii, c.tpos, a,
@@ -578,6 +746,22 @@ static int fl_precond(FlCtx &c, const double *a) {
 
 // x = M^-1 b (direct_solve), t: scratch
 static int fl_direct(FlCtx &c, int ncmp, const double *a, const double *b, double *t, double *x, const FlKS *ks) {
+  if (c.one_launch) {
+    const size_t cnt = (size_t)ncmp * c.n;
+    const int G = fl_sweep_grid(c);
+    k_fl_mark<<<fl_mark_grid(cnt), BS, 0, c.stream>>>(cnt, t, c.sweep_int, ks, ncmp);
+#define FWD(C) k_fl_sweep_sf<true, C><<<G, BS, 0, c.stream>>>(c.n, c.lf_rows, c.ia, c.ja, c.kii, a, c.pc, b, t, \
+                                                              c.sweep_int, c.sweep_int + 3, ks)
+#define BWD(C) k_fl_sweep_sf<false, C><<<G, BS, 0, c.stream>>>(c.n, c.lb_rows, c.ia, c.ja, c.kii, a, c.pc, t, x, \
+                                                               c.sweep_int + 1, c.sweep_int + 3, ks)
+    if (ncmp == 1) FWD(1); else if (ncmp == 2) FWD(2); else FWD(3);
+    k_fl_mark<<<fl_mark_grid(cnt), BS, 0, c.stream>>>(cnt, x, c.sweep_int + 1, ks, ncmp);
+    if (ncmp == 1) BWD(1); else if (ncmp == 2) BWD(2); else BWD(3);
+#undef FWD
+#undef BWD
+    FVK_HIP(hipGetLastError());
+    return 0;
+  }
   for (size_t l = 0; l + 1 < c.lf_ptr.size(); ++l) {
     const int cnt = c.lf_ptr[l + 1] - c.lf_ptr[l];
     k_fl_fwd_level<<<grid_for(cnt), BS, 0, c.stream>>>(c.lf_rows + c.lf_ptr[l], cnt, c.n, ncmp, c.ia, c.ja, a, c.pc,
@@ -1088,6 +1272,13 @@ static int fl_ctx_build(FlCtx &c, int nd, int n_vrt, const double *x3, int nb, c
 #undef A
 #undef U
   if (rc) return rc;
+  {
+    int mx = 0;
+    for (int i = 0; i < n; ++i) mx = std::max({mx, kii[i] - ia[i], ia[i + 1] - 1 - kii[i]});
+    c.one_launch = mx <= kSweepSlots;
+  }
+  FVK_HIP(hipDeviceGetAttribute(&c.n_cu, hipDeviceAttributeMultiprocessorCount, c.device));
+  if ((rc = fl_alloc(c, &c.sweep_int, 4))) return rc;
   std::vector<int> lrows;
   fl_levels(ia, ja, n, true, c.lf_ptr, lrows);
   if ((rc = fl_up(c, &c.lf_rows, lrows))) return rc;
@@ -1204,6 +1395,19 @@ static int ff_construct_pde(FlFlow &F) {
   return 0;
 }
 
+// FVK_ERR_HIP when a one-launch factorisation or sweep exceeded its wait
+// bound (its results are then invalid)
+static int fl_sweep_status(FlCtx &c) {
+  int e = 0;
+  FVK_HIP(hipMemcpyAsync(&e, c.sweep_int + 3, sizeof e, hipMemcpyDeviceToHost, c.stream));
+  FVK_HIP(hipStreamSynchronize(c.stream));
+  if (e) {
+    set_error("triangular sweep: a wait exceeded its bound; the preconditioner's results are invalid");
+    return FVK_ERR_HIP;
+  }
+  return 0;
+}
+
 static int ff_solve(FlFlow &F, int isol, int ncmp, double *phi, double *rhs, int n_iter, double reduc, double *res0) {
   FlCtx &c = *F.c;
   int rc = fl_solve(c, isol, ncmp, F.val, phi, rhs, n_iter, reduc);
@@ -1212,7 +1416,7 @@ static int ff_solve(FlFlow &F, int isol, int ncmp, double *phi, double *rhs, int
   FVK_HIP(hipMemcpyAsync(k, c.ks, sizeof(FlKS) * ncmp, hipMemcpyDeviceToHost, c.stream));
   FVK_HIP(hipStreamSynchronize(c.stream));
   for (int q = 0; q < ncmp; ++q) res0[q] = k[q].norm;
-  return 0;
+  return fl_sweep_status(c);
 }
 
 static int ff_construct_pp(FlFlow &F) {
@@ -1280,6 +1484,36 @@ int fvk_fl_mesh_read_gmsh(const char *path, int n_dim, double l_sc, fvk_fl_mesh 
   if (!e.empty()) { set_error(e); return FVK_ERR_IO; }
   fvk_fl_mesh *m = new fvk_fl_mesh;
   e = fl_mesh_from_gmsh(m->m, g, n_dim, l_sc);
+  if (!e.empty()) { delete m; set_error(e); return FVK_ERR_ARG; }
+  *out = m;
+  return FVK_OK;
+}
+
+int fvk_fl_mesh_from_records(int n_dim, int n_node, const double *x3, int n_elm, const int *e_ty, const int *e_tag,
+                             const int *e_vptr, const int *e_vrt, fvk_fl_mesh **out) {
+  if (!x3 || !e_ty || !e_tag || !e_vptr || !e_vrt || !out || n_node < 1 || n_elm < 1) {
+    set_error("bad argument");
+    return FVK_ERR_ARG;
+  }
+  static const int kNvrt[8] = {0, 2, 3, 4, 4, 8, 6, 5};
+  GmshRaw g;
+  g.node_id.resize(n_node);
+  for (int i = 0; i < n_node; ++i) g.node_id[i] = i + 1;
+  g.node_x.assign(x3, x3 + 3 * (size_t)n_node);
+  g.e_vptr.assign(1, 0);
+  for (int e = 0; e < n_elm; ++e) {
+    if (e_ty[e] < 1 || e_ty[e] > 7 || e_vptr[e + 1] - e_vptr[e] != kNvrt[e_ty[e]]) {
+      set_error("element " + std::to_string(e + 1) + ": bad type or vertex count");
+      return FVK_ERR_ARG;
+    }
+    g.e_idx.push_back(e + 1);
+    g.e_ty.push_back(e_ty[e]);
+    g.e_tag0.push_back(e_tag[e]);
+    for (int k = e_vptr[e]; k < e_vptr[e + 1]; ++k) g.e_vrt.push_back(e_vrt[k]);
+    g.e_vptr.push_back((int)g.e_vrt.size());
+  }
+  fvk_fl_mesh *m = new fvk_fl_mesh;
+  const std::string e = fl_mesh_from_gmsh(m->m, g, n_dim, 1.0);
   if (!e.empty()) { delete m; set_error(e); return FVK_ERR_ARG; }
   *out = m;
   return FVK_OK;
@@ -1410,7 +1644,28 @@ int fvk_fl_lin_eqns_solver(fvk_fl_ctx *cc, const double *a_ij, double *phi, doub
     else *norm = k.norm;
   }
   hipFree(q);
-  return rc;
+  return rc ? rc : fl_sweep_status(c);
+}
+
+int fvk_fl_ctx_set_param(fvk_fl_ctx *cc, const char *name, double value) {
+  if (!cc || !name) { set_error("null argument"); return FVK_ERR_ARG; }
+  FlCtx &c = cc->c;
+  if (!std::strcmp(name, "sweep_one_launch") && (value == 0 || value == 1)) {
+    if (value == 1) {
+      int mx = 0;
+      for (int i = 0; i < c.n; ++i) {
+        int kd = -1;
+        for (int k = c.h_ia[i]; k < c.h_ia[i + 1]; ++k)
+          if (c.h_ja[k] == i) kd = k;
+        mx = std::max({mx, kd - c.h_ia[i], c.h_ia[i + 1] - 1 - kd});
+      }
+      if (mx > kSweepSlots) { set_error("a row has more lower or upper entries than one launch holds"); return FVK_ERR_ARG; }
+    }
+    c.one_launch = value != 0;
+    return FVK_OK;
+  }
+  set_error(std::string("unknown parameter or value out of range: ") + name);
+  return FVK_ERR_ARG;
 }
 
 int fvk_fl_flow_create(fvk_fl_ctx *cc, const double *params, fvk_fl_flow **out) {

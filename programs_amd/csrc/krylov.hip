@@ -27,6 +27,7 @@
 #include <cstring>
 
 #include "cgcore.h"
+#include "sweep.h"
 
 namespace fvk {
 
@@ -132,17 +133,6 @@ __global__ __launch_bounds__(BS) void k_sweep_bwd(DMesh M, const int *__restrict
 // (kSweepSpin passes per chunk); past the bound the kernel records an error
 // and its blocks stop taking chunks.  lmask / umask: bit k of a row is set
 // when slot k holds a lower / upper neighbour (rank order).
-constexpr unsigned long long kSweepMark = 0x7FF4DEADBEEF0001ull;
-constexpr unsigned kSweepSpin = 1u << 20;
-
-__device__ __forceinline__ unsigned long long ld_agent(const double *p) {
-  return __hip_atomic_load((const unsigned long long *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_agent(double *p, double v) {
-  __hip_atomic_store((unsigned long long *)p, (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
-                     __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // out[0 .. cnt) = kSweepMark and *ticket = 0, unless the gate is closed
 __global__ __launch_bounds__(BS) void k_sweep_mark(size_t cnt, double *__restrict__ out, int *__restrict__ ticket,
                                                    const KS *ks, int nvc) {
@@ -168,48 +158,10 @@ __global__ __launch_bounds__(BS) void k_sweep_masks(DMesh M, unsigned char *__re
   umask[r] = (unsigned char)up;
 }
 
-// The chunk loop shared by the three kernels: row(pos, fin, k) advances one
-// row as far as its values allow and sets fin when the row is written.
-template <class Row>
-__device__ __forceinline__ void sweep_chunks(int n, int *ticket, int *err, Row &row) {
-  __shared__ int s_chunk, s_abort;
-  if (threadIdx.x == 0) s_abort = 0;
-  for (;;) {
-    if (threadIdx.x == 0) s_chunk = atomicAdd(ticket, 1);
-    __syncthreads();
-    const int chunk = s_chunk;
-    const int abort = s_abort;
-    __syncthreads();
-    if (abort || (long)chunk * BS >= (long)n) return;
-    const int pos = chunk * BS + threadIdx.x;
-    bool fin = pos >= n;
-    if (!fin) row.start(pos);
-    unsigned spin = 0, idle = 0;
-    while (!__all(fin)) {
-      bool moved = false;
-      if (!fin) {
-        fin = row.advance();
-        moved = fin;
-      }
-      if (++spin > kSweepSpin) {
-        if (!fin) { *err = 1; s_abort = 1; }
-        break;
-      }
-      idle = __any(moved) ? 0 : idle + 1;
-      if (!__all(fin)) {
-        if (idle < 8) __builtin_amdgcn_s_sleep(1);
-        else __builtin_amdgcn_s_sleep(8);
-      }
-    }
-    __syncthreads();
-  }
-}
-
 // A row keeps its slots' columns and coefficients in registers (read once,
 // all slots at a time) and each pass issues the loads of all values it still
 // waits for at once, then takes them in slot order up to the first missing
 // one: one memory round trip per pass.  At most kSweepSlots slots.
-constexpr int kSweepSlots = 8;
 
 // DIC (imat 1) / DILU (imat 2) factor (solv_precond.f90:10-29)
 struct PrecondRow {

@@ -32,6 +32,22 @@ class FlMesh:
     def read_gmsh(cls, path, n_dim, l_sc=1.0):
         h = _vp()
         _check(lib().fvk_fl_mesh_read_gmsh(str(path).encode(), int(n_dim), float(l_sc), C.byref(h)))
+        return cls._from_handle(h)
+
+    @classmethod
+    def from_records(cls, n_dim, x3, e_ty, e_tag, e_vptr, e_vrt):
+        """From element records: x3 (n_node, 3); per element its gmsh type,
+        physical tag and node numbers (e_vptr 0-based offsets, e_vrt 1-based
+        nodes in gmsh order)."""
+        h = _vp()
+        x = _f64(x3)
+        ty, tg, vp, vr = (_i32(a) for a in (e_ty, e_tag, e_vptr, e_vrt))
+        _check(lib().fvk_fl_mesh_from_records(int(n_dim), len(x), _d(x), len(ty), _i(ty), _i(tg), _i(vp), _i(vr),
+                                              C.byref(h)))
+        return cls._from_handle(h)
+
+    @classmethod
+    def _from_handle(cls, h):
         try:
             info = np.zeros(8, np.int32)
             _check(lib().fvk_fl_mesh_get_info(h, _i(info)))
@@ -80,6 +96,9 @@ class FlCtx:
 
     destroy = _release
 
+    def set_param(self, name, value):
+        _check(lib().fvk_fl_ctx_set_param(self.h, name.encode(), float(value)))
+
     def geometry(self):
         m = self.mesh
         nd, n, nf = m.n_dim, m.n_dmn, m.n_fce
@@ -124,17 +143,14 @@ def boundary_values(mesh: FlMesh, norm, bnd_type, bnd_index, value):
     ub = np.zeros((mesh.n_bnd, nd))
     bid = BOUNDARY_OFFSET.get(bnd_type, 0) + int(bnd_index)
     v = np.asarray(value, np.float64)
-    for f in range(mesh.n_fce):
-        jn = int(mesh.fce_elm[f, 1])
-        if jn >= 0:
-            continue
-        b = -jn - 1
-        if mesh.bnd_tag[b] != bid:
-            continue
-        if np.sqrt(np.dot(v[:3], v[:3])) < 1e-20:
-            ub[b] = -norm[f, :nd] * v[3]
-        else:
-            ub[b] = v[:nd] * v[3]
+    bf = np.nonzero(mesh.fce_elm[:, 1] < 0)[0]
+    b = -mesh.fce_elm[bf, 1] - 1
+    sel = mesh.bnd_tag[b] == bid
+    f, b = bf[sel], b[sel]
+    if np.sqrt(np.dot(v[:3], v[:3])) < 1e-20:
+        ub[b] = -np.asarray(norm)[f, :nd] * v[3]
+    else:
+        ub[b] = v[:nd] * v[3]
     return ub
 
 

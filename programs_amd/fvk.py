@@ -184,10 +184,12 @@ def lib():
         "fvk_fmt_list_real": [C.c_double, C.c_char_p, C.c_int],
         "fvk_fl_mesh_read_gmsh": [C.c_char_p, C.c_int, C.c_double, C.POINTER(_vp)],
         "fvk_fl_mesh_get_info": [_vp, _ip],
+        "fvk_fl_mesh_from_records": [C.c_int, C.c_int, _dp, C.c_int, _ip, _ip, _ip, _ip, C.POINTER(_vp)],
         "fvk_fl_mesh_get_arrays": [_vp, _dp] + [_ip] * 9,
         "fvk_fl_ctx_create": [C.c_int, C.c_int, _dp, C.c_int, _ip, C.c_int, _ip, _ip, C.c_int, _ip, _ip, _ip,
                               _ip, _ip, C.c_int, C.POINTER(_vp)],
         "fvk_fl_geometry": [_vp] + [_dp] * 10,
+        "fvk_fl_ctx_set_param": [_vp, C.c_char_p, C.c_double],
         "fvk_fl_ls_gradient": [_vp, C.c_int, _dp, _dp, _dp, C.c_int],
         "fvk_fl_lin_eqns_solver": [_vp, _dp, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int],
         "fvk_fl_flow_create": [_vp, _dp, C.POINTER(_vp)],

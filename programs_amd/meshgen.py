@@ -56,3 +56,50 @@ def channel_delaunay(n_tri, length=10.0, height=1.0, jitter=0.3, seed=0x5EED0005
     v_vptr = (np.arange(nt + 1) * 3).astype(np.int32)
     v_vrt = (simp.ravel() + 1).astype(np.int32)
     return pts, v_ty, v_vptr, v_vrt, bnd.ravel()
+
+
+def box_records(n, jitter=0.0, seed=0x5EED0006, lid=31, wall=32):
+    """Lid-driven cavity as gmsh element records for the flowlab path: the
+    unit cube in n^3 hexahedra (gmsh vertex order), interior vertices moved by
+    U(-jitter h, jitter h) per coordinate (numpy's PCG64 seeded with seed),
+    boundary quadrilaterals tagged lid (z = 1) or wall.  Returns
+    (x3, e_ty, e_tag, e_vptr, e_vrt) for flowlab.FlMesh.from_records:
+    boundary elements first, then the cells in x-fastest order."""
+    m = n + 1
+    g = np.arange(m, dtype=np.float64) / n
+    z, y, x = np.meshgrid(g, g, g, indexing="ij")
+    x3 = np.stack([x.ravel(), y.ravel(), z.ravel()], 1)
+    if jitter > 0:
+        idx = np.arange(m)
+        kz, ky, kx = np.meshgrid(idx, idx, idx, indexing="ij")
+        inner = ((kx > 0) & (kx < n) & (ky > 0) & (ky < n) & (kz > 0) & (kz < n)).ravel()
+        rng = np.random.default_rng(seed)
+        d = rng.uniform(-jitter / n, jitter / n, size=x3.shape)
+        x3[inner] += d[inner]
+    node = lambda i, j, k: (k * m + j) * m + i + 1
+    i, j, k = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    i, j, k = (a.transpose(2, 1, 0).ravel() for a in (i, j, k))  # x fastest
+    hexes = np.stack([node(i, j, k), node(i + 1, j, k), node(i + 1, j + 1, k), node(i, j + 1, k),
+                      node(i, j, k + 1), node(i + 1, j, k + 1), node(i + 1, j + 1, k + 1), node(i, j + 1, k + 1)], 1)
+    a, b = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    a, b = a.ravel(), b.ravel()
+    quads, tags = [], []
+    for side in range(6):
+        ax, hi = side // 2, side % 2
+        c = np.full_like(a, n if hi else 0)
+        if ax == 0:
+            q = [node(c, a, b), node(c, a + 1, b), node(c, a + 1, b + 1), node(c, a, b + 1)]
+        elif ax == 1:
+            q = [node(a, c, b), node(a + 1, c, b), node(a + 1, c, b + 1), node(a, c, b + 1)]
+        else:
+            q = [node(a, b, c), node(a + 1, b, c), node(a + 1, b + 1, c), node(a, b + 1, c)]
+        quads.append(np.stack(q, 1))
+        tags.append(np.full(len(a), lid if (ax == 2 and hi) else wall))
+    quads = np.concatenate(quads)
+    tags = np.concatenate(tags)
+    nq, nh = len(quads), len(hexes)
+    e_ty = np.concatenate([np.full(nq, 3), np.full(nh, 5)]).astype(np.int32)
+    e_tag = np.concatenate([tags, np.zeros(nh)]).astype(np.int32)
+    e_vptr = np.concatenate([[0], np.cumsum(np.concatenate([np.full(nq, 4), np.full(nh, 8)]))]).astype(np.int32)
+    e_vrt = np.concatenate([quads.ravel(), hexes.ravel()]).astype(np.int32)
+    return x3, e_ty, e_tag, e_vptr, e_vrt

@@ -15,15 +15,14 @@ fields.  Run from the repository root:  python -m tests.golden.make_flowlab
 import gzip
 import os
 import shutil
-import subprocess
 import sys
 import tempfile
 
 import numpy as np
 
+from oracle.flowlab_ref import available, read_output, run, write_input
+
 HERE = os.path.dirname(os.path.abspath(__file__))
-ROOT = os.path.dirname(os.path.dirname(HERE))
-DRIVER = os.path.join(ROOT, "oracle", "_ref", "flowlab", "flowlab_ref_driver")
 
 # name: mesh file, n_dim, density, viscosity, urf_u, urf_p, blending, boundary (type, index, value),
 #       outer iterations, residual drop, iterations dumped
@@ -52,80 +51,6 @@ def mesh_path(fname, tmp):
     return p
 
 
-def write_input(path, m, par, n_iter, n_dump, vel_bnd):
-    i4 = lambda a: np.asarray(a, "<i4").tobytes()
-    f8 = lambda a: np.asarray(a, "<f8").tobytes()
-    with open(path, "wb") as f:
-        f.write(i4([m.n_dim, m.n_vrt, m.n_bnd, m.n_dmn, m.n_fce, m.nnz, n_iter, n_dump]))
-        f.write(f8(par))
-        f.write(f8(m.x))  # (n_vrt, 3) row-major = x(3, n_vrt) column-major
-        f.write(i4(m.elm_vptr)); f.write(i4(m.elm_vrt))
-        f.write(i4(m.bnd_tag))
-        f.write(i4(m.fce_elm))  # (n_fce, 2) row-major = col(2*n_fce)
-        f.write(i4(m.fce_vptr)); f.write(i4(m.fce_vrt))
-        f.write(i4(m.ng_ptr)); f.write(i4(m.ng_col))
-        f.write(f8(vel_bnd))  # (n_bnd, n_dim) row-major = phi%bnd(n_dim, n_bnd)
-
-
-class Reader:
-    def __init__(self, path):
-        self.b = open(path, "rb").read()
-        self.o = 0
-
-    def f8(self, *shape):
-        k = int(np.prod(shape))
-        a = np.frombuffer(self.b, "<f8", k, self.o).copy()
-        self.o += 8 * k
-        # Fortran (a, b, c) column-major -> numpy (c, b, a)
-        return a.reshape(tuple(reversed(shape))) if len(shape) > 1 else a
-
-    def i4(self):
-        v = int(np.frombuffer(self.b, "<i4", 1, self.o)[0])
-        self.o += 4
-        return v
-
-
-def read_output(path, m, n_iter, n_dump):
-    r = Reader(path)
-    nd, n, nf, nb, nnz = m.n_dim, m.n_dmn, m.n_fce, m.n_bnd, m.nnz
-    geo = [("x_vc", (nd, n)), ("vol", (n,)), ("x_fc", (nd, nf)), ("norm", (nd, nf)), ("area", (nf,)),
-           ("x_pc", (nd, nf)), ("x_nc", (nd, nf)), ("l_pn", (nf,)), ("w_fp", (nf,)), ("ls", (nd, nd, n))]
-    dump = [("pres_grad", (nd, 1, n)), ("pres_bnd", (1, nb)), ("vel_grad", (nd, nd, n)), ("mom_val", (nnz,)),
-            ("mom_rhs", (nd, n)), ("mom_diag", (n,)), ("vel_bnd", (nd, nb)), ("vel_solved", (nd, n)),
-            ("mom_resid", (nd, n)), ("mom_res0", (3,)), ("pp_val", (nnz,)), ("pp_rhs", (n,)), ("spd", (nf,)),
-            ("pp", (1, n)), ("pp_res0", (1,)), ("pp_grad", (nd, 1, n)), ("pp_bnd", (1, nb)), ("vel", (nd, n)),
-            ("pres", (1, n))]
-    fin = [("vel", (nd, n)), ("pres", (1, n)), ("pp", (1, n)), ("spd", (nf,)), ("vel_bnd", (nd, nb)),
-           ("pres_bnd", (1, nb))]
-    sz = lambda lst: 8 * sum(int(np.prod(s)) for _, s in lst)
-    # iterations run: the file holds geometry, per iteration its dumps and 5
-    # doubles, then the count and the final fields
-    total, k = len(r.b), None
-    for kk in range(0, n_iter + 1):
-        if sz(geo) + sum(sz(dump) * (it <= n_dump) + 40 for it in range(1, kk + 1)) + 4 + sz(fin) == total:
-            k = kk
-            break
-    assert k is not None, "unexpected driver output size"
-    o = {"geo_" + name: r.f8(*shp) for name, shp in geo}
-    res0, gl = [], []
-    for it in range(1, k + 1):
-        if it <= n_dump:
-            for name, shp in dump:
-                a = r.f8(*shp)
-                o["it%d_%s" % (it, name)] = a.reshape(a.shape[0], -1)[:, 0] if shp[0] == 1 and len(shp) == 2 else (
-                    a[:, 0, :] if len(shp) == 3 and shp[1] == 1 else a)
-        res0.append(r.f8(4))
-        gl.append(r.f8(1)[0])
-    o["n_done"] = np.int32(r.i4())
-    o["res0"] = np.array(res0).reshape(-1, 4)
-    o["norm_gl"] = np.array(gl)
-    for name, shp in fin:
-        a = r.f8(*shp)
-        o["final_" + name] = a[:, 0] if shp[0] == 1 and len(shp) == 2 else a
-    assert r.o == total and int(o["n_done"]) == k, (r.o, total, int(o["n_done"]), k)
-    return o
-
-
 def run_case(name, spec, tmp):
     from programs_amd import flowlab as fl
     fname, nd, rho, mu, urf_u, urf_p, blc, bnd, n_iter, drop, n_dump = spec
@@ -134,11 +59,11 @@ def run_case(name, spec, tmp):
     inp, out = os.path.join(tmp, name + ".in"), os.path.join(tmp, name + ".out")
     # geometry first (no iterations): the boundary values need the normals
     write_input(inp, m, par, 0, 0, np.zeros((m.n_bnd, nd)))
-    subprocess.run([DRIVER, inp, out], check=True)
+    run(inp, out)
     g = read_output(out, m, 0, 0)
     ub = fl.boundary_values(m, g["geo_norm"], *bnd)
     write_input(inp, m, par, n_iter, n_dump, ub)
-    subprocess.run([DRIVER, inp, out], check=True)
+    run(inp, out)
     o = read_output(out, m, n_iter, n_dump)
     o["vel_bnd0"] = ub
     if m.n_dmn > SLIM:
@@ -154,7 +79,7 @@ def run_case(name, spec, tmp):
 
 
 def main():
-    if not os.path.exists(DRIVER):
+    if not available():
         sys.exit("build the reference first: bash oracle/build_ref.sh")
     names = sys.argv[1:] or list(CASES)
     with tempfile.TemporaryDirectory() as tmp:
