@@ -481,6 +481,13 @@ void fvk_fl_mesh_free(fvk_fl_mesh *m);
 int fvk_fl_ctx_create(int n_dim, int n_vrt, const double *x3, int n_bnd, const int *bnd_tag, int n_dmn,
                       const int *elm_vptr, const int *elm_vrt, int n_fce, const int *fce_elm, const int *fce_vptr,
                       const int *fce_vrt, const int *ng_ptr, const int *ng_col, int device, fvk_fl_ctx **out);
+/* A context holding only a cell matrix pattern (ia n+1, ja nnz, 1-based
+   CSR as solv_m receives them), for fvk_fl_lin_eqns_solver alone: the
+   drop-in solv_m of flowlab, cfdpack and pcns (one source, byte-identical in
+   the three).  fvk_fl_csr_matches: 1 when (n, ia, ja) is the context's
+   pattern. */
+int fvk_fl_csr_create(int n, const int *ia, const int *ja, int device, fvk_fl_ctx **out);
+int fvk_fl_csr_matches(const fvk_fl_ctx *ctx, int n, const int *ia, const int *ja);
 void fvk_fl_ctx_destroy(fvk_fl_ctx *ctx);
 /* "sweep_one_launch" 1 (default when every row has at most 8 lower and 8
    upper entries): the factor and direct_solve's sweeps in one launch each,

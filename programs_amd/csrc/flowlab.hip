@@ -45,6 +45,10 @@ struct FlCtx {
   int *j1 = nullptr, *jn = nullptr, *btag = nullptr;
   // matrix pattern
   int *ia = nullptr, *ja = nullptr, *kii = nullptr, *kij = nullptr, *kji = nullptr, *tpos = nullptr;
+  // a row's lower entries are [ia, klo) (the reference's loops stop at the
+  // first column >= the row), its upper entries [kup, ia+1) (taken from the
+  // end down to the first column <= the row)
+  int *klo = nullptr, *kup = nullptr;
   int *cptr = nullptr, *cfl = nullptr;
   // triangular sweeps by levels (lower / upper dependencies)
   std::vector<int> lf_ptr, lb_ptr;
@@ -566,7 +570,7 @@ __global__ __launch_bounds__(BS) void k_fl_mark(size_t cnt, double *out, int *ti
 // preconditioner (solv_m.f90:224-250) for one row: the lower entries in
 // column order
 struct FlPcRow {
-  const int *ia, *ja, *kii, *tpos, *order;
+  const int *ia, *ja, *kii, *klo, *tpos, *order;
   const double *a;
   double *pc;
   int r, k, nl;
@@ -576,7 +580,7 @@ struct FlPcRow {
   __device__ void start(int pos) {
     r = order[pos];
     const int lo = ia[r];
-    nl = kii[r] - lo;
+    nl = klo[r] - lo;
     s = a[kii[r]];
     k = 0;
 #pragma unroll
@@ -613,7 +617,7 @@ struct FlPcRow {
 // writes out from the upper entries, last column first
 template <bool FWD, int NC>
 struct FlSweepRow {
-  const int *ia, *ja, *kii, *order;
+  const int *ia, *ja, *klo, *kup, *order;
   const double *a, *pc, *src;
   double *out;
   int n, r, k, nl;
@@ -625,11 +629,11 @@ struct FlSweepRow {
     int first, step;
     if (FWD) {
       first = ia[r];
-      nl = kii[r] - first;
+      nl = klo[r] - first;
       step = 1;
     } else {
       first = ia[r + 1] - 1;
-      nl = first - kii[r];
+      nl = first - kup[r] + 1;
       step = -1;
     }
     k = 0;
@@ -679,23 +683,23 @@ struct FlSweepRow {
 
 __global__ __launch_bounds__(BS) void k_fl_pc_sf(int n, const int *__restrict__ order, const int *__restrict__ ia,
                                                  const int *__restrict__ ja, const int *__restrict__ kii,
-                                                 const int *__restrict__ tpos, const double *__restrict__ a,
-                                                 double *pc, int *ticket, int *err) {
+                                                 const int *__restrict__ klo, const int *__restrict__ tpos,
+                                                 const double *__restrict__ a, double *pc, int *ticket, int *err) {
   FlPcRow row;
-  row.ia = ia; row.ja = ja; row.kii = kii; row.tpos = tpos; row.order = order; row.a = a; row.pc = pc;
+  row.ia = ia; row.ja = ja; row.kii = kii; row.klo = klo; row.tpos = tpos; row.order = order; row.a = a; row.pc = pc;
   row.r = 0; row.k = 0; row.nl = 0; row.s = 0.0;
   sweep_chunks(n, ticket, err, row);
 }
 
 template <bool FWD, int NC>
 __global__ __launch_bounds__(BS) void k_fl_sweep_sf(int n, const int *__restrict__ order, const int *__restrict__ ia,
-                                                    const int *__restrict__ ja, const int *__restrict__ kii,
-                                                    const double *__restrict__ a, const double *__restrict__ pc,
-                                                    const double *__restrict__ src, double *out, int *ticket,
-                                                    int *err, const FlKS *ks) {
+                                                    const int *__restrict__ ja, const int *__restrict__ klo,
+                                                    const int *__restrict__ kup, const double *__restrict__ a,
+                                                    const double *__restrict__ pc, const double *__restrict__ src,
+                                                    double *out, int *ticket, int *err, const FlKS *ks) {
   if (fl_all_done(ks, NC)) return;
   FlSweepRow<FWD, NC> row;
-  row.ia = ia; row.ja = ja; row.kii = kii; row.order = order; row.a = a; row.pc = pc; row.src = src;
+  row.ia = ia; row.ja = ja; row.klo = klo; row.kup = kup; row.order = order; row.a = a; row.pc = pc; row.src = src;
   row.out = out; row.n = n; row.r = 0; row.k = 0; row.nl = 0;
   sweep_chunks(n, ticket, err, row);
 }
@@ -730,7 +734,7 @@ static int fl_mark_grid(size_t cnt) { return (int)std::max<size_t>(1, std::min<s
 static int fl_precond(FlCtx &c, const double *a) {
   if (c.one_launch) {
     k_fl_mark<<<fl_mark_grid(c.n), BS, 0, c.stream>>>(c.n, c.pc, c.sweep_int + 2, nullptr, 1);
-    k_fl_pc_sf<<<fl_sweep_grid(c), BS, 0, c.stream>>>(c.n, c.lf_rows, c.ia, c.ja, c.kii, c.tpos, a, c.pc,
+    k_fl_pc_sf<<<fl_sweep_grid(c), BS, 0, c.stream>>>(c.n, c.lf_rows, c.ia, c.ja, c.kii, c.klo, c.tpos, a, c.pc,
                                                        c.sweep_int + 2, c.sweep_int + 3);
     FVK_HIP(hipGetLastError());
     return 0;
@@ -750,10 +754,10 @@ static int fl_direct(FlCtx &c, int ncmp, const double *a, const double *b, doubl
     const size_t cnt = (size_t)ncmp * c.n;
     const int G = fl_sweep_grid(c);
     k_fl_mark<<<fl_mark_grid(cnt), BS, 0, c.stream>>>(cnt, t, c.sweep_int, ks, ncmp);
-#define FWD(C) k_fl_sweep_sf<true, C><<<G, BS, 0, c.stream>>>(c.n, c.lf_rows, c.ia, c.ja, c.kii, a, c.pc, b, t, \
-                                                              c.sweep_int, c.sweep_int + 3, ks)
-#define BWD(C) k_fl_sweep_sf<false, C><<<G, BS, 0, c.stream>>>(c.n, c.lb_rows, c.ia, c.ja, c.kii, a, c.pc, t, x, \
-                                                               c.sweep_int + 1, c.sweep_int + 3, ks)
+#define FWD(C) k_fl_sweep_sf<true, C><<<G, BS, 0, c.stream>>>(c.n, c.lf_rows, c.ia, c.ja, c.klo, c.kup, a, c.pc, b, \
+                                                              t, c.sweep_int, c.sweep_int + 3, ks)
+#define BWD(C) k_fl_sweep_sf<false, C><<<G, BS, 0, c.stream>>>(c.n, c.lb_rows, c.ia, c.ja, c.klo, c.kup, a, c.pc, t, \
+                                                               x, c.sweep_int + 1, c.sweep_int + 3, ks)
     if (ncmp == 1) FWD(1); else if (ncmp == 2) FWD(2); else FWD(3);
     k_fl_mark<<<fl_mark_grid(cnt), BS, 0, c.stream>>>(cnt, x, c.sweep_int + 1, ks, ncmp);
     if (ncmp == 1) BWD(1); else if (ncmp == 2) BWD(2); else BWD(3);
@@ -1180,11 +1184,68 @@ int fl_grad(FlCtx &c, int ncmp, const double *phi, double *phib, double *g, int 
 }
 
 // ---------------------------------------------------------------- host set-up
+// The cell matrix pattern (1-based CSR) and the Krylov workspace.
+static int fl_csr_setup(FlCtx &c, int n, const int *ngp, const int *ngc) {
+  c.n = n;
+  c.nnz = ngp[n] - 1;
+  if (ngp[0] != 1 || c.nnz < n) { set_error("bad matrix pattern"); return FVK_ERR_ARG; }
+  std::vector<int> ia(n + 1), ja(c.nnz);
+  for (int i = 0; i <= n; ++i) {
+    ia[i] = ngp[i] - 1;
+    if (i && ia[i] < ia[i - 1]) { set_error("bad matrix pattern"); return FVK_ERR_ARG; }
+  }
+  for (int k = 0; k < c.nnz; ++k) {
+    ja[k] = ngc[k] - 1;
+    if (ja[k] < 0 || ja[k] >= n) { set_error("matrix column out of range"); return FVK_ERR_ARG; }
+  }
+  auto csr_index = [&](int i, int j) {
+    for (int k = ia[i]; k < ia[i + 1]; ++k)
+      if (ja[k] == j) return k;
+    return -1;
+  };
+  std::vector<int> kii(n), klo(n), kup(n), tpos(c.nnz, -1);
+  int mx = 0;
+  for (int i = 0; i < n; ++i) {
+    if ((kii[i] = csr_index(i, i)) < 0) { set_error("matrix pattern lacks a diagonal entry"); return FVK_ERR_ARG; }
+    int k = ia[i];
+    while (k < ia[i + 1] && ja[k] < i) ++k;
+    klo[i] = k;
+    k = ia[i + 1];
+    while (k > ia[i] && ja[k - 1] > i) --k;
+    kup[i] = k;
+    mx = std::max({mx, klo[i] - ia[i], ia[i + 1] - kup[i]});
+    for (int q = ia[i]; q < klo[i]; ++q)
+      if ((tpos[q] = csr_index(ja[q], i)) < 0) { set_error("matrix pattern is not symmetric"); return FVK_ERR_ARG; }
+  }
+  c.one_launch = mx <= kSweepSlots;
+  c.h_ia = ia;
+  c.h_ja = ja;
+  int rc = 0;
+#define A(ptr, cnt_) if (!rc) rc = fl_alloc(c, &ptr, cnt_)
+#define U(ptr, vec) if (!rc) rc = fl_up(c, &ptr, vec)
+  U(c.ia, ia); U(c.ja, ja); U(c.kii, kii); U(c.klo, klo); U(c.kup, kup); U(c.tpos, tpos);
+  A(c.wk, (size_t)6 * 3 * n); A(c.pc, n); A(c.part, (size_t)6 * RG); A(c.ks, 3); A(c.scal, 8);
+  A(c.sweep_int, 4);
+#undef A
+#undef U
+  if (rc) return rc;
+  FVK_HIP(hipDeviceGetAttribute(&c.n_cu, hipDeviceAttributeMultiprocessorCount, c.device));
+  std::vector<int> lrows;
+  fl_levels(ia, ja, n, true, c.lf_ptr, lrows);
+  if ((rc = fl_up(c, &c.lf_rows, lrows))) return rc;
+  fl_levels(ia, ja, n, false, c.lb_ptr, lrows);
+  if ((rc = fl_up(c, &c.lb_rows, lrows))) return rc;
+  return 0;
+}
+
 static int fl_ctx_build(FlCtx &c, int nd, int n_vrt, const double *x3, int nb, const int *btag, int n,
                         const int *evp, const int *evr, int nf, const int *fce_elm, const int *fvp, const int *fvr,
                         const int *ngp, const int *ngc) {
   c.nd = nd; c.n_vrt = n_vrt; c.nb = nb; c.n = n; c.nf = nf;
-  c.nnz = ngp[n] - 1;
+  {
+    const int rc = fl_csr_setup(c, n, ngp, ngc);
+    if (rc) return rc;
+  }
   // element and face numbering -> 0-based local indices
   std::vector<int> j1(nf), jn(nf);
   for (int f = 0; f < nf; ++f) {
@@ -1199,30 +1260,18 @@ static int fl_ctx_build(FlCtx &c, int nd, int n_vrt, const double *x3, int nb, c
       jn[f] = b;  // -(boundary element), i.e. -(b0 + 1) with b0 0-based
     }
   }
-  std::vector<int> ia(n + 1), ja(c.nnz);
-  for (int i = 0; i <= n; ++i) ia[i] = ngp[i] - 1;
-  for (int k = 0; k < c.nnz; ++k) {
-    ja[k] = ngc[k] - 1;
-    if (ja[k] < 0 || ja[k] >= n) { set_error("matrix column out of range"); return FVK_ERR_ARG; }
-  }
+  const std::vector<int> &ia = c.h_ia, &ja = c.h_ja;
   auto csr_index = [&](int i, int j) {
     for (int k = ia[i]; k < ia[i + 1]; ++k)
       if (ja[k] == j) return k;
     return -1;
   };
-  std::vector<int> kii(n), kij(nf, -1), kji(nf, -1), tpos(c.nnz, -1);
-  for (int i = 0; i < n; ++i)
-    if ((kii[i] = csr_index(i, i)) < 0) { set_error("matrix pattern lacks a diagonal entry"); return FVK_ERR_ARG; }
+  std::vector<int> kij(nf, -1), kji(nf, -1);
   for (int f = 0; f < nf; ++f)
     if (jn[f] >= 0) {
       kij[f] = csr_index(j1[f], jn[f]);
       kji[f] = csr_index(jn[f], j1[f]);
       if (kij[f] < 0 || kji[f] < 0) { set_error("matrix pattern lacks a face's entries"); return FVK_ERR_ARG; }
-    }
-  for (int i = 0; i < n; ++i)
-    for (int k = ia[i]; k < ia[i + 1]; ++k) {
-      if (ja[k] >= i) break;
-      if ((tpos[k] = csr_index(ja[k], i)) < 0) { set_error("matrix pattern is not symmetric"); return FVK_ERR_ARG; }
     }
   // cell face lists, ascending
   std::vector<int> cnt(n + 1, 0), cfl;
@@ -1253,7 +1302,7 @@ static int fl_ctx_build(FlCtx &c, int nd, int n_vrt, const double *x3, int nb, c
       set_error("face with too few vertices");
       return FVK_ERR_ARG;
     }
-  c.h_ia = ia; c.h_ja = ja; c.h_j1 = j1; c.h_jn = jn;
+  c.h_j1 = j1; c.h_jn = jn;
   c.h_btag.assign(btag, btag + nb);
   int rc = 0;
 #define A(ptr, cnt_) if (!rc) rc = fl_alloc(c, &ptr, cnt_)
@@ -1263,27 +1312,14 @@ static int fl_ctx_build(FlCtx &c, int nd, int n_vrt, const double *x3, int nb, c
   int *devp = nullptr, *devr = nullptr, *dfvp = nullptr, *dfvr = nullptr;
   U(dxv, xv); U(devp, evp0); U(devr, evr0); U(dfvp, fvp0); U(dfvr, fvr0);
   U(c.j1, j1); U(c.jn, jn); U(c.btag, c.h_btag);
-  U(c.ia, ia); U(c.ja, ja); U(c.kii, kii); U(c.kij, kij); U(c.kji, kji); U(c.tpos, tpos);
+  U(c.kij, kij); U(c.kji, kji);
   U(c.cptr, cnt); U(c.cfl, cfl);
   A(c.xvc, (size_t)nd * n); A(c.vol, n); A(c.volr, n); A(c.ls, (size_t)nd * nd * n);
   A(c.xfc, (size_t)nd * nf); A(c.nrm, (size_t)nd * nf); A(c.area, nf); A(c.xpc, (size_t)nd * nf);
   A(c.xnc, (size_t)nd * nf); A(c.lpn, nf); A(c.wfp, nf); A(c.fvol, nf);
-  A(c.wk, (size_t)6 * 3 * n); A(c.pc, n); A(c.part, (size_t)6 * RG); A(c.ks, 3); A(c.scal, 8);
 #undef A
 #undef U
   if (rc) return rc;
-  {
-    int mx = 0;
-    for (int i = 0; i < n; ++i) mx = std::max({mx, kii[i] - ia[i], ia[i + 1] - 1 - kii[i]});
-    c.one_launch = mx <= kSweepSlots;
-  }
-  FVK_HIP(hipDeviceGetAttribute(&c.n_cu, hipDeviceAttributeMultiprocessorCount, c.device));
-  if ((rc = fl_alloc(c, &c.sweep_int, 4))) return rc;
-  std::vector<int> lrows;
-  fl_levels(ia, ja, n, true, c.lf_ptr, lrows);
-  if ((rc = fl_up(c, &c.lf_rows, lrows))) return rc;
-  fl_levels(ia, ja, n, false, c.lb_ptr, lrows);
-  if ((rc = fl_up(c, &c.lb_rows, lrows))) return rc;
   k_fl_centre<<<grid_for(n), BS, 0, c.stream>>>(n, nd, dxv, devp, devr, c.xvc);
   k_fl_face_geom<<<grid_for(nf), BS, 0, c.stream>>>(nf, n, nd, dxv, dfvp, dfvr, c.j1, c.jn, c.xvc, c.xfc, c.nrm,
                                                      c.area, c.xpc, c.xnc, c.lpn, c.wfp, c.fvol);
@@ -1561,6 +1597,31 @@ int fvk_fl_ctx_create(int n_dim, int n_vrt, const double *x3, int n_bnd, const i
   return FVK_OK;
 }
 
+int fvk_fl_csr_create(int n, const int *ia, const int *ja, int device, fvk_fl_ctx **out) {
+  if (!ia || !ja || !out || n < 1) { set_error("bad argument"); return FVK_ERR_ARG; }
+  fvk_fl_ctx *cc = new fvk_fl_ctx;
+  FlCtx &c = cc->c;
+  c.device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete cc; set_error(hipGetErrorString(e)); return FVK_ERR_HIP; }
+  const int rc = fl_csr_setup(c, n, ia, ja);
+  if (rc) { fl_ctx_free(c); delete cc; return rc; }
+  *out = cc;
+  return FVK_OK;
+}
+
+int fvk_fl_csr_matches(const fvk_fl_ctx *cc, int n, const int *ia, const int *ja) {
+  if (!cc || !ia || !ja) return 0;
+  const FlCtx &c = cc->c;
+  if (n != c.n || ia[n] - 1 != c.nnz) return 0;
+  for (int i = 0; i <= n; ++i)
+    if (ia[i] - 1 != c.h_ia[i]) return 0;
+  for (int k = 0; k < c.nnz; ++k)
+    if (ja[k] - 1 != c.h_ja[k]) return 0;
+  return 1;
+}
+
 void fvk_fl_ctx_destroy(fvk_fl_ctx *cc) {
   if (!cc) return;
   fl_ctx_free(cc->c);
@@ -1654,10 +1715,10 @@ int fvk_fl_ctx_set_param(fvk_fl_ctx *cc, const char *name, double value) {
     if (value == 1) {
       int mx = 0;
       for (int i = 0; i < c.n; ++i) {
-        int kd = -1;
-        for (int k = c.h_ia[i]; k < c.h_ia[i + 1]; ++k)
-          if (c.h_ja[k] == i) kd = k;
-        mx = std::max({mx, kd - c.h_ia[i], c.h_ia[i + 1] - 1 - kd});
+        int lo = c.h_ia[i], up = c.h_ia[i + 1];
+        while (lo < c.h_ia[i + 1] && c.h_ja[lo] < i) ++lo;
+        while (up > c.h_ia[i] && c.h_ja[up - 1] > i) --up;
+        mx = std::max({mx, lo - c.h_ia[i], c.h_ia[i + 1] - up});
       }
       if (mx > kSweepSlots) { set_error("a row has more lower or upper entries than one launch holds"); return FVK_ERR_ARG; }
     }

@@ -43,3 +43,26 @@ $FC $FFLAGS -o "$OUT/fvk_ref_driver" $objs -L"$ROOT/programs_amd" -lfvk \
    programs_amd/fortran/fvk_grd_solv.f90 programs_amd/fortran/fvk_cfd_setup.f90 \
    oracle/ref/fvk_ref_driver.f90 | sha256sum | cut -d' ' -f1) > "$OUT/sources.sha256"
 echo "built $OUT/fvk_ref_driver"
+
+# flowlab: its own Fortran modules with solv_m replaced by fvk_fl_solv.f90
+# (lin_eqns_solver on the GPU), behind the golden-vector driver
+# oracle/ref/flowlab_ref_driver.f90 -> oracle/_ref/dropin_flowlab/
+FLOWLAB=${FVK_FLOWLAB_DIR:-$REF/../flowlab/src}
+OUTF=$ROOT/oracle/_ref/dropin_flowlab
+mkdir -p "$OUTF" "$SCRATCH/flowlab"
+fobjs=""
+funit() {
+  local src=$1 name
+  name=$(basename "$src" .f90)
+  (cd "$SCRATCH/flowlab" && $FC $FFLAGS -cpp -c "$src" -o "$name.o")
+  fobjs="$fobjs $SCRATCH/flowlab/$name.o"
+}
+for m in prec_m constants_m vector_utils_m matrix_utils_m graph_m hash_m; do funit "$FLOWLAB/$m.f90"; done
+funit "$HERE/fvk_fl_solv.f90"
+for m in geometry_m transport_m; do funit "$FLOWLAB/$m.f90"; done
+funit "$ROOT/oracle/ref/flowlab_ref_driver.f90"
+$FC $FFLAGS -o "$OUTF/flowlab_ref_driver" $fobjs -L"$ROOT/programs_amd" -lfvk \
+    -Wl,-rpath,'$ORIGIN/../../../programs_amd'
+(cd "$ROOT" && sha256sum programs_amd/fortran/fvk_fl_solv.f90 oracle/ref/flowlab_ref_driver.f90 \
+   | sha256sum | cut -d' ' -f1) > "$OUTF/sources.sha256"
+echo "built $OUTF/flowlab_ref_driver"

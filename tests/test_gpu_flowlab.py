@@ -210,3 +210,45 @@ def test_one_launch_sweeps_bitwise_on_a_box():
     bitwise(r1, r0, "residual norms")
     for k in ("vel", "pres", "pp", "spd"):
         bitwise(s1[k], s0[k], k)
+
+
+# ---------------------------------------------------------------- Fortran drop-in
+DROPIN = os.path.join(os.path.dirname(GOLDEN), "..", "oracle", "_ref", "dropin_flowlab", "flowlab_ref_driver")
+DROPIN_STAMP = os.path.join(os.path.dirname(DROPIN), "sources.sha256")
+DROPIN_SOURCES = ["programs_amd/fortran/fvk_fl_solv.f90", "oracle/ref/flowlab_ref_driver.f90"]
+
+
+def dropin_sources_hash():
+    import hashlib
+    root = os.path.dirname(os.path.dirname(GOLDEN))
+    lines = "".join("%s  %s\n" % (hashlib.sha256(open(os.path.join(root, s), "rb").read()).hexdigest(), s)
+                    for s in DROPIN_SOURCES)
+    return hashlib.sha256(lines.encode()).hexdigest()
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_fortran_dropin_solv_m(name, tmp_path):
+    """flowlab's own Fortran (geometry_m, transport_m and the golden driver's
+    outer loop) linked with the drop-in solv_m (programs_amd/fortran/
+    fvk_fl_solv.f90: lin_eqns_solver on the GPU through the C ABI): its run
+    must give the reference's residual norms and fields (1e-9).  The
+    executable is built by programs_amd/fortran/build_dropin.sh in the
+    development container and travels with the tree; a missing or stale one
+    fails."""
+    from oracle import flowlab_ref
+    assert os.path.exists(DROPIN), "drop-in executable missing: run programs_amd/fortran/build_dropin.sh"
+    assert open(DROPIN_STAMP).read().strip() == dropin_sources_hash(), "drop-in executable is stale: rebuild it"
+    fl, rec, spec, m, ctx = setup(name, tmp_path)
+    ctx.destroy()
+    say("drop-in %s: %d cells" % (name, m.n_dmn))
+    inp, out = os.path.join(str(tmp_path), "a.in"), os.path.join(str(tmp_path), "a.out")
+    par = [spec["rho"], spec["mu"], spec["urf_u"], spec["urf_p"], spec["bl_c"], spec["res_drop"]]
+    flowlab_ref.write_input(inp, m, par, spec["n_iter"], 0, rec["vel_bnd0"])
+    import subprocess
+    subprocess.run([DROPIN, inp, out], check=True, preexec_fn=flowlab_ref._big_stack)
+    o = flowlab_ref.read_output(out, m, spec["n_iter"], 0)
+    assert int(o["n_done"]) == int(rec["n_done"])
+    bitwise(o["geo_vol"], rec["geo_vol"], "vol (reference code)")
+    close(o["res0"], rec["res0"], 1e-9, "residual norms, all iterations")
+    for k in ("vel", "pres", "pp", "spd", "vel_bnd"):
+        close(o["final_" + k], rec["final_" + k], 1e-9, k)
