@@ -486,8 +486,9 @@ def flowlab_leg(fvk, args):
     """flowlab's serial CSR SIMPLE (flowlab/src/navierstokes.f90:515-641) on
     an n^3 lid-driven cavity (+-10% jitter, Re 100, urf 0.7 / 0.3, blending
     0.5): outer iterations per second with the factor and sweeps in one
-    launch each and with one launch per level, beside one outer iteration of
-    flowlab's own Fortran (flang -O2, one core) on the same mesh."""
+    launch each (blocks owning row ranges; rows in level order in chunks)
+    and with one launch per level, beside one outer iteration of flowlab's
+    own Fortran (flang -O2, one core) on the same mesh."""
     from programs_amd import flowlab as fl
     from programs_amd import meshgen
     n = args.flowlab
@@ -502,21 +503,25 @@ def flowlab_leg(fvk, args):
     out = dict(workload="flowlab CSR SIMPLE, %d^3 lid-driven cavity (+-10%% jitter), Re 100, momentum BiCGStab and "
                         "pressure CG (25 iterations, reduction 0.01) with flowlab's factor" % n,
                cells=m.n_dmn, faces=m.n_fce, nnz=m.nnz, mesh_build_s=t_mesh, context_s=t_ctx)
-    for one in (1, 0):
+    # one launch per sweep with row ranges owned by blocks (the default), with
+    # rows taken in level order in chunks, and one launch per level
+    for key, one, parts in (("one_launch", 1, 1), ("one_launch_chunks", 1, 0), ("per_level_launches", 0, 0)):
         ctx.set_param("sweep_one_launch", one)
+        if one:
+            ctx.set_param("sweep_parts", parts)
         fw = fl.FlFlow(ctx, *par)
         fw.set(vel_bnd=ub)
         fw.run(2, 0.0)
-        K = args.flowlab_its if one else max(1, args.flowlab_its // 2)
+        K = args.flowlab_its if parts else max(1, args.flowlab_its // 2)
         t0 = time.perf_counter()
         res0, gl = fw.run(K, 0.0)
         dt = time.perf_counter() - t0
-        key = "one_launch" if one else "per_level_launches"
         out[key] = dict(outer_iterations=K, ms_per_outer=dt * 1e3 / K, outer_its_per_s=K / dt,
                         norm_gl_last=float(gl[-1]))
         log("flowlab %d^3 %s: %.1f ms per outer iteration" % (n, key, dt * 1e3 / K))
         fw.destroy()
     ctx.set_param("sweep_one_launch", 1)
+    ctx.set_param("sweep_parts", 1)
     if not args.no_cpu:
         from oracle import flowlab_ref
         if flowlab_ref.available():

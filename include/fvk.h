@@ -491,8 +491,13 @@ int fvk_fl_csr_matches(const fvk_fl_ctx *ctx, int n, const int *ia, const int *j
 void fvk_fl_ctx_destroy(fvk_fl_ctx *ctx);
 /* "sweep_one_launch" 1 (default when every row has at most 8 lower and 8
    upper entries): the factor and direct_solve's sweeps in one launch each,
-   rows waiting for their values; 0: one launch per level.  Both give the
-   same bits.  No counterpart in the reference. */
+   rows waiting for their values; 0: one launch per level.  "sweep_parts" 1
+   (default with one launch): each block of the launch owns ranges of
+   "sweep_part_rows" consecutive rows (default 4096, 256..6400) and walks them
+   level by level, values inside a range passing through LDS; 0: rows taken
+   in level order in chunks.  "sweep_blocks" (0: one per CU): blocks of the
+   one-launch kernels.  All give the same bits.  No counterpart in the
+   reference. */
 int fvk_fl_ctx_set_param(fvk_fl_ctx *ctx, const char *name, double value);
 /* xdmn%x_vc (n_dim x n_dmn), vol, x_fc, norm, area, x_pc, x_nc, l_pn, w_fp
    (per face), ls (n_dim x n_dim x n_dmn); NULL skips */

@@ -57,6 +57,20 @@ struct FlCtx {
   // no row has more than kSweepSlots lower or upper entries
   bool one_launch = true;
   int n_cu = 256;
+  int sweep_blocks = 0;  // blocks of the one-launch kernels (0: one per CU)
+  // partitioned one-launch sweeps (sweep_parts): forward / backward level
+  // order of each row range, for ranges of R1 rows (one component, the
+  // factor) and R3 rows (two or three components)
+  struct Parts {
+    int P = 0, R = 0, W = 0;
+    int *rows = nullptr, *first = nullptr, *cnt = nullptr, *aux = nullptr, *cols = nullptr;
+    int *step = nullptr, *sp = nullptr;
+    SweepTabs tabs() const { return SweepTabs{rows, first, cnt, aux, cols, W}; }
+  };
+  Parts pf, pb;
+  bool parts = false;
+  int part_rows = 4096;  // R: LDS of one block = components x R doubles
+  std::vector<int> h_levf, h_levb, h_kii, h_klo, h_kup;
   int *sweep_int = nullptr;  // tickets: forward, backward, factor; [3] wait-bound error
   // Krylov work: 6 vectors x 3 components, partials, scalars
   double *wk = nullptr, *pc = nullptr, *part = nullptr;
@@ -82,6 +96,16 @@ static int fl_up(FlCtx &c, T **p, const std::vector<T> &v) {
   if (rc) return rc;
   if (!v.empty()) FVK_HIP(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
   return 0;
+}
+
+static void fl_free(FlCtx &c, void *p) {
+  if (!p) return;
+  for (size_t k = 0; k < c.owned.size(); ++k)
+    if (c.owned[k] == p) {
+      hipFree(p);
+      c.owned.erase(c.owned.begin() + (long)k);
+      return;
+    }
 }
 
 static double *wkv(FlCtx &c, int k) { return c.wk + (size_t)k * 3 * c.n; }
@@ -704,10 +728,151 @@ __global__ __launch_bounds__(BS) void k_fl_sweep_sf(int n, const int *__restrict
   sweep_chunks(n, ticket, err, row);
 }
 
+// the same rows for sweep_parts: a position's descriptor holds the row, its
+// first entry (forward: ia; backward: the last entry, walked down), the
+// entry count and (factor) the diagonal entry
+template <bool FWD, int NC>
+struct FlPtSweep {
+  const double *a, *pc, *src;
+  int n;
+  double cc, s[NC];
+  double av[kSweepSlots];
+  __device__ void fetch(const SweepDesc &d, int MW) {
+    constexpr int st = FWD ? 1 : -1;
+#pragma unroll
+    for (int t = 0; t < kSweepSlots; ++t)
+      if (t < MW) av[t] = a[t < d.nl ? d.first + st * t : d.first];
+    cc = pc[d.r];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) s[q] = src[(size_t)q * n + d.r];
+  }
+  __device__ void begin() {
+    if (!FWD) {
+      const double cof = 1.0 / (cc + SMALL);
+#pragma unroll
+      for (int q = 0; q < NC; ++q) s[q] = cof * s[q];
+    }
+  }
+  __device__ void take(int t, const double *w) {
+#pragma unroll
+    for (int q = 0; q < NC; ++q) s[q] = s[q] - av[t] * w[q];
+  }
+  __device__ double result(int q) const { return s[q] * cc; }
+};
+
+struct FlPtPc {
+  const double *a;
+  const int *tpos;
+  double s;
+  double a1[kSweepSlots], a2[kSweepSlots];
+  __device__ void fetch(const SweepDesc &d, int MW) {
+    s = a[d.aux];
+#pragma unroll
+    for (int t = 0; t < kSweepSlots; ++t)
+      if (t < MW) {
+        const int k = t < d.nl ? d.first + t : d.aux;
+        a1[t] = a[k];
+        a2[t] = a[t < d.nl ? tpos[k] : k];
+      }
+  }
+  __device__ void begin() {}
+  __device__ void take(int t, const double *w) { s = s - a1[t] * w[0] * a2[t]; }
+  __device__ double result(int) const { return 1.0 / (s + SMALL); }
+};
+
+template <int MW>
+__global__ __launch_bounds__(BS) void k_fl_pc_pt(int n, int R, int P, SweepTabs T, const int *__restrict__ step,
+                                                 const int *__restrict__ sp, const int *__restrict__ tpos,
+                                                 const double *__restrict__ a, double *pc, int *ticket, int *err) {
+  FlPtPc row;
+  row.a = a; row.tpos = tpos;
+  sweep_parts<1, MW>(true, n, R, P, T, step, sp, pc, ticket, err, row);
+}
+
+template <bool FWD, int NC, int MW>
+__global__ __launch_bounds__(BS) void k_fl_sweep_pt(int n, int R, int P, SweepTabs T, const int *__restrict__ step,
+                                                    const int *__restrict__ sp, const double *__restrict__ a,
+                                                    const double *__restrict__ pc, const double *__restrict__ src,
+                                                    double *out, int *ticket, int *err, const FlKS *ks) {
+  if (fl_all_done(ks, NC)) return;
+  FlPtSweep<FWD, NC> row;
+  row.a = a; row.pc = pc; row.src = src; row.n = n;
+  sweep_parts<NC, MW>(FWD, n, R, P, T, step, sp, out, ticket, err, row);
+}
+
 // ---------------------------------------------------------------- host
+// Ranges of R rows in the sweep's order and, in each, its rows by level
+// (ties by row), with each position's descriptor: the tables of sweep_parts.
+static int fl_parts_build(FlCtx &c, const std::vector<int> &lev, bool fwd, int R, FlCtx::Parts &pt) {
+  const int n = c.n;
+  const std::vector<int> &ia = c.h_ia, &ja = c.h_ja;
+  pt.R = R;
+  pt.P = (n + R - 1) / R;
+  std::vector<int> rows, first(n), cnt(n), aux(n), step, sp;
+  rows.reserve(n);
+  std::vector<std::pair<int, int>> buf;
+  for (int q = 0; q < pt.P; ++q) {
+    const int lo = fwd ? q * R : std::max(0, n - (q + 1) * R);
+    const int hi = fwd ? std::min(n, (q + 1) * R) : n - q * R;
+    buf.clear();
+    for (int r = lo; r < hi; ++r) buf.emplace_back(lev[r], r);
+    std::sort(buf.begin(), buf.end());
+    sp.push_back((int)step.size());
+    for (size_t t = 0; t < buf.size(); ++t) {
+      if (t == 0 || buf[t].first != buf[t - 1].first) step.push_back((int)rows.size());
+      rows.push_back(buf[t].second);
+    }
+    step.push_back((int)rows.size());
+  }
+  sp.push_back((int)step.size());
+  int W = 0;
+  for (int p = 0; p < n; ++p) {
+    const int r = rows[p];
+    first[p] = fwd ? ia[r] : ia[r + 1] - 1;
+    cnt[p] = fwd ? c.h_klo[r] - ia[r] : ia[r + 1] - c.h_kup[r];
+    aux[p] = c.h_kii[r];
+    W = std::max(W, cnt[p]);
+  }
+  if (W > kSweepSlots) { set_error("a row has more lower or upper entries than one launch holds"); return FVK_ERR_ARG; }
+  pt.W = W;
+  std::vector<int> cols((size_t)std::max(W, 1) * n, 0);
+  for (int p = 0; p < n; ++p)
+    for (int t = 0; t < cnt[p]; ++t) cols[(size_t)t * n + p] = ja[fwd ? first[p] + t : first[p] - t];
+  int rc = fl_up(c, &pt.rows, rows);
+  if (!rc) rc = fl_up(c, &pt.first, first);
+  if (!rc) rc = fl_up(c, &pt.cnt, cnt);
+  if (!rc) rc = fl_up(c, &pt.aux, aux);
+  if (!rc) rc = fl_up(c, &pt.cols, cols);
+  if (!rc) rc = fl_up(c, &pt.step, step);
+  if (!rc) rc = fl_up(c, &pt.sp, sp);
+  return rc;
+}
+
+static int fl_parts_setup(FlCtx &c) {
+  for (FlCtx::Parts *pt : {&c.pf, &c.pb}) {
+    for (int *q : {pt->rows, pt->first, pt->cnt, pt->aux, pt->cols, pt->step, pt->sp}) fl_free(c, q);
+    *pt = FlCtx::Parts();
+  }
+  int rc = fl_parts_build(c, c.h_levf, true, c.part_rows, c.pf);
+  if (!rc) rc = fl_parts_build(c, c.h_levb, false, c.part_rows, c.pb);
+  if (rc) return rc;
+  const int bytes = 3 * c.part_rows * 8;
+  if (bytes > 65536) {
+    const void *k[] = {(const void *)k_fl_pc_pt<4>, (const void *)k_fl_pc_pt<8>,
+                       (const void *)k_fl_sweep_pt<true, 1, 4>, (const void *)k_fl_sweep_pt<false, 1, 4>,
+                       (const void *)k_fl_sweep_pt<true, 2, 4>, (const void *)k_fl_sweep_pt<false, 2, 4>,
+                       (const void *)k_fl_sweep_pt<true, 3, 4>, (const void *)k_fl_sweep_pt<false, 3, 4>,
+                       (const void *)k_fl_sweep_pt<true, 1, 8>, (const void *)k_fl_sweep_pt<false, 1, 8>,
+                       (const void *)k_fl_sweep_pt<true, 2, 8>, (const void *)k_fl_sweep_pt<false, 2, 8>,
+                       (const void *)k_fl_sweep_pt<true, 3, 8>, (const void *)k_fl_sweep_pt<false, 3, 8>};
+    for (const void *f : k) FVK_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+  }
+  return 0;
+}
+
 static int fl_levels(const std::vector<int> &ia, const std::vector<int> &ja, int n, bool fwd,
-                     std::vector<int> &ptr, std::vector<int> &rows) {
-  std::vector<int> lev(n, 0);
+                     std::vector<int> &ptr, std::vector<int> &rows, std::vector<int> &lev) {
+  lev.assign(n, 0);
   int nl = 0;
   for (int s = 0; s < n; ++s) {
     const int i = fwd ? s : n - 1 - s;
@@ -728,10 +893,28 @@ static int fl_levels(const std::vector<int> &ia, const std::vector<int> &ja, int
   return 0;
 }
 
-static int fl_sweep_grid(const FlCtx &c) { return std::max(1, std::min((c.n + BS - 1) / BS, c.n_cu)); }
+static int fl_sweep_grid(const FlCtx &c) {
+  return std::max(1, std::min((c.n + BS - 1) / BS, c.sweep_blocks > 0 ? c.sweep_blocks : c.n_cu));
+}
 static int fl_mark_grid(size_t cnt) { return (int)std::max<size_t>(1, std::min<size_t>((cnt + BS - 1) / BS, 4096)); }
 
+static int fl_pt_grid(const FlCtx &c, const FlCtx::Parts &pt) {
+  return std::max(1, std::min(pt.P, c.sweep_blocks > 0 ? c.sweep_blocks : c.n_cu));
+}
+
 static int fl_precond(FlCtx &c, const double *a) {
+  if (c.one_launch && c.parts) {
+    const FlCtx::Parts &P = c.pf;
+    k_fl_mark<<<fl_mark_grid(c.n), BS, 0, c.stream>>>(c.n, c.pc, c.sweep_int + 2, nullptr, 1);
+    if (P.W <= 4)
+      k_fl_pc_pt<4><<<fl_pt_grid(c, P), BS, (size_t)P.R * 8, c.stream>>>(c.n, P.R, P.P, P.tabs(), P.step, P.sp, c.tpos,
+                                                                        a, c.pc, c.sweep_int + 2, c.sweep_int + 3);
+    else
+      k_fl_pc_pt<8><<<fl_pt_grid(c, P), BS, (size_t)P.R * 8, c.stream>>>(c.n, P.R, P.P, P.tabs(), P.step, P.sp, c.tpos,
+                                                                        a, c.pc, c.sweep_int + 2, c.sweep_int + 3);
+    FVK_HIP(hipGetLastError());
+    return 0;
+  }
   if (c.one_launch) {
     k_fl_mark<<<fl_mark_grid(c.n), BS, 0, c.stream>>>(c.n, c.pc, c.sweep_int + 2, nullptr, 1);
     k_fl_pc_sf<<<fl_sweep_grid(c), BS, 0, c.stream>>>(c.n, c.lf_rows, c.ia, c.ja, c.kii, c.klo, c.tpos, a, c.pc,
@@ -750,6 +933,29 @@ static int fl_precond(FlCtx &c, const double *a) {
 
 // x = M^-1 b (direct_solve), t: scratch
 static int fl_direct(FlCtx &c, int ncmp, const double *a, const double *b, double *t, double *x, const FlKS *ks) {
+  if (c.one_launch && c.parts) {
+    const size_t cnt = (size_t)ncmp * c.n;
+    const FlCtx::Parts &F = c.pf, &B = c.pb;
+    const size_t lds = (size_t)ncmp * c.part_rows * 8;
+    k_fl_mark<<<fl_mark_grid(cnt), BS, 0, c.stream>>>(cnt, t, c.sweep_int, ks, ncmp);
+#define SW(D, C, MW, X, S, O, TK) k_fl_sweep_pt<D, C, MW><<<fl_pt_grid(c, X), BS, lds, c.stream>>>(c.n, X.R, X.P, \
+    X.tabs(), X.step, X.sp, a, c.pc, S, O, TK, c.sweep_int + 3, ks)
+#define SWC(D, X, S, O, TK)                                                              \
+    if (X.W <= 4) {                                                                      \
+      if (ncmp == 1) SW(D, 1, 4, X, S, O, TK); else if (ncmp == 2) SW(D, 2, 4, X, S, O, TK); \
+      else SW(D, 3, 4, X, S, O, TK);                                                     \
+    } else {                                                                             \
+      if (ncmp == 1) SW(D, 1, 8, X, S, O, TK); else if (ncmp == 2) SW(D, 2, 8, X, S, O, TK); \
+      else SW(D, 3, 8, X, S, O, TK);                                                     \
+    }
+    SWC(true, F, b, t, c.sweep_int)
+    k_fl_mark<<<fl_mark_grid(cnt), BS, 0, c.stream>>>(cnt, x, c.sweep_int + 1, ks, ncmp);
+    SWC(false, B, t, x, c.sweep_int + 1)
+#undef SWC
+#undef SW
+    FVK_HIP(hipGetLastError());
+    return 0;
+  }
   if (c.one_launch) {
     const size_t cnt = (size_t)ncmp * c.n;
     const int G = fl_sweep_grid(c);
@@ -1220,6 +1426,9 @@ static int fl_csr_setup(FlCtx &c, int n, const int *ngp, const int *ngc) {
   c.one_launch = mx <= kSweepSlots;
   c.h_ia = ia;
   c.h_ja = ja;
+  c.h_kii = kii;
+  c.h_klo = klo;
+  c.h_kup = kup;
   int rc = 0;
 #define A(ptr, cnt_) if (!rc) rc = fl_alloc(c, &ptr, cnt_)
 #define U(ptr, vec) if (!rc) rc = fl_up(c, &ptr, vec)
@@ -1231,10 +1440,14 @@ static int fl_csr_setup(FlCtx &c, int n, const int *ngp, const int *ngc) {
   if (rc) return rc;
   FVK_HIP(hipDeviceGetAttribute(&c.n_cu, hipDeviceAttributeMultiprocessorCount, c.device));
   std::vector<int> lrows;
-  fl_levels(ia, ja, n, true, c.lf_ptr, lrows);
+  fl_levels(ia, ja, n, true, c.lf_ptr, lrows, c.h_levf);
   if ((rc = fl_up(c, &c.lf_rows, lrows))) return rc;
-  fl_levels(ia, ja, n, false, c.lb_ptr, lrows);
+  fl_levels(ia, ja, n, false, c.lb_ptr, lrows, c.h_levb);
   if ((rc = fl_up(c, &c.lb_rows, lrows))) return rc;
+  if (c.one_launch) {
+    if ((rc = fl_parts_setup(c))) return rc;
+    c.parts = true;
+  }
   return 0;
 }
 
@@ -1723,6 +1936,21 @@ int fvk_fl_ctx_set_param(fvk_fl_ctx *cc, const char *name, double value) {
       if (mx > kSweepSlots) { set_error("a row has more lower or upper entries than one launch holds"); return FVK_ERR_ARG; }
     }
     c.one_launch = value != 0;
+    return FVK_OK;
+  }
+  if (!std::strcmp(name, "sweep_parts") && (value == 0 || value == 1)) {
+    if (value == 1 && !c.pf.rows) { set_error("a row has more lower or upper entries than one launch holds"); return FVK_ERR_ARG; }
+    c.parts = value != 0;
+    return FVK_OK;
+  }
+  if (!std::strcmp(name, "sweep_part_rows") && value >= 256 && value <= 6400 && value == (int)value) {
+    if (!c.pf.rows) { set_error("a row has more lower or upper entries than one launch holds"); return FVK_ERR_ARG; }
+    FVK_HIP(hipStreamSynchronize(c.stream));
+    c.part_rows = (int)value;
+    return fl_parts_setup(c);
+  }
+  if (!std::strcmp(name, "sweep_blocks") && value >= 0 && value <= 65536 && value == (int)value) {
+    c.sweep_blocks = (int)value;
     return FVK_OK;
   }
   set_error(std::string("unknown parameter or value out of range: ") + name);

@@ -66,4 +66,212 @@ __device__ __forceinline__ void sweep_chunks(int n, int *ticket, int *err, Row &
 // at most this many lower (upper) entries per row are kept in registers
 constexpr int kSweepSlots = 8;
 
+// Partitioned sweep.  The rows are cut into P ranges of R consecutive rows
+// (forward: range q = [qR, (q+1)R); backward: counted from the end); a block
+// takes the ranges in ticket order (so a range waits only for ranges taken
+// earlier) and walks its rows level by level in the sweep's level order, a
+// barrier between levels.  Values of rows inside the range pass through LDS
+// (NC x R doubles); values from other ranges are read from `out` with
+// agent-scope loads until they are no longer the mark.  Range q's levels
+// are positions [step[k], step[k+1]) for k in [sp[q], sp[q+1]-1).
+//
+// Each position p has a descriptor: its row, the row's first entry, entry
+// count, one more row-specific index, and the entries' columns (slot-major,
+// W slots of n, padding 0).  The first BS positions of each level are
+// software-pipelined: the descriptors two levels ahead, the row's
+// coefficients and the values from other ranges one level ahead, all loads
+// issued without branches (a lane past the level's end loads the level's
+// last position and is not run), so a level waits for memory only when a
+// value from another range has not yet arrived.
+//
+// Row interface: fetch(d, MW) loads the row's coefficients (entries t < MW,
+// entries t >= d.nl from a valid address, unused); begin() does the arithmetic on them that precedes the
+// entries; take(t, v) adds entry t with the NC values v[] (called in entry
+// order, t < d.nl); result(c) is component c of the row's value.
+struct SweepTabs {
+  const int *rows, *first, *cnt, *aux, *cols;
+  int W;
+};
+
+struct SweepDesc {
+  int r, first, nl, aux;
+  int j[kSweepSlots];
+};
+
+// read-only tables with a uniform index: scalar loads, which do not wait
+// behind the vector loads in flight
+typedef const __attribute__((address_space(4))) int *const_int_p;
+__device__ __forceinline__ int ld_uniform(const int *p, int i) { return ((const_int_p)p)[i]; }
+
+// descriptor of position p of the level [pb, pe) (p >= pe: a copy of the
+// level's last position with no entries, not run); MW >= W entry slots are
+// loaded, slots t >= W from slot W-1 (unused)
+template <int MW>
+__device__ __forceinline__ void sweep_desc(const SweepTabs &T, int n, int p, int pe, SweepDesc &d) {
+  const bool live = p < pe;
+  const int q = live ? p : pe - 1;
+  d.r = T.rows[q];
+  d.first = T.first[q];
+  const int c = T.cnt[q];
+  d.nl = live ? c : -1;
+  d.aux = T.aux[q];
+#pragma unroll
+  for (int t = 0; t < MW; ++t) d.j[t] = T.cols[(size_t)(t < T.W ? t : T.W - 1) * n + q];
+}
+
+// values of entries outside [lo, hi): one agent-scope load per entry slot
+// t < MW (slots that need none load the row's own value, unused)
+template <int NC, int MW>
+__device__ __forceinline__ void sweep_ext(const SweepDesc &d, int lo, int hi, int n, const double *out,
+                                          unsigned long long (&v)[kSweepSlots][NC]) {
+#pragma unroll
+  for (int t = 0; t < MW; ++t) {
+    const bool ext = t < d.nl && (d.j[t] < lo || d.j[t] >= hi);
+    const int jj = ext ? d.j[t] : d.r;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[t][c] = ld_agent(out + (size_t)c * n + jj);
+  }
+}
+
+// run one row (d.nl >= 0): values inside the range from LDS, the others from
+// v[] (polled again while they are the mark); false if the wait bound ran out
+template <int NC, int MW, class Row>
+__device__ __forceinline__ bool sweep_row(const SweepDesc &d, Row &row,
+                                          unsigned long long (&v)[kSweepSlots][NC], int lo, int hi, int R, int n,
+                                          double *out, double *lv, int *err) {
+  row.begin();
+  unsigned need = 0;
+#pragma unroll
+  for (int t = 0; t < MW; ++t) {
+    if (t >= d.nl) continue;
+    const int j = d.j[t];
+    if (j >= lo && j < hi) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) v[t][c] = (unsigned long long)__double_as_longlong(lv[c * R + (j - lo)]);
+    } else {
+      bool ready = true;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) ready = ready && v[t][c] != kSweepMark;
+      if (!ready) need |= 1u << t;
+    }
+  }
+  unsigned spin = 0;
+  while (need) {
+    if (++spin > kSweepSpin ||
+        ((spin & 255u) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+      __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+    for (int t = 0; t < MW; ++t)
+      if ((need >> t) & 1u)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) v[t][c] = ld_agent(out + (size_t)c * n + d.j[t]);
+#pragma unroll
+    for (int t = 0; t < MW; ++t) {
+      if (!((need >> t) & 1u)) continue;
+      bool ready = true;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) ready = ready && v[t][c] != kSweepMark;
+      if (ready) need &= ~(1u << t);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < MW; ++t)
+    if (t < d.nl) {
+      double w[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) w[c] = __longlong_as_double((long long)v[t][c]);
+      row.take(t, w);
+    }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const double x = row.result(c);
+    lv[c * R + (d.r - lo)] = x;
+    st_agent(out + (size_t)c * n + d.r, x);
+  }
+  return true;
+}
+
+template <int NC, class Row>
+struct SweepStage {
+  SweepDesc d;
+  Row row;
+  unsigned long long v[kSweepSlots][NC];
+};
+
+template <int NC, int MW, class Row>
+__device__ __forceinline__ void sweep_parts(bool fwd, int n, int R, int P, const SweepTabs &T,
+                                            const int *__restrict__ step, const int *__restrict__ sp, double *out,
+                                            int *ticket, int *err, const Row &proto) {
+  extern __shared__ double lv[];
+  __shared__ int s_q, s_abort;
+  if (threadIdx.x == 0) s_abort = 0;
+  const int tid = threadIdx.x;
+  for (;;) {
+    if (tid == 0) s_q = atomicAdd(ticket, 1);
+    __syncthreads();
+    const int q = s_q;
+    if (q >= P || s_abort) return;
+    const int lo = fwd ? q * R : (n - (q + 1) * R > 0 ? n - (q + 1) * R : 0);
+    const int hi = fwd ? (n < (q + 1) * R ? n : (q + 1) * R) : n - q * R;
+    const int k0 = ld_uniform(sp, q), k1 = ld_uniform(sp, q + 1) - 1;
+    // descriptors of level k (past the last level: dead copies of it)
+    auto desc = [&](int k, SweepDesc &d) {
+      const bool in = k < k1;
+      const int kk = in ? k : k1 - 1;
+      const int pb = ld_uniform(step, kk), pe = ld_uniform(step, kk + 1);
+      sweep_desc<MW>(T, n, in ? pb + tid : pe, pe, d);
+    };
+    auto stage = [&](const SweepDesc &d, SweepStage<NC, Row> &S) {
+      S.d = d;
+      S.row.fetch(S.d, MW);
+      sweep_ext<NC, MW>(S.d, lo, hi, n, out, S.v);
+    };
+    // positions of level k beyond the first BS, not pipelined
+    auto rest = [&](int k) -> bool {
+      const int e = ld_uniform(step, k + 1);
+      bool ok = true;
+      for (int p = ld_uniform(step, k) + tid + BS; ok && p < e; p += BS) {
+        SweepDesc d;
+        SweepStage<NC, Row> Z;
+        Z.row = proto;
+        sweep_desc<MW>(T, n, p, e, d);
+        stage(d, Z);
+        ok = sweep_row<NC, MW>(Z.d, Z.row, Z.v, lo, hi, R, n, out, lv, err);
+      }
+      return ok;
+    };
+    SweepStage<NC, Row> X, Y;
+    X.row = proto;
+    Y.row = proto;
+    SweepDesc D;
+    desc(k0, D);
+    stage(D, X);
+    desc(k0 + 1, D);
+    bool ok = true;
+    for (int k = k0; k < k1; k += 2) {
+      // level k in X; level k+1 into Y; descriptors of level k+2
+      stage(D, Y);
+      desc(k + 2, D);
+      if (ok && X.d.nl >= 0) ok = sweep_row<NC, MW>(X.d, X.row, X.v, lo, hi, R, n, out, lv, err);
+      if (ok) ok = rest(k);
+      if (!ok) s_abort = 1;
+      __syncthreads();
+      if (s_abort) return;
+      if (k + 1 >= k1) break;
+      // level k+1 in Y; level k+2 into X; descriptors of level k+3
+      stage(D, X);
+      desc(k + 3, D);
+      if (ok && Y.d.nl >= 0) ok = sweep_row<NC, MW>(Y.d, Y.row, Y.v, lo, hi, R, n, out, lv, err);
+      if (ok) ok = rest(k + 1);
+      if (!ok) s_abort = 1;
+      __syncthreads();
+      if (s_abort) return;
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace fvk

@@ -186,30 +186,52 @@ def test_outer_iterations_match_reference(name, tmp_path):
     ctx.destroy()
 
 
-def test_one_launch_sweeps_bitwise_on_a_box():
-    """The factor and direct_solve's sweeps in one launch each against one
-    launch per level: the same bits after three outer iterations on a 40^3
-    jittered cavity (64,000 cells, 118 levels per sweep)."""
+SWEEP_MODES = [
+    ("one launch per level", {"sweep_one_launch": 0}),
+    ("rows in level order, chunks", {"sweep_parts": 0}),
+    ("ranges of 4096 rows", {}),
+    ("ranges of 256 rows, 7 blocks", {"sweep_part_rows": 256, "sweep_blocks": 7}),
+]
+
+
+@pytest.mark.parametrize("mesh", ["box40", "fl_prism3"])
+def test_one_launch_sweeps_bitwise(mesh, tmp_path):
+    """The factor and direct_solve's sweeps in one launch each (rows taken in
+    level order in chunks; or blocks owning ranges of rows, walked level by
+    level with values inside a range through LDS, including blocks that take
+    many ranges) against one launch per level: the same bits after three
+    outer iterations.  A 40^3 jittered cavity (64,000 cells, 118 levels per
+    sweep, three lower entries a row) and the prism mesh (five)."""
     require_lib()
     from programs_amd import flowlab as fl
     from programs_amd import meshgen
-    m = fl.FlMesh.from_records(3, *meshgen.box_records(40, jitter=0.1))
+    if mesh == "box40":
+        m = fl.FlMesh.from_records(3, *meshgen.box_records(40, jitter=0.1))
+        bnd = ("Wall", 1, (1, 0, 0, 1))
+    else:
+        rec, spec, path = load(mesh, tmp_path)
+        m = fl.FlMesh.read_gmsh(path, spec["n_dim"])
+        bnd = None
     out = []
-    for mode in (0, 1):
+    for label, params in SWEEP_MODES:
         ctx = fl.FlCtx(m)
-        ctx.set_param("sweep_one_launch", mode)
-        say("box 40^3, one launch %d" % mode)
-        ub = fl.boundary_values(m, ctx.geometry()["norm"], "Wall", 1, (1, 0, 0, 1))
+        for k, v in params.items():
+            ctx.set_param(k, v)
+        say("%s: %s" % (mesh, label))
         fw = fl.FlFlow(ctx, 1.0, 0.01, 0.7, 0.3, 0.5)
-        fw.set(vel_bnd=ub)
+        if bnd:
+            fw.set(vel_bnd=fl.boundary_values(m, ctx.geometry()["norm"], *bnd))
+        else:
+            fw.set(vel_bnd=rec["vel_bnd0"])
         res0, gl = fw.run(3, 0.0)
-        out.append((res0, fw.get()))
+        out.append((label, res0, fw.get()))
         fw.destroy()
         ctx.destroy()
-    (r0, s0), (r1, s1) = out
-    bitwise(r1, r0, "residual norms")
-    for k in ("vel", "pres", "pp", "spd"):
-        bitwise(s1[k], s0[k], k)
+    _, r0, s0 = out[0]
+    for label, r1, s1 in out[1:]:
+        bitwise(r1, r0, label + ": residual norms")
+        for k in ("vel", "pres", "pp", "spd"):
+            bitwise(s1[k], s0[k], label + ": " + k)
 
 
 # ---------------------------------------------------------------- Fortran drop-in
