@@ -285,10 +285,11 @@ def simple_leg(fvk, args, model):
         fl.destroy()
     # the reference's shipped configuration (cfd_setup.f90:528,581,662,742):
     # SIMPLER pressure, DILU momentum and DIC pressure preconditioners, urf_p
-    # 0.1; factor and triangular sweeps in one launch each (rows wait for the
-    # values they need), and for comparison one launch per level
-    # one application of the DIC / DILU factor (both sweeps), per-level
-    # launches against one launch per sweep at 1, 2 and 4 resident blocks per CU
+    # 0.1; factor and triangular sweeps in one launch each (the default:
+    # blocks owning ranges of rows, values inside a range through LDS), and
+    # for comparison one launch with rows taken in level order in chunks and
+    # one launch per level.  First one application of the DIC / DILU factor
+    # (both sweeps) in the three forms.
     if "shipped" in legs_on:
         fvk.sys_random(c, 0x5EED0003, 1.0)
         sw = {"levels": c.levels()}
@@ -297,33 +298,35 @@ def simple_leg(fvk, args, model):
             fvk.set_param(c, "sweep_one_launch", 0)
             sw[key] = {"per_level_launches_us": fvk.bench_sweep(c, imat, nvc, 5) * 1e6}
             fvk.set_param(c, "sweep_one_launch", 1)
-            for bpc in (1, 2, 4):
-                fvk.set_param(c, "sweep_blocks_per_cu", bpc)
-                sw[key]["one_launch_bpc%d_us" % bpc] = fvk.bench_sweep(c, imat, nvc, 20) * 1e6
-        fvk.set_param(c, "sweep_blocks_per_cu", 2)
+            fvk.set_param(c, "sweep_parts", 0)
+            sw[key]["one_launch_chunks_us"] = fvk.bench_sweep(c, imat, nvc, 20) * 1e6
+            fvk.set_param(c, "sweep_parts", 1)
+            sw[key]["one_launch_ranges_us"] = fvk.bench_sweep(c, imat, nvc, 20) * 1e6
         out["sweep_apply"] = sw
         log("SIMPLE %d^3: factor application %s" % (n, json.dumps(sw)))
-    shipped = [(m_, 1) for m_ in modes] + [("eager", 0)] if "shipped" in legs_on else []
-    for mode, one in shipped:
-        log("SIMPLE %d^3: shipped configuration, %s, %s" % (n, mode, "one launch per sweep" if one
-                                                            else "one launch per level"))
-        fvk.set_param(c, "sweep_one_launch", one)
+    shipped = ([(m_, "ranges") for m_ in modes] + [("eager", "chunks"), ("eager", "per_level")]
+               if "shipped" in legs_on else [])
+    for mode, form in shipped:
+        log("SIMPLE %d^3: shipped configuration, %s, sweeps: %s" % (n, mode, form))
+        fvk.set_param(c, "sweep_one_launch", int(form != "per_level"))
+        fvk.set_param(c, "sweep_parts", int(form == "ranges"))
         cfg = fvk.flow_cfg()
         fl = fvk.Flow(c, cfg)
         fl.set_bnd_u(31, [1.0, 0.0, 0.0])
         run = fl.run if mode == "eager" else fl.run_graph
         run(1, 2)
         c.sync()
-        its = args.simple_its if one else max(2, args.simple_its // 4)
+        its = args.simple_its
         t0 = time.perf_counter()
         h = run(3, its)
         c.sync()
         dt = (time.perf_counter() - t0) / its
-        key = "shipped_" + mode + ("" if one else "_per_level_launches")
+        key = "shipped_" + mode + {"ranges": "", "chunks": "_chunks", "per_level": "_per_level_launches"}[form]
         out[key] = dict(outer_its_per_s=1.0 / dt, ms_per_outer=dt * 1e3, outer_iterations=its,
                         mx_res_n_last=h[-1]["mx_res_n"], levels=c.levels())
         fl.destroy()
     fvk.set_param(c, "sweep_one_launch", 1)
+    fvk.set_param(c, "sweep_parts", 1)
     # C2: outer iterations 1..simple_tol_its with every pressure-correction
     # system solved to the FP64 relative L1 residual tol from x = 0, plain
     # FP64 PCG against the learned FP32-inner solve, replayed from HIP

@@ -348,6 +348,7 @@ static int ctx_build(Ctx &c, const Topo &t, int device, int flags) {
   FVK_HIP(hipDeviceGetAttribute(&c.n_cu, hipDeviceAttributeMultiprocessorCount, c.device));
   if (Ws > 8) c.sweep_one_launch = false;  // slot masks hold 8 slots
   if ((rc = sweep_masks_build(c))) return rc;
+  if ((rc = sweep_parts_build(c))) return rc;
   if ((rc = dalloc(c, &c.wk, (size_t)8 * wk_ld(n)))) return rc;
   if ((rc = dalloc(c, &c.ks, 3))) return rc;
   if ((rc = dalloc(c, &c.ksr, 6))) return rc;
@@ -986,8 +987,20 @@ int fvk_ctx_set_param(fvk_ctx *cc, const char *name, double value) {
   else if (!std::strcmp(name, "dc_eta_margin") && value > 0 && value <= 1) X.eta_margin = value;
   else if (!std::strcmp(name, "dc_replace_every") && value >= 0 && value <= 1e6 && value == (int)value)
     X.replace_every = (int)value;
-  else if (!std::strcmp(name, "sweep_one_launch") && (value == 0 || value == 1))
+  else if (!std::strcmp(name, "sweep_one_launch") && (value == 0 || value == 1)) {
+    const bool was = cc->c.sweep_one_launch;
     cc->c.sweep_one_launch = value != 0 && cc->c.Ws <= 8;
+    if (cc->c.sweep_one_launch && !was) {
+      const int rc = sweep_parts_build(cc->c);
+      if (rc) return rc;
+    }
+  } else if (!std::strcmp(name, "sweep_parts") && (value == 0 || value == 1))
+    cc->c.sweep_parts = value != 0;
+  else if (!std::strcmp(name, "sweep_part_rows") && value >= 256 && value <= 6400 && value == (int)value) {
+    cc->c.sweep_part_rows = (int)value;
+    const int rc = sweep_parts_build(cc->c);
+    if (rc) return rc;
+  }
   else if (!std::strcmp(name, "sweep_blocks_per_cu") && value >= 1 && value <= 8 && value == (int)value)
     cc->c.sweep_blocks_per_cu = (int)value;
   else { set_error(std::string("unknown parameter or value out of range: ") + name); return FVK_ERR_ARG; }

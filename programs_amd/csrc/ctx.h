@@ -99,6 +99,17 @@ struct Ctx {
   double *sweep_t = nullptr;      // forward sweep output (3 x n)
   int *sweep_int = nullptr;       // tickets: forward, backward, factor; [3] = wait-bound error
   unsigned char *sweep_lmask = nullptr, *sweep_umask = nullptr;  // lower / upper slots of each row
+  // partitioned one-launch sweeps (sweep_parts in sweep.h): ranges of
+  // sweep_part_rows rows consecutive in the reference numbering, each walked
+  // level by level by one block
+  struct SweepParts {
+    int P = 0, R = 0, W = 0;
+    int *rows = nullptr, *first = nullptr, *cnt = nullptr, *aux = nullptr, *lrow = nullptr, *cols = nullptr,
+        *lcol = nullptr, *step = nullptr, *sp = nullptr;
+  };
+  SweepParts spf, spb;
+  bool sweep_parts = true;
+  int sweep_part_rows = 4096;
   // Krylov workspace (3 components)
   double *wk = nullptr;           // 8 vectors x 3 components x n
   KS *ks = nullptr;               // 3 components: state after the last kernel / solve
@@ -225,6 +236,9 @@ int precond_apply(Ctx &c, int imat, int nvc, const double *b, double *x, const K
 // FVK_ERR_HIP if a one-launch factorisation or sweep exceeded its wait bound
 int sweep_status(Ctx &c);
 int sweep_masks_build(Ctx &c);
+// the tables of the partitioned sweeps (after sweep_masks_build); frees the
+// previous ones
+int sweep_parts_build(Ctx &c);
 int spmv(Ctx &c, int nvc, const double *x, double *y);
 // a[0..n) = v and b = a as kernels (capturable in a HIP graph)
 int fill(Ctx &c, double *a, size_t n, double v);

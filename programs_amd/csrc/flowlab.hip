@@ -63,9 +63,10 @@ struct FlCtx {
   // factor) and R3 rows (two or three components)
   struct Parts {
     int P = 0, R = 0, W = 0;
-    int *rows = nullptr, *first = nullptr, *cnt = nullptr, *aux = nullptr, *cols = nullptr;
+    int *rows = nullptr, *first = nullptr, *cnt = nullptr, *aux = nullptr, *lrow = nullptr, *cols = nullptr,
+        *lcol = nullptr;
     int *step = nullptr, *sp = nullptr;
-    SweepTabs tabs() const { return SweepTabs{rows, first, cnt, aux, cols, W}; }
+    SweepTabs tabs() const { return SweepTabs{rows, first, cnt, aux, lrow, cols, lcol, W}; }
   };
   Parts pf, pb;
   bool parts = false;
@@ -758,6 +759,7 @@ struct FlPtSweep {
     for (int q = 0; q < NC; ++q) s[q] = s[q] - av[t] * w[q];
   }
   __device__ double result(int q) const { return s[q] * cc; }
+  __device__ void done(const SweepDesc &) {}
 };
 
 struct FlPtPc {
@@ -778,6 +780,7 @@ struct FlPtPc {
   __device__ void begin() {}
   __device__ void take(int t, const double *w) { s = s - a1[t] * w[0] * a2[t]; }
   __device__ double result(int) const { return 1.0 / (s + SMALL); }
+  __device__ void done(const SweepDesc &) {}
 };
 
 template <int MW>
@@ -786,7 +789,7 @@ __global__ __launch_bounds__(BS) void k_fl_pc_pt(int n, int R, int P, SweepTabs 
                                                  const double *__restrict__ a, double *pc, int *ticket, int *err) {
   FlPtPc row;
   row.a = a; row.tpos = tpos;
-  sweep_parts<1, MW>(true, n, R, P, T, step, sp, pc, ticket, err, row);
+  sweep_parts<1, MW>(n, R, P, T, step, sp, pc, ticket, err, row);
 }
 
 template <bool FWD, int NC, int MW>
@@ -797,7 +800,7 @@ __global__ __launch_bounds__(BS) void k_fl_sweep_pt(int n, int R, int P, SweepTa
   if (fl_all_done(ks, NC)) return;
   FlPtSweep<FWD, NC> row;
   row.a = a; row.pc = pc; row.src = src; row.n = n;
-  sweep_parts<NC, MW>(FWD, n, R, P, T, step, sp, out, ticket, err, row);
+  sweep_parts<NC, MW>(n, R, P, T, step, sp, out, ticket, err, row);
 }
 
 // ---------------------------------------------------------------- host
@@ -808,7 +811,7 @@ static int fl_parts_build(FlCtx &c, const std::vector<int> &lev, bool fwd, int R
   const std::vector<int> &ia = c.h_ia, &ja = c.h_ja;
   pt.R = R;
   pt.P = (n + R - 1) / R;
-  std::vector<int> rows, first(n), cnt(n), aux(n), step, sp;
+  std::vector<int> rows, first(n), cnt(n), aux(n), lrow(n), step, sp;
   rows.reserve(n);
   std::vector<std::pair<int, int>> buf;
   for (int q = 0; q < pt.P; ++q) {
@@ -825,7 +828,7 @@ static int fl_parts_build(FlCtx &c, const std::vector<int> &lev, bool fwd, int R
     step.push_back((int)rows.size());
   }
   sp.push_back((int)step.size());
-  int W = 0;
+  int W = 1;
   for (int p = 0; p < n; ++p) {
     const int r = rows[p];
     first[p] = fwd ? ia[r] : ia[r + 1] - 1;
@@ -835,14 +838,25 @@ static int fl_parts_build(FlCtx &c, const std::vector<int> &lev, bool fwd, int R
   }
   if (W > kSweepSlots) { set_error("a row has more lower or upper entries than one launch holds"); return FVK_ERR_ARG; }
   pt.W = W;
-  std::vector<int> cols((size_t)std::max(W, 1) * n, 0);
-  for (int p = 0; p < n; ++p)
-    for (int t = 0; t < cnt[p]; ++t) cols[(size_t)t * n + p] = ja[fwd ? first[p] + t : first[p] - t];
+  // a range's rows: forward [qR, (q+1)R), backward counted from the end
+  auto range_lo = [&](int r) { return fwd ? r / R * R : std::max(0, n - (n - 1 - r) / R * R - R); };
+  std::vector<int> cols((size_t)W * n, 0), lcol((size_t)W * n, -1);
+  for (int p = 0; p < n; ++p) {
+    const int r = rows[p], lo = range_lo(r);
+    lrow[p] = r - lo;
+    for (int t = 0; t < cnt[p]; ++t) {
+      const int j = ja[fwd ? first[p] + t : first[p] - t];
+      cols[(size_t)t * n + p] = j;
+      if (range_lo(j) == lo) lcol[(size_t)t * n + p] = j - lo;
+    }
+  }
   int rc = fl_up(c, &pt.rows, rows);
   if (!rc) rc = fl_up(c, &pt.first, first);
   if (!rc) rc = fl_up(c, &pt.cnt, cnt);
   if (!rc) rc = fl_up(c, &pt.aux, aux);
+  if (!rc) rc = fl_up(c, &pt.lrow, lrow);
   if (!rc) rc = fl_up(c, &pt.cols, cols);
+  if (!rc) rc = fl_up(c, &pt.lcol, lcol);
   if (!rc) rc = fl_up(c, &pt.step, step);
   if (!rc) rc = fl_up(c, &pt.sp, sp);
   return rc;
@@ -850,7 +864,8 @@ static int fl_parts_build(FlCtx &c, const std::vector<int> &lev, bool fwd, int R
 
 static int fl_parts_setup(FlCtx &c) {
   for (FlCtx::Parts *pt : {&c.pf, &c.pb}) {
-    for (int *q : {pt->rows, pt->first, pt->cnt, pt->aux, pt->cols, pt->step, pt->sp}) fl_free(c, q);
+    for (int *q : {pt->rows, pt->first, pt->cnt, pt->aux, pt->lrow, pt->cols, pt->lcol, pt->step, pt->sp})
+      fl_free(c, q);
     *pt = FlCtx::Parts();
   }
   int rc = fl_parts_build(c, c.h_levf, true, c.part_rows, c.pf);

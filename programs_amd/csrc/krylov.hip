@@ -323,6 +323,102 @@ __global__ __launch_bounds__(BS) void k_sweep_bwd_sf(DMesh M, const int *__restr
   sweep_run<false, NVC>(M, order, umask, spv, c, cr, t, x, ticket, err);
 }
 
+// The same factor and sweeps for sweep_parts: a position's descriptor holds
+// the row, its slot mask (first), the number of lower (upper) slots and the
+// columns of those slots in slot order.
+__device__ __forceinline__ void slots_of(unsigned m, int (&sl)[kSweepSlots]) {
+#pragma unroll
+  for (int t = 0; t < kSweepSlots; ++t) {
+    sl[t] = m ? __builtin_ctz(m) : 0;
+    m &= m - 1u;
+  }
+}
+
+struct PtPrecond {
+  const int *spf;
+  size_t ld;
+  int imat;
+  const double *diag, *ngp, *ngn;
+  double *cr;
+  double d;
+  double a1[kSweepSlots], a2[kSweepSlots];
+  __device__ void fetch(const SweepDesc &q, int MW) {
+    int sl[kSweepSlots];
+    slots_of((unsigned)q.first, sl);
+    d = diag[q.r];
+#pragma unroll
+    for (int t = 0; t < kSweepSlots; ++t)
+      if (t < MW) {
+        const int f0 = spf[(size_t)sl[t] * ld + q.r];
+        const int f = t < q.nl ? f0 : 0;
+        a1[t] = imat == 1 ? ngn[f] : ngp[f];
+        a2[t] = ngn[f];
+      }
+  }
+  __device__ void begin() {}
+  __device__ void take(int t, const double *w) {
+    if (imat == 1) d = d - a1[t] * a2[t] * w[0];
+    else d = d - a1[t] * w[0] * a2[t];
+  }
+  __device__ double result(int) const { return 1.0 / (d + SMALL); }
+  __device__ void done(const SweepDesc &q) { cr[q.r] = d; }
+};
+
+template <bool FWD, int NVC>
+struct PtSweep {
+  const double *spv, *c, *cr, *src;
+  size_t ld;
+  int n;
+  double cc, crr, acc[NVC];
+  double a[kSweepSlots];
+  __device__ void fetch(const SweepDesc &q, int MW) {
+    int sl[kSweepSlots];
+    slots_of((unsigned)q.first, sl);
+#pragma unroll
+    for (int t = 0; t < kSweepSlots; ++t)
+      if (t < MW) a[t] = spv[(size_t)sl[t] * ld + q.r];
+    cc = c[q.r];
+    if (!FWD) crr = cr[q.r];
+#pragma unroll
+    for (int v = 0; v < NVC; ++v) acc[v] = src[v * n + q.r];
+  }
+  __device__ void begin() {
+    if (!FWD) {
+#pragma unroll
+      for (int v = 0; v < NVC; ++v) acc[v] = acc[v] * crr;
+    }
+  }
+  __device__ void take(int t, const double *w) {
+#pragma unroll
+    for (int v = 0; v < NVC; ++v) acc[v] = acc[v] - a[t] * w[v];
+  }
+  __device__ double result(int v) const { return acc[v] * cc; }
+  __device__ void done(const SweepDesc &) {}
+};
+
+template <int MW>
+__global__ __launch_bounds__(BS) void k_precond_pt(int n, int R, int P, SweepTabs T, const int *__restrict__ step,
+                                                   const int *__restrict__ sp, DMesh M, int imat,
+                                                   const double *__restrict__ diag, const double *__restrict__ ngp,
+                                                   const double *__restrict__ ngn, double *c, double *__restrict__ cr,
+                                                   int *ticket, int *err) {
+  PtPrecond row;
+  row.spf = M.spf; row.ld = M.ld; row.imat = imat; row.diag = diag; row.ngp = ngp; row.ngn = ngn; row.cr = cr;
+  sweep_parts<1, MW>(n, R, P, T, step, sp, c, ticket, err, row);
+}
+
+template <bool FWD, int NVC, int MW>
+__global__ __launch_bounds__(BS) void k_sweep_pt(int n, int R, int P, SweepTabs T, const int *__restrict__ step,
+                                                 const int *__restrict__ sp, size_t ld, const double *__restrict__ spv,
+                                                 const double *__restrict__ c, const double *__restrict__ cr,
+                                                 const double *__restrict__ src, double *out, int *ticket, int *err,
+                                                 const KS *ks) {
+  if (all_done(ks, NVC)) return;
+  PtSweep<FWD, NVC> row;
+  row.spv = spv; row.c = c; row.cr = cr; row.src = src; row.ld = ld; row.n = n;
+  sweep_parts<NVC, MW>(n, R, P, T, step, sp, out, ticket, err, row);
+}
+
 template <int WS>
 __global__ __launch_bounds__(BS) void k_spmv(DMesh M, int nvc, const double *__restrict__ diag,
                                              const double *__restrict__ spv, const double *__restrict__ x,
@@ -787,10 +883,149 @@ int sweep_masks_build(Ctx &c) {
   return 0;
 }
 
+static void free_owned(Ctx &c, void *p) {
+  if (!p) return;
+  for (size_t k = 0; k < c.owned.size(); ++k)
+    if (c.owned[k] == p) {
+      hipFree(p);
+      c.owned.erase(c.owned.begin() + (long)k);
+      return;
+    }
+}
+
+template <class T>
+static int upload_v(Ctx &c, T **dst, const std::vector<T> &src) {
+  int rc = dalloc(c, dst, src.size());
+  if (rc) return rc;
+  if (!src.empty()) FVK_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// Ranges of R rows consecutive in the reference numbering (the order of
+// the sweeps: forward ascending, backward descending) and, in each, its
+// rows by level (ties by rank), with the descriptors of sweep_parts.
+static int parts_build(Ctx &c, bool fwd, const std::vector<int> &rank, const std::vector<int> &spc,
+                       const std::vector<unsigned char> &mask, Ctx::SweepParts &pt) {
+  const int n = c.n, R = c.sweep_part_rows, Ws = c.M.Ws;
+  const size_t ld = (size_t)c.M.ld;
+  std::vector<int> row_of(n);
+  for (int r = 0; r < n; ++r) row_of[rank[r]] = r;
+  // sweep position of each row (0 = first reached) and its level
+  auto pos_of = [&](int r) { return fwd ? rank[r] : n - 1 - rank[r]; };
+  std::vector<int> lev(n, 0);
+  for (int k = 0; k < n; ++k) {
+    const int r = row_of[fwd ? k : n - 1 - k];
+    int l = 0;
+    for (int s = 0; s < Ws; ++s)
+      if ((mask[r] >> s) & 1u) l = std::max(l, lev[spc[(size_t)s * ld + r]] + 1);
+    lev[r] = l;
+  }
+  pt.R = R;
+  pt.P = (n + R - 1) / R;
+  std::vector<int> rows, first(n), cnt(n), aux(n, 0), lrow(n), step, sp;
+  rows.reserve(n);
+  std::vector<std::pair<int, int>> buf;
+  for (int q = 0; q < pt.P; ++q) {
+    buf.clear();
+    for (int k = q * R; k < std::min(n, (q + 1) * R); ++k) {
+      const int r = row_of[fwd ? k : n - 1 - k];
+      buf.emplace_back(lev[r], k);
+    }
+    std::sort(buf.begin(), buf.end());
+    sp.push_back((int)step.size());
+    for (size_t t = 0; t < buf.size(); ++t) {
+      if (t == 0 || buf[t].first != buf[t - 1].first) step.push_back((int)rows.size());
+      rows.push_back(row_of[fwd ? buf[t].second : n - 1 - buf[t].second]);
+    }
+    step.push_back((int)rows.size());
+  }
+  sp.push_back((int)step.size());
+  int W = 1;
+  for (int p = 0; p < n; ++p) {
+    first[p] = mask[rows[p]];
+    cnt[p] = __builtin_popcount(mask[rows[p]]);
+    W = std::max(W, cnt[p]);
+  }
+  pt.W = W;
+  std::vector<int> cols((size_t)W * n, 0), lcol((size_t)W * n, -1);
+  for (int p = 0; p < n; ++p) {
+    const int r = rows[p], q = pos_of(r) / R;
+    lrow[p] = pos_of(r) - q * R;
+    int t = 0;
+    for (int s = 0; s < Ws; ++s)
+      if ((mask[r] >> s) & 1u) {
+        const int j = spc[(size_t)s * ld + r];
+        cols[(size_t)t * n + p] = j;
+        if (pos_of(j) / R == q) lcol[(size_t)t * n + p] = pos_of(j) - q * R;
+        ++t;
+      }
+  }
+  int rc = upload_v(c, &pt.rows, rows);
+  if (!rc) rc = upload_v(c, &pt.first, first);
+  if (!rc) rc = upload_v(c, &pt.cnt, cnt);
+  if (!rc) rc = upload_v(c, &pt.aux, aux);
+  if (!rc) rc = upload_v(c, &pt.lrow, lrow);
+  if (!rc) rc = upload_v(c, &pt.cols, cols);
+  if (!rc) rc = upload_v(c, &pt.lcol, lcol);
+  if (!rc) rc = upload_v(c, &pt.step, step);
+  if (!rc) rc = upload_v(c, &pt.sp, sp);
+  return rc;
+}
+
+int sweep_parts_build(Ctx &c) {
+  for (Ctx::SweepParts *pt : {&c.spf, &c.spb}) {
+    for (int *q : {pt->rows, pt->first, pt->cnt, pt->aux, pt->lrow, pt->cols, pt->lcol, pt->step, pt->sp})
+      free_owned(c, q);
+    *pt = Ctx::SweepParts();
+  }
+  if (!c.sweep_one_launch) return 0;
+  const int n = c.n;
+  const size_t ld = (size_t)c.M.ld;
+  std::vector<int> rank(n), spc((size_t)c.M.Ws * ld);
+  std::vector<unsigned char> lm(n), um(n);
+  FVK_HIP(hipStreamSynchronize(c.stream));
+  FVK_HIP(hipMemcpy(rank.data(), c.M.rank, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
+  FVK_HIP(hipMemcpy(spc.data(), c.M.spc, spc.size() * sizeof(int), hipMemcpyDeviceToHost));
+  FVK_HIP(hipMemcpy(lm.data(), c.sweep_lmask, n, hipMemcpyDeviceToHost));
+  FVK_HIP(hipMemcpy(um.data(), c.sweep_umask, n, hipMemcpyDeviceToHost));
+  int rc = parts_build(c, true, rank, spc, lm, c.spf);
+  if (!rc) rc = parts_build(c, false, rank, spc, um, c.spb);
+  if (rc) return rc;
+  const int bytes = 3 * c.sweep_part_rows * 8;
+  if (bytes > 65536) {
+    const void *k[] = {(const void *)k_sweep_pt<true, 1, 4>, (const void *)k_sweep_pt<false, 1, 4>,
+                       (const void *)k_sweep_pt<true, 2, 4>, (const void *)k_sweep_pt<false, 2, 4>,
+                       (const void *)k_sweep_pt<true, 3, 4>, (const void *)k_sweep_pt<false, 3, 4>,
+                       (const void *)k_sweep_pt<true, 1, 8>, (const void *)k_sweep_pt<false, 1, 8>,
+                       (const void *)k_sweep_pt<true, 2, 8>, (const void *)k_sweep_pt<false, 2, 8>,
+                       (const void *)k_sweep_pt<true, 3, 8>, (const void *)k_sweep_pt<false, 3, 8>};
+    for (const void *f : k) FVK_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+  }
+  return 0;
+}
+
+static SweepTabs tabs_of(const Ctx::SweepParts &p) {
+  return SweepTabs{p.rows, p.first, p.cnt, p.aux, p.lrow, p.cols, p.lcol, p.W};
+}
+static int pt_grid(const Ctx &c, const Ctx::SweepParts &p) {
+  return std::max(1, std::min(p.P, c.n_cu * c.sweep_blocks_per_cu));
+}
+
 int precond_build(Ctx &c, int imat) {
   const int n = c.n;
   if (imat == 0) {
     k_precond_jacobi<<<grid_for(n), BS, 0, c.stream>>>(n, c.diag, c.pc_c, c.pc_cr);
+  } else if (c.sweep_one_launch && c.sweep_parts) {
+    const Ctx::SweepParts &P = c.spf;
+    k_sweep_mark<<<mark_grid(n), BS, 0, c.stream>>>(n, c.pc_c, c.sweep_int + 2, nullptr, 1);
+    if (P.W <= 4)
+      k_precond_pt<4><<<pt_grid(c, P), BS, (size_t)P.R * 8, c.stream>>>(n, P.R, P.P, tabs_of(P), P.step, P.sp, c.M,
+                                                                          imat, c.diag, c.ngp, c.ngn, c.pc_c,
+                                                                          c.pc_cr, c.sweep_int + 2, c.sweep_int + 3);
+    else
+      k_precond_pt<8><<<pt_grid(c, P), BS, (size_t)P.R * 8, c.stream>>>(n, P.R, P.P, tabs_of(P), P.step, P.sp, c.M,
+                                                                          imat, c.diag, c.ngp, c.ngn, c.pc_c,
+                                                                          c.pc_cr, c.sweep_int + 2, c.sweep_int + 3);
   } else if (c.sweep_one_launch) {
     k_sweep_mark<<<mark_grid(n), BS, 0, c.stream>>>(n, c.pc_c, c.sweep_int + 2, nullptr, 1);
     k_precond_sf<<<sweep_grid(c), BS, 0, c.stream>>>(c.M, c.lf.rows, c.sweep_lmask, imat, c.diag, c.ngp, c.ngn,
@@ -812,6 +1047,26 @@ int precond_apply(Ctx &c, int imat, int nvc, const double *b, double *x, const K
   const int n = c.n;
   if (imat == 0) {
     k_jacobi_apply<<<grid_for(n), BS, 0, c.stream>>>(n, nvc, c.pc_c, b, x, gate);
+  } else if (c.sweep_one_launch && c.sweep_parts) {
+    const size_t cnt = (size_t)nvc * n;
+    const Ctx::SweepParts &F = c.spf, &B = c.spb;
+    const size_t lds = (size_t)nvc * c.sweep_part_rows * 8, ld = (size_t)c.M.ld;
+    k_sweep_mark<<<mark_grid(cnt), BS, 0, c.stream>>>(cnt, c.sweep_t, c.sweep_int, gate, nvc);
+#define SW(D, V, MW, X, CR, S, O, TK) k_sweep_pt<D, V, MW><<<pt_grid(c, X), BS, lds, c.stream>>>(n, X.R, X.P, \
+    tabs_of(X), X.step, X.sp, ld, c.spv, c.pc_c, CR, S, O, TK, c.sweep_int + 3, gate)
+#define SWC(D, X, CR, S, O, TK)                                                                          \
+    if (X.W <= 4) {                                                                                      \
+      if (nvc == 1) SW(D, 1, 4, X, CR, S, O, TK); else if (nvc == 2) SW(D, 2, 4, X, CR, S, O, TK);       \
+      else SW(D, 3, 4, X, CR, S, O, TK);                                                                 \
+    } else {                                                                                             \
+      if (nvc == 1) SW(D, 1, 8, X, CR, S, O, TK); else if (nvc == 2) SW(D, 2, 8, X, CR, S, O, TK);       \
+      else SW(D, 3, 8, X, CR, S, O, TK);                                                                 \
+    }
+    SWC(true, F, nullptr, b, c.sweep_t, c.sweep_int)
+    k_sweep_mark<<<mark_grid(cnt), BS, 0, c.stream>>>(cnt, x, c.sweep_int + 1, gate, nvc);
+    SWC(false, B, c.pc_cr, c.sweep_t, x, c.sweep_int + 1)
+#undef SWC
+#undef SW
   } else if (c.sweep_one_launch) {
     const size_t cnt = (size_t)nvc * n;
     k_sweep_mark<<<mark_grid(cnt), BS, 0, c.stream>>>(cnt, c.sweep_t, c.sweep_int, gate, nvc);

@@ -66,18 +66,19 @@ __device__ __forceinline__ void sweep_chunks(int n, int *ticket, int *err, Row &
 // at most this many lower (upper) entries per row are kept in registers
 constexpr int kSweepSlots = 8;
 
-// Partitioned sweep.  The rows are cut into P ranges of R consecutive rows
-// (forward: range q = [qR, (q+1)R); backward: counted from the end); a block
-// takes the ranges in ticket order (so a range waits only for ranges taken
-// earlier) and walks its rows level by level in the sweep's level order, a
-// barrier between levels.  Values of rows inside the range pass through LDS
+// Partitioned sweep.  The rows are cut into P ranges of R rows consecutive
+// in the sweep's order (range q holds the rows the sweep reaches q-th); a
+// block takes the ranges in ticket order (so a range waits only for ranges
+// taken earlier) and walks its rows level by level, a barrier between
+// levels.  Values of rows inside the range pass through LDS
 // (NC x R doubles); values from other ranges are read from `out` with
 // agent-scope loads until they are no longer the mark.  Range q's levels
 // are positions [step[k], step[k+1]) for k in [sp[q], sp[q+1]-1).
 //
-// Each position p has a descriptor: its row, the row's first entry, entry
-// count, one more row-specific index, and the entries' columns (slot-major,
-// W slots of n, padding 0).  The first BS positions of each level are
+// Each position p has a descriptor: its row, a row-specific first-entry
+// word, the entry count, one more row-specific index, the row's own LDS
+// slot (its index in the range), and per entry (slot-major, W slots of n)
+// the column and the column's LDS slot (-1: another range).  The first BS positions of each level are
 // software-pipelined: the descriptors two levels ahead, the row's
 // coefficients and the values from other ranges one level ahead, all loads
 // issued without branches (a lane past the level's end loads the level's
@@ -87,15 +88,16 @@ constexpr int kSweepSlots = 8;
 // Row interface: fetch(d, MW) loads the row's coefficients (entries t < MW,
 // entries t >= d.nl from a valid address, unused); begin() does the arithmetic on them that precedes the
 // entries; take(t, v) adds entry t with the NC values v[] (called in entry
-// order, t < d.nl); result(c) is component c of the row's value.
+// order, t < d.nl); result(c) is component c of the row's value; done(d)
+// runs after the row's value is stored.
 struct SweepTabs {
-  const int *rows, *first, *cnt, *aux, *cols;
+  const int *rows, *first, *cnt, *aux, *lrow, *cols, *lcol;
   int W;
 };
 
 struct SweepDesc {
-  int r, first, nl, aux;
-  int j[kSweepSlots];
+  int r, first, nl, aux, lr;
+  int j[kSweepSlots], li[kSweepSlots];
 };
 
 // read-only tables with a uniform index: scalar loads, which do not wait
@@ -115,18 +117,23 @@ __device__ __forceinline__ void sweep_desc(const SweepTabs &T, int n, int p, int
   const int c = T.cnt[q];
   d.nl = live ? c : -1;
   d.aux = T.aux[q];
+  d.lr = T.lrow[q];
 #pragma unroll
-  for (int t = 0; t < MW; ++t) d.j[t] = T.cols[(size_t)(t < T.W ? t : T.W - 1) * n + q];
+  for (int t = 0; t < MW; ++t) {
+    const size_t o = (size_t)(t < T.W ? t : T.W - 1) * n + q;
+    d.j[t] = T.cols[o];
+    d.li[t] = T.lcol[o];
+  }
 }
 
-// values of entries outside [lo, hi): one agent-scope load per entry slot
+// values of entries in other ranges: one agent-scope load per entry slot
 // t < MW (slots that need none load the row's own value, unused)
 template <int NC, int MW>
-__device__ __forceinline__ void sweep_ext(const SweepDesc &d, int lo, int hi, int n, const double *out,
+__device__ __forceinline__ void sweep_ext(const SweepDesc &d, int n, const double *out,
                                           unsigned long long (&v)[kSweepSlots][NC]) {
 #pragma unroll
   for (int t = 0; t < MW; ++t) {
-    const bool ext = t < d.nl && (d.j[t] < lo || d.j[t] >= hi);
+    const bool ext = t < d.nl && d.li[t] < 0;
     const int jj = ext ? d.j[t] : d.r;
 #pragma unroll
     for (int c = 0; c < NC; ++c) v[t][c] = ld_agent(out + (size_t)c * n + jj);
@@ -137,17 +144,17 @@ __device__ __forceinline__ void sweep_ext(const SweepDesc &d, int lo, int hi, in
 // v[] (polled again while they are the mark); false if the wait bound ran out
 template <int NC, int MW, class Row>
 __device__ __forceinline__ bool sweep_row(const SweepDesc &d, Row &row,
-                                          unsigned long long (&v)[kSweepSlots][NC], int lo, int hi, int R, int n,
-                                          double *out, double *lv, int *err) {
+                                          unsigned long long (&v)[kSweepSlots][NC], int R, int n, double *out,
+                                          double *lv, int *err) {
   row.begin();
   unsigned need = 0;
 #pragma unroll
   for (int t = 0; t < MW; ++t) {
     if (t >= d.nl) continue;
-    const int j = d.j[t];
-    if (j >= lo && j < hi) {
+    const int l = d.li[t];
+    if (l >= 0) {
 #pragma unroll
-      for (int c = 0; c < NC; ++c) v[t][c] = (unsigned long long)__double_as_longlong(lv[c * R + (j - lo)]);
+      for (int c = 0; c < NC; ++c) v[t][c] = (unsigned long long)__double_as_longlong(lv[c * R + l]);
     } else {
       bool ready = true;
 #pragma unroll
@@ -188,9 +195,10 @@ __device__ __forceinline__ bool sweep_row(const SweepDesc &d, Row &row,
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     const double x = row.result(c);
-    lv[c * R + (d.r - lo)] = x;
+    lv[c * R + d.lr] = x;
     st_agent(out + (size_t)c * n + d.r, x);
   }
+  row.done(d);
   return true;
 }
 
@@ -202,7 +210,7 @@ struct SweepStage {
 };
 
 template <int NC, int MW, class Row>
-__device__ __forceinline__ void sweep_parts(bool fwd, int n, int R, int P, const SweepTabs &T,
+__device__ __forceinline__ void sweep_parts(int n, int R, int P, const SweepTabs &T,
                                             const int *__restrict__ step, const int *__restrict__ sp, double *out,
                                             int *ticket, int *err, const Row &proto) {
   extern __shared__ double lv[];
@@ -214,8 +222,6 @@ __device__ __forceinline__ void sweep_parts(bool fwd, int n, int R, int P, const
     __syncthreads();
     const int q = s_q;
     if (q >= P || s_abort) return;
-    const int lo = fwd ? q * R : (n - (q + 1) * R > 0 ? n - (q + 1) * R : 0);
-    const int hi = fwd ? (n < (q + 1) * R ? n : (q + 1) * R) : n - q * R;
     const int k0 = ld_uniform(sp, q), k1 = ld_uniform(sp, q + 1) - 1;
     // descriptors of level k (past the last level: dead copies of it)
     auto desc = [&](int k, SweepDesc &d) {
@@ -227,7 +233,7 @@ __device__ __forceinline__ void sweep_parts(bool fwd, int n, int R, int P, const
     auto stage = [&](const SweepDesc &d, SweepStage<NC, Row> &S) {
       S.d = d;
       S.row.fetch(S.d, MW);
-      sweep_ext<NC, MW>(S.d, lo, hi, n, out, S.v);
+      sweep_ext<NC, MW>(S.d, n, out, S.v);
     };
     // positions of level k beyond the first BS, not pipelined
     auto rest = [&](int k) -> bool {
@@ -239,7 +245,7 @@ __device__ __forceinline__ void sweep_parts(bool fwd, int n, int R, int P, const
         Z.row = proto;
         sweep_desc<MW>(T, n, p, e, d);
         stage(d, Z);
-        ok = sweep_row<NC, MW>(Z.d, Z.row, Z.v, lo, hi, R, n, out, lv, err);
+        ok = sweep_row<NC, MW>(Z.d, Z.row, Z.v, R, n, out, lv, err);
       }
       return ok;
     };
@@ -255,7 +261,7 @@ __device__ __forceinline__ void sweep_parts(bool fwd, int n, int R, int P, const
       // level k in X; level k+1 into Y; descriptors of level k+2
       stage(D, Y);
       desc(k + 2, D);
-      if (ok && X.d.nl >= 0) ok = sweep_row<NC, MW>(X.d, X.row, X.v, lo, hi, R, n, out, lv, err);
+      if (ok && X.d.nl >= 0) ok = sweep_row<NC, MW>(X.d, X.row, X.v, R, n, out, lv, err);
       if (ok) ok = rest(k);
       if (!ok) s_abort = 1;
       __syncthreads();
@@ -264,7 +270,7 @@ __device__ __forceinline__ void sweep_parts(bool fwd, int n, int R, int P, const
       // level k+1 in Y; level k+2 into X; descriptors of level k+3
       stage(D, X);
       desc(k + 3, D);
-      if (ok && Y.d.nl >= 0) ok = sweep_row<NC, MW>(Y.d, Y.row, Y.v, lo, hi, R, n, out, lv, err);
+      if (ok && Y.d.nl >= 0) ok = sweep_row<NC, MW>(Y.d, Y.row, Y.v, R, n, out, lv, err);
       if (ok) ok = rest(k + 1);
       if (!ok) s_abort = 1;
       __syncthreads();

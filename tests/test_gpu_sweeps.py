@@ -1,5 +1,9 @@
-"""DIC / DILU factor and triangular sweeps in one launch each (rows wait for
-the values they need) against the level-by-level launches and the oracle.
+"""DIC / DILU factor and triangular sweeps in one launch each against the
+level-by-level launches and the oracle.  One launch: blocks owning ranges of
+rows consecutive in the reference numbering, walked level by level with the
+values inside a range passing through LDS (the default; ranges of 4096 rows,
+and of 256 rows so that each block takes many ranges), or rows taken in level
+order in chunks; rows wait for the values they need from elsewhere.
 
 Bar: bitwise.  Both forms perform the same operations in the same order for
 every row (solv_precond.f90:10-29, solv_direct.f90:11-42); only the
@@ -41,12 +45,26 @@ def random_matrix(m, seed):
     return pl, ngp, ngn
 
 
-def both_modes(fvk, ctx, fn):
-    fvk.set_param(ctx, "sweep_one_launch", 0)
-    a = fn()
-    fvk.set_param(ctx, "sweep_one_launch", 1)
-    b = fn()
-    return a, b
+MODES = [
+    ("one launch per level", {"sweep_one_launch": 0}),
+    ("chunks in level order", {"sweep_one_launch": 1, "sweep_parts": 0}),
+    ("ranges of 4096 rows", {"sweep_one_launch": 1, "sweep_parts": 1, "sweep_part_rows": 4096}),
+    ("ranges of 256 rows", {"sweep_one_launch": 1, "sweep_parts": 1, "sweep_part_rows": 256}),
+]
+
+
+def set_mode(fvk, ctx, params):
+    for k, v in params.items():
+        fvk.set_param(ctx, k, v)
+
+
+def all_modes(fvk, ctx, fn):
+    out = []
+    for label, params in MODES:
+        set_mode(fvk, ctx, params)
+        out.append((label, fn()))
+    set_mode(fvk, ctx, MODES[2][1])
+    return out
 
 
 @pytest.mark.parametrize("n,rcm", [(32, False), (32, True), (96, False)])
@@ -59,11 +77,14 @@ def test_factor_and_sweeps_one_launch_bitwise(n, rcm):
     fvk.set_matrix(ctx, pl, ngp, ngn)
     b = np.random.default_rng(n).standard_normal(m.n_vol)
     for imat in (1, 2):
-        (c0, cr0), (c1, cr1) = both_modes(fvk, ctx, lambda: fvk.solv_precond(ctx, imat))
-        assert np.array_equal(c0, c1) and np.array_equal(cr0, cr1), ("factor", imat)
-        x0, x1 = both_modes(fvk, ctx, lambda: fvk.solv_direct(ctx, imat, b))
-        assert np.array_equal(x0, x1), ("sweeps", imat)
-        assert np.all(np.isfinite(x1))
+        f = all_modes(fvk, ctx, lambda: fvk.solv_precond(ctx, imat))
+        c0, cr0 = f[0][1]
+        for label, (c1, cr1) in f[1:]:
+            assert np.array_equal(c0, c1) and np.array_equal(cr0, cr1), ("factor", imat, label)
+        x = all_modes(fvk, ctx, lambda: fvk.solv_direct(ctx, imat, b))
+        for label, x1 in x[1:]:
+            assert np.array_equal(x[0][1], x1), ("sweeps", imat, label)
+        assert np.all(np.isfinite(x[0][1]))
         say("  imat %d: factor and sweeps bitwise, levels %s" % (imat, ctx.levels()))
     if n <= 32:
         A = fvo.OSys(fvo.OMesh.box(n, n, n, jitter=0.05, seed=0x5EED0100 + n), pl, ngp, ngn)
@@ -90,8 +111,9 @@ def test_krylov_on_one_launch_sweeps_bitwise():
         rn, st = fvk.solv_bicg(ctx, 2, 1e-30, 6, x, r)
         return np.concatenate([x.ravel(), r.ravel(), rn])
 
-    a, b = both_modes(fvk, ctx, bicg)
-    assert np.array_equal(a, b), "BiCGStab (3 right-hand sides, DILU)"
+    r = all_modes(fvk, ctx, bicg)
+    for label, b in r[1:]:
+        assert np.array_equal(r[0][1], b), ("BiCGStab (3 right-hand sides, DILU)", label)
     fvk.set_matrix(ctx, pl, ngp, ngp)
     bb = b3[:, 0] - b3[:, 0].mean()
 
@@ -101,8 +123,9 @@ def test_krylov_on_one_launch_sweeps_bitwise():
         rn, st = fvk.solv_cg(ctx, 1, 1e-30, 20, x, r)
         return np.concatenate([x.ravel(), r.ravel(), rn])
 
-    a, b = both_modes(fvk, ctx, cg)
-    assert np.array_equal(a, b), "PCG (DIC)"
+    r = all_modes(fvk, ctx, cg)
+    for label, b in r[1:]:
+        assert np.array_equal(r[0][1], b), ("PCG (DIC)", label)
     ctx.destroy()
     m.free()
 
@@ -114,9 +137,9 @@ def test_shipped_simple_one_launch_bitwise(graph):
     say("shipped SIMPLE: box %d^3 graph %s" % (n, graph))
     m = fvk.Mesh.box(n, n, n, jitter=0.02, seed=0x5EED0301)
     out = []
-    for mode in (0, 1):
+    for label, params in MODES:
         ctx = fvk.grd_solv(m, rcm=True)
-        fvk.set_param(ctx, "sweep_one_launch", mode)
+        set_mode(fvk, ctx, params)
         fl = fvk.Flow(ctx, fvk.flow_cfg())
         fl.set_bnd_u(31, [1.0, 0.0, 0.0])
         h = (fl.run_graph if graph else fl.run)(1, 5)
@@ -124,9 +147,10 @@ def test_shipped_simple_one_launch_bitwise(graph):
         out.append((st, [x["mx_res_n"] for x in h]))
         fl.destroy()
         ctx.destroy()
-    (s0, h0), (s1, h1) = out
-    for k in ("u_cell", "p", "dpdx", "spd"):
-        assert np.array_equal(s0[k], s1[k]), k
-    assert h0 == h1
-    say("  residual history %s" % h1)
+    s0, h0 = out[0]
+    for (label, _), (s1, h1) in zip(MODES[1:], out[1:]):
+        for k in ("u_cell", "p", "dpdx", "spd"):
+            assert np.array_equal(s0[k], s1[k]), (label, k)
+        assert h0 == h1, label
+    say("  residual history %s" % h0)
     m.free()
