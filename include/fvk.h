@@ -250,7 +250,15 @@ int fvk_sys_get_x(fvk_ctx *ctx, double *x);
    "dc_max_breakdowns" (inner breakdowns followed by a restart from the
    current x before the solve gives up, default 1), "dc_carry" (1: a
    correction step continues the previous step's search direction; 0:
-   restart, default). */
+   restart, default).  DIC / DILU factor and sweeps (solv_precond,
+   solv_direct): "sweep_one_launch" (1, default when no row has more than 8
+   slots: one launch each, rows waiting for their values; 0: one launch per
+   level), "sweep_parts" (1, default: each block owns ranges of
+   "sweep_part_rows" rows consecutive in the reference numbering, default
+   4096, 256..6400, and walks them level by level, values inside a range
+   passing through LDS; 0: rows taken in level order in chunks by
+   "sweep_blocks_per_cu" resident blocks per CU, default 1).  All give the
+   same bits. */
 int fvk_ctx_set_param(fvk_ctx *ctx, const char *name, double value);
 
 /* The factorised preconditioner z = G G^T r, G = D^-1/2 (I + L) Delta, with

@@ -94,7 +94,7 @@ struct Ctx {
   Levels lf, lb;                  // forward / backward level sets
   // one-launch factorisation and sweeps (else one launch per level)
   bool sweep_one_launch = true;
-  int sweep_blocks_per_cu = 1;    // resident blocks per CU of the one-launch kernels (1: measured fastest)
+  int sweep_blocks_per_cu = 1;    // resident blocks per CU of the chunked one-launch kernels (1: measured fastest)
   int n_cu = 256;
   double *sweep_t = nullptr;      // forward sweep output (3 x n)
   int *sweep_int = nullptr;       // tickets: forward, backward, factor; [3] = wait-bound error

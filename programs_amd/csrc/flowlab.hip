@@ -913,8 +913,11 @@ static int fl_sweep_grid(const FlCtx &c) {
 }
 static int fl_mark_grid(size_t cnt) { return (int)std::max<size_t>(1, std::min<size_t>((cnt + BS - 1) / BS, 4096)); }
 
-static int fl_pt_grid(const FlCtx &c, const FlCtx::Parts &pt) {
-  return std::max(1, std::min(pt.P, c.sweep_blocks > 0 ? c.sweep_blocks : c.n_cu));
+// blocks of the partitioned kernels (sweep_blocks 0: as many per CU as the
+// LDS of a block lets reside, at most two)
+static int fl_pt_grid(const FlCtx &c, const FlCtx::Parts &pt, size_t lds) {
+  const int per_cu = std::max(1, std::min<int>(2, (int)(163840 / (lds + 1024))));
+  return std::max(1, std::min(pt.P, c.sweep_blocks > 0 ? c.sweep_blocks : c.n_cu * per_cu));
 }
 
 static int fl_precond(FlCtx &c, const double *a) {
@@ -922,10 +925,10 @@ static int fl_precond(FlCtx &c, const double *a) {
     const FlCtx::Parts &P = c.pf;
     k_fl_mark<<<fl_mark_grid(c.n), BS, 0, c.stream>>>(c.n, c.pc, c.sweep_int + 2, nullptr, 1);
     if (P.W <= 4)
-      k_fl_pc_pt<4><<<fl_pt_grid(c, P), BS, (size_t)P.R * 8, c.stream>>>(c.n, P.R, P.P, P.tabs(), P.step, P.sp, c.tpos,
+      k_fl_pc_pt<4><<<fl_pt_grid(c, P, (size_t)P.R * 8), BS, (size_t)P.R * 8, c.stream>>>(c.n, P.R, P.P, P.tabs(), P.step, P.sp, c.tpos,
                                                                         a, c.pc, c.sweep_int + 2, c.sweep_int + 3);
     else
-      k_fl_pc_pt<8><<<fl_pt_grid(c, P), BS, (size_t)P.R * 8, c.stream>>>(c.n, P.R, P.P, P.tabs(), P.step, P.sp, c.tpos,
+      k_fl_pc_pt<8><<<fl_pt_grid(c, P, (size_t)P.R * 8), BS, (size_t)P.R * 8, c.stream>>>(c.n, P.R, P.P, P.tabs(), P.step, P.sp, c.tpos,
                                                                         a, c.pc, c.sweep_int + 2, c.sweep_int + 3);
     FVK_HIP(hipGetLastError());
     return 0;
@@ -953,7 +956,7 @@ static int fl_direct(FlCtx &c, int ncmp, const double *a, const double *b, doubl
     const FlCtx::Parts &F = c.pf, &B = c.pb;
     const size_t lds = (size_t)ncmp * c.part_rows * 8;
     k_fl_mark<<<fl_mark_grid(cnt), BS, 0, c.stream>>>(cnt, t, c.sweep_int, ks, ncmp);
-#define SW(D, C, MW, X, S, O, TK) k_fl_sweep_pt<D, C, MW><<<fl_pt_grid(c, X), BS, lds, c.stream>>>(c.n, X.R, X.P, \
+#define SW(D, C, MW, X, S, O, TK) k_fl_sweep_pt<D, C, MW><<<fl_pt_grid(c, X, lds), BS, lds, c.stream>>>(c.n, X.R, X.P, \
     X.tabs(), X.step, X.sp, a, c.pc, S, O, TK, c.sweep_int + 3, ks)
 #define SWC(D, X, S, O, TK)                                                              \
     if (X.W <= 4) {                                                                      \

@@ -1007,8 +1007,12 @@ int sweep_parts_build(Ctx &c) {
 static SweepTabs tabs_of(const Ctx::SweepParts &p) {
   return SweepTabs{p.rows, p.first, p.cnt, p.aux, p.lrow, p.cols, p.lcol, p.W};
 }
-static int pt_grid(const Ctx &c, const Ctx::SweepParts &p) {
-  return std::max(1, std::min(p.P, c.n_cu * c.sweep_blocks_per_cu));
+// blocks of the partitioned kernels: as many per CU as the LDS of a block
+// lets reside, at most two (measured: two blocks of 32 KiB per CU beat one;
+// the ranges a grid cannot hold at once wait for a block to finish)
+static int pt_grid(const Ctx &c, const Ctx::SweepParts &p, size_t lds) {
+  const int per_cu = std::max(1, std::min<int>(2, (int)(163840 / (lds + 1024))));
+  return std::max(1, std::min(p.P, c.n_cu * per_cu));
 }
 
 int precond_build(Ctx &c, int imat) {
@@ -1019,11 +1023,11 @@ int precond_build(Ctx &c, int imat) {
     const Ctx::SweepParts &P = c.spf;
     k_sweep_mark<<<mark_grid(n), BS, 0, c.stream>>>(n, c.pc_c, c.sweep_int + 2, nullptr, 1);
     if (P.W <= 4)
-      k_precond_pt<4><<<pt_grid(c, P), BS, (size_t)P.R * 8, c.stream>>>(n, P.R, P.P, tabs_of(P), P.step, P.sp, c.M,
+      k_precond_pt<4><<<pt_grid(c, P, (size_t)P.R * 8), BS, (size_t)P.R * 8, c.stream>>>(n, P.R, P.P, tabs_of(P), P.step, P.sp, c.M,
                                                                           imat, c.diag, c.ngp, c.ngn, c.pc_c,
                                                                           c.pc_cr, c.sweep_int + 2, c.sweep_int + 3);
     else
-      k_precond_pt<8><<<pt_grid(c, P), BS, (size_t)P.R * 8, c.stream>>>(n, P.R, P.P, tabs_of(P), P.step, P.sp, c.M,
+      k_precond_pt<8><<<pt_grid(c, P, (size_t)P.R * 8), BS, (size_t)P.R * 8, c.stream>>>(n, P.R, P.P, tabs_of(P), P.step, P.sp, c.M,
                                                                           imat, c.diag, c.ngp, c.ngn, c.pc_c,
                                                                           c.pc_cr, c.sweep_int + 2, c.sweep_int + 3);
   } else if (c.sweep_one_launch) {
@@ -1052,7 +1056,7 @@ int precond_apply(Ctx &c, int imat, int nvc, const double *b, double *x, const K
     const Ctx::SweepParts &F = c.spf, &B = c.spb;
     const size_t lds = (size_t)nvc * c.sweep_part_rows * 8, ld = (size_t)c.M.ld;
     k_sweep_mark<<<mark_grid(cnt), BS, 0, c.stream>>>(cnt, c.sweep_t, c.sweep_int, gate, nvc);
-#define SW(D, V, MW, X, CR, S, O, TK) k_sweep_pt<D, V, MW><<<pt_grid(c, X), BS, lds, c.stream>>>(n, X.R, X.P, \
+#define SW(D, V, MW, X, CR, S, O, TK) k_sweep_pt<D, V, MW><<<pt_grid(c, X, lds), BS, lds, c.stream>>>(n, X.R, X.P, \
     tabs_of(X), X.step, X.sp, ld, c.spv, c.pc_c, CR, S, O, TK, c.sweep_int + 3, gate)
 #define SWC(D, X, CR, S, O, TK)                                                                          \
     if (X.W <= 4) {                                                                                      \
