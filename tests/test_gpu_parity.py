@@ -8,9 +8,10 @@ Bars (stated per assertion):
   * 1e-10 relative (max-norm) for anything downstream of a dot product or L1
     norm (PCG, BiCGStab, SIMPLE fields and residual histories): the device
     reduces in a fixed tree, the reference sequentially.  Where the reference
-    algorithm itself is ill-conditioned, the bar widens to 30x the change the
+    algorithm itself is ill-conditioned, the bar widens to 3x the change the
     oracle shows when only its own summation order is switched from
-    sequential to pairwise (sens_tol); that change is printed.
+    sequential to pairwise (sens_tol), capped per case (WIDEN_CAP); that
+    change and the error reached are printed.
 """
 import numpy as np
 import pytest
@@ -23,18 +24,39 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-10
 
+# The only solves of the golden cases whose reference algorithm moves by more
+# than TOL/3 when its summation order changes: BiCGStab preconditioned with
+# DIC, a factor for symmetric matrices, on the non-symmetric momentum matrix
+# of the 2D channel, a combination the reference does not ship (its momentum
+# solves use DILU, cfd_setup.f90:528).  The oracle moves by 6.8e-4 after 6
+# iterations and by 1.8e-7 after the 5 iterations with reduc 0.1.  The
+# widened bar of every other solve is capped at TOL.
+WIDEN_CAP = {("chan2d_shipped", "bicg6", 1): 2.5e-3, ("chan2d_shipped", "bicg5r", 1): 1e-6}
 
-def sens_tol(run):
-    """max(TOL, 30 * |run(sequential) - run(pairwise)| / |run(sequential)|)
-    for a callable returning a flat array from the oracle."""
+
+def sens_tol(run, key):
+    """max(TOL, 3 * |run(sequential) - run(pairwise)| / |run(sequential)|)
+    for a callable returning a flat array from the oracle, capped at
+    WIDEN_CAP.get(key, TOL)."""
     a = np.asarray(run(), np.float64).ravel()
     with fvo.sum_order(1):
         b = np.asarray(run(), np.float64).ravel()
     den = np.max(np.abs(a)) if a.size else 0.0
     sens = float(np.max(np.abs(a - b)) / den) if den > 0 else 0.0
-    if sens > TOL / 30:
-        print("order sensitivity of the reference algorithm: %.2e" % sens)
-    return max(TOL, 30.0 * sens)
+    tol = max(TOL, 3.0 * sens)
+    cap = WIDEN_CAP.get(key, TOL)
+    if 3.0 * sens > TOL:
+        print("%s: order sensitivity of the reference algorithm %.2e, bar %.2e (cap %.1e)" % (key, sens, tol, cap))
+    assert tol <= cap, "%s: the reference algorithm's order sensitivity %.2e exceeds the cap %.1e" % (key, sens, cap)
+    return tol
+
+
+def check_rel(x, ref, tol, what):
+    """assert_rel, printing the error reached where the bar is widened."""
+    if tol > TOL:
+        from tests.helpers import relerr
+        print("%s: relative error %.2e against the widened bar %.2e" % (what, relerr(x, ref), tol))
+    assert_rel(x, ref, tol, what)
 
 
 def product_mesh(rec):
@@ -64,8 +86,8 @@ def test_solvers_match_reference(name, rcm):
     om = G.oracle_mesh(rec)
     A = fvo.OSys(om, rec["d_mom_pl"], rec["d_mom_ngp"], rec["d_mom_ngn"])
     for imat in (0, 1, 2):
-        tol6 = sens_tol(lambda: A.solve("bicg", imat, 1e-30, 6, b3)[0])
-        tol5 = sens_tol(lambda: A.solve("bicg", imat, 0.1, 5, b3)[0])
+        tol6 = sens_tol(lambda: A.solve("bicg", imat, 1e-30, 6, b3)[0], (name, "bicg6", imat))
+        tol5 = sens_tol(lambda: A.solve("bicg", imat, 0.1, 5, b3)[0], (name, "bicg5r", imat))
         c, cr = fvk.solv_precond(ctx, imat)
         assert np.array_equal(c, rec["d_mom_c%d" % imat]), ("precond c exact", imat)
         assert np.array_equal(cr, rec["d_mom_cr%d" % imat]), ("precond c_r exact", imat)
@@ -73,19 +95,19 @@ def test_solvers_match_reference(name, rcm):
         x = np.zeros((nv, nd))
         r = b3.copy()
         rn, st = fvk.solv_bicg(ctx, imat, 1e-30, 6, x, r)
-        assert_rel(x.ravel(), rec["d_mom_bicg_x%d" % imat], tol6, "bicg x imat %d" % imat)
+        check_rel(x.ravel(), rec["d_mom_bicg_x%d" % imat], tol6, "bicg x imat %d" % imat)
         assert_rel(r.T.ravel(), rec["d_mom_bicg_r%d" % imat], 0.0, "bicg residual (solv_res, exact)")
         assert_rel(rn, rec["d_mom_bicg_n%d" % imat], TOL, "bicg res0_n")
         x = np.zeros((nv, nd))
         r = b3.copy()
         fvk.solv_bicg(ctx, imat, 0.1, 5, x, r)
-        assert_rel(x.ravel(), rec["d_mom_bicgr_x%d" % imat], tol5, "bicg reduc=0.1 x imat %d" % imat)
+        check_rel(x.ravel(), rec["d_mom_bicgr_x%d" % imat], tol5, "bicg reduc=0.1 x imat %d" % imat)
     fvk.set_matrix(ctx, rec["d_pp_pl"], rec["d_pp_ngp"], rec["d_pp_ngp"])
     P = fvo.OSys(om, rec["d_pp_pl"], rec["d_pp_ngp"], rec["d_pp_ngp"])
     bb = b3[:, 0] - np.cumsum(b3[:, 0])[-1] / nv
     for imat in (0, 1, 2):
-        tolc = sens_tol(lambda: P.solve("cg", imat, 1e-30, 15, bb)[0])
-        tolr = sens_tol(lambda: P.solve("cg", imat, 0.1, 20, bb, not_r0=True)[0])
+        tolc = sens_tol(lambda: P.solve("cg", imat, 1e-30, 15, bb)[0], (name, "cg15", imat))
+        tolr = sens_tol(lambda: P.solve("cg", imat, 0.1, 20, bb, not_r0=True)[0], (name, "cg20r", imat))
         c, _ = fvk.solv_precond(ctx, imat)
         assert np.array_equal(c, rec["d_pp_c%d" % imat])
         assert np.array_equal(fvk.solv_direct(ctx, imat, bb), rec["d_pp_dir%d" % imat])
@@ -93,11 +115,11 @@ def test_solvers_match_reference(name, rcm):
         r = bb.reshape(-1, 1).copy()
         rn, st = fvk.solv_cg(ctx, imat, 1e-30, 15, x, r)
         assert st.iters[0] == 15
-        assert_rel(x[:, 0], rec["d_pp_cg_x%d" % imat], tolc, "cg x imat %d" % imat)
+        check_rel(x[:, 0], rec["d_pp_cg_x%d" % imat], tolc, "cg x imat %d" % imat)
         x = np.zeros((nv, 1))
         r = bb.reshape(-1, 1).copy()
         fvk.solv_cg(ctx, imat, 0.1, 20, x, r, not_r0=True)
-        assert_rel(x[:, 0], rec["d_pp_cgr_x%d" % imat], tolr, "cg reduc=0.1 x imat %d" % imat)
+        check_rel(x[:, 0], rec["d_pp_cgr_x%d" % imat], tolr, "cg reduc=0.1 x imat %d" % imat)
     ctx.destroy()
 
 
@@ -177,7 +199,7 @@ def test_simple_matches_reference(name):
     fvk = require_lib()
     rec = G.load(name)
     spec = rec["spec"]
-    tol = sens_tol(lambda: oracle_flow_fields(rec))
+    tol = sens_tol(lambda: oracle_flow_fields(rec), (name, "simple", 0))
     m = product_mesh(rec)
     ctx, fl, hist = run_flow(fvk, m, spec, flow_cfg_for(fvk, rec))
     H = {k: np.array([h[k] for h in hist]) for k in ("reduc", "redpp", "red_p", "mx_res_n", "redpp_sum")}
