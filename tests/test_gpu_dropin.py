@@ -80,7 +80,7 @@ def test_fortran_dropin_matches_reference(name, tmp_path):
     for k in ("grad0_1", "grad0_2", "grad0_3", "grad2_1", "grad2_2", "grad2_3"):
         assert np.array_equal(d[k], rec["d_" + k]), k
     # flow fields and histories
-    tol = sens_tol(lambda: oracle_flow_fields(rec))
+    tol = sens_tol(lambda: oracle_flow_fields(rec), (name, "simple", 0))
     assert_rel(h["reduc"], rec["h_reduc"], tol, "reduc: history")
     assert_rel(h["redpp"], rec["h_redpp"], tol, "redpp: history")
     assert_rel(d["hist_REDUC"], rec["d_hist_REDUC"], tol, "REDUC history")
@@ -107,10 +107,10 @@ def test_fortran_dropin_matches_reference(name, tmp_path):
             c, _ = P.precond(imat)
             assert np.array_equal(d["pp_c%d" % imat], c), ("pp precond", imat)
             assert np.array_equal(d["pp_dir%d" % imat], P.direct(imat, bb)), ("pp direct", imat)
-        tb = sens_tol(lambda: A.solve("bicg", 0, 0.1, 5, b3)[0])
+        tb = sens_tol(lambda: A.solve("bicg", 0, 0.1, 5, b3)[0], (name, "bicg5r", 0))
         xb = A.solve("bicg", 0, 0.1, 5, b3)[0]
         assert_rel(d["mom_bicgr_x0"], xb.ravel(), tb, "bicg reduc 0.1 jacobi")
-        tc = sens_tol(lambda: P.solve("cg", 0, 0.1, 20, bb, not_r0=True)[0])
+        tc = sens_tol(lambda: P.solve("cg", 0, 0.1, 20, bb, not_r0=True)[0], (name, "cg20r", 0))
         xc = P.solve("cg", 0, 0.1, 20, bb, not_r0=True)[0]
         assert_rel(d["pp_cgr_x0"], xc.ravel(), tc, "cg reduc 0.1 jacobi")
     finally:

@@ -31,7 +31,10 @@ TOL = 1e-10
 # solves use DILU, cfd_setup.f90:528).  The oracle moves by 6.8e-4 after 6
 # iterations and by 1.8e-7 after the 5 iterations with reduc 0.1.  The
 # widened bar of every other solve is capped at TOL.
-WIDEN_CAP = {("chan2d_shipped", "bicg6", 1): 2.5e-3, ("chan2d_shipped", "bicg5r", 1): 1e-6}
+# Beyond the golden cases, SIMPLE on the 32^3 cavity over 200 outer iterations
+# (configuration C1, relaxed) moves by 2.5e-5 with the summation order.
+WIDEN_CAP = {("chan2d_shipped", "bicg6", 1): 2.5e-3, ("chan2d_shipped", "bicg5r", 1): 1e-6,
+             ("box32", "simple200", 0): 1e-4}
 
 
 def sens_tol(run, key):

@@ -82,7 +82,7 @@ def test_cavity_32cube_200_outer_iterations():
     om = fvo.OMesh.box(n, n, n)
     try:
         t0 = time.time()
-        tol = sens_tol(lambda: flat(*oracle_run(om, RELAXED, n_outer)))
+        tol = sens_tol(lambda: flat(*oracle_run(om, RELAXED, n_outer)), ("box32", "simple200", 0))
         H, F = oracle_run(om, RELAXED, n_outer)
         print("oracle: %.1f s for 3 runs of %d outer iterations" % (time.time() - t0, n_outer), flush=True)
     finally:
@@ -145,7 +145,7 @@ def test_pressure_to_tolerance_matches_oracle(name):
     om = G.oracle_mesh(rec)
     try:
         run = lambda: flat(*oracle_run(om, kw, spec["n_outer"], bc))  # noqa: E731
-        tol = sens_tol(run)
+        tol = sens_tol(run, (name, "p_tol", 0))
         H, F = oracle_run(om, kw, spec["n_outer"], bc)
     finally:
         om.free()
