@@ -544,6 +544,76 @@ def flowlab_leg(fvk, args):
     return out
 
 
+def dropin_leg(fvk, args):
+    """spray-simulation's SIMPLE in the shipped configuration on an n^3
+    cavity (+-2% jitter, Re 100, urf_p 0.1) three ways on the same Gambit
+    mesh: the reference's own Fortran (flang -O2, one core: its cfd_setup_eqn
+    and cfd_setup_pp, cfd_setup.f90), the same Fortran host program with the
+    drop-in modules (programs_amd/fortran: the assembly and solves on the GPU,
+    the flow state pushed to and pulled from the host's grd structure around
+    every cfd_setup call), and the native GPU flow (fvk.Flow, state resident
+    in HBM).  Milliseconds per outer iteration over iterations 3..K+2 (the
+    host drivers print each iteration's time of the two cfd_setup calls)."""
+    import subprocess
+    import tempfile
+    from programs_amd import meshgen
+    here = os.path.dirname(os.path.abspath(__file__))
+    n, K = args.dropin, args.dropin_its
+    N = K + 2
+    m = fvk.Mesh.box(n, n, n, jitter=0.02, seed=0x5EED0001)
+    vx, ty, vp, vr = m.cells()
+    out = dict(workload="spray-simulation SIMPLE, shipped configuration (SIMPLER, DILU momentum, DIC pressure), "
+                        "%d^3 lid-driven cavity (+-2%% jitter), Re 100, urf_p 0.1" % n,
+               cells=m.n_vol, outer_iterations=K)
+
+    def big_stack():
+        import resource
+        soft, hard = resource.getrlimit(resource.RLIMIT_STACK)
+        resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
+
+    with tempfile.TemporaryDirectory() as tmp:
+        neu = os.path.join(tmp, "box.neu")
+        meshgen.write_neu(neu, 3, vx, ty, vp, vr, meshgen.box_bnd(n, n, n), np.full(len(ty), 6))
+        m.free()
+        for key, exe in (("reference_fortran", "oracle/_ref/shipped/fvk_ref_driver"),
+                         ("fortran_dropin", "oracle/_ref/dropin/fvk_ref_driver")):
+            path = os.path.join(here, exe)
+            if not os.path.exists(path):
+                log("drop-in leg: %s missing, skipped" % exe)
+                continue
+            cmd = [path, os.path.join(here, "tests", "golden", "gambit.vrt"), neu, os.path.join(tmp, key + ".bin"),
+                   str(N), "1.0", "0.01", "0.1", "31", "1.0", "0.0", "0.0"]
+            t0 = time.perf_counter()
+            p = subprocess.run(cmd, capture_output=True, text=True, cwd=tmp, timeout=600, preexec_fn=big_stack)
+            wall = time.perf_counter() - t0
+            if p.returncode != 0:
+                raise RuntimeError("%s failed: %s" % (exe, (p.stdout + p.stderr)[-800:]))
+            t = [float(ln.split()[2]) for ln in p.stdout.splitlines() if ln.startswith(" TIME17:")]
+            ms = 1e3 * sum(t[2:N]) / K
+            out[key] = dict(ms_per_outer=ms, process_s=wall)
+            log("drop-in leg %d^3 %s: %.2f ms per outer iteration (process %.1f s)" % (n, key, ms, wall))
+        m2 = fvk.Mesh.read_neu(neu)
+        ctx = fvk.grd_solv(m2)
+        fl = fvk.Flow(ctx, fvk.flow_cfg())
+        fl.set_bnd_u(31, [1.0, 0.0, 0.0])
+        fl.run(1, 2)
+        ctx.sync()
+        t0 = time.perf_counter()
+        fl.run(3, K)
+        ctx.sync()
+        out["native_gpu"] = dict(ms_per_outer=(time.perf_counter() - t0) * 1e3 / K)
+        log("drop-in leg %d^3 native: %.2f ms per outer iteration" % (n, out["native_gpu"]["ms_per_outer"]))
+        fl.destroy()
+        ctx.destroy()
+        m2.free()
+    if "reference_fortran" in out:
+        r = out["reference_fortran"]["ms_per_outer"]
+        for k in ("fortran_dropin", "native_gpu"):
+            if k in out:
+                out[k]["speedup_vs_reference_fortran"] = r / out[k]["ms_per_outer"]
+    return out
+
+
 def setup_leg(fvk, n, jitter):
     """Mesh connectivity and geometry of the n^3 box: on the host against on
     the GPU (same arrays bit for bit), and the device context build."""
@@ -604,6 +674,11 @@ def main():
     ap.add_argument("--channel-pretrain-lognormal", type=int, default=3, help="lognormal systems per outer")
     ap.add_argument("--channel-only", action="store_true", help="run the channel leg alone")
     ap.add_argument("--simple-only", action="store_true", help="run the SIMPLE leg alone (no network)")
+    ap.add_argument("--dropin", type=int, default=24,
+                    help="cells per side of the Fortran drop-in leg (0: off; the reference's own grd_conn is O(n^2) "
+                         "in the faces)")
+    ap.add_argument("--dropin-its", type=int, default=5)
+    ap.add_argument("--dropin-only", action="store_true", help="run the Fortran drop-in leg alone")
     ap.add_argument("--flowlab", type=int, default=96, help="cells per side of the flowlab CSR SIMPLE leg (0: off)")
     ap.add_argument("--flowlab-its", type=int, default=6)
     ap.add_argument("--flowlab-only", action="store_true", help="run the flowlab leg alone")
@@ -618,6 +693,9 @@ def main():
         if args.channel <= 0:
             sys.exit("--channel-only needs --channel N (triangles)")
         print(json.dumps({"channel": channel_leg(fvk, args)}), flush=True)
+        return
+    if args.dropin_only:
+        print(json.dumps({"dropin": dropin_leg(fvk, args)}), flush=True)
         return
     if args.flowlab_only:
         print(json.dumps({"flowlab": flowlab_leg(fvk, args)}), flush=True)
@@ -832,6 +910,8 @@ def main():
         out["channel"] = channel_leg(fvk, args)
     if args.flowlab > 0:
         out["flowlab"] = flowlab_leg(fvk, args)
+    if args.dropin > 0:
+        out["dropin"] = dropin_leg(fvk, args)
     if not args.no_cpu:
         port = cpu_baseline(ctx, fvk, n, args.jitter, r["iters"], args.cpu_sample)
         ref = cpu_baseline_reference(ctx, mesh, fvk, n, r["iters"], args.cpu_ref_sample)

@@ -28,6 +28,7 @@ program fvk_ref_driver
   type(phi_t),allocatable::phs(:)
   type(grd_t),pointer::gp
   integer,parameter::uo=77
+  integer(8)::t0,t1,t2,t3,crate
 
   call get_command_argument(1,vrt_file)
   call get_command_argument(2,neu_file)
@@ -157,9 +158,12 @@ program fvk_ref_driver
   end do
   bb=b3(:,1)
 
+  call system_clock(count_rate=crate)
   do iter=1,n_outer
     mx_res_n=zero; c_in=zero; c_out=zero
+    call system_clock(t0)
     call cfd_setup_eqn(1,1,1,1,iter,.true.)
+    call system_clock(t1)
     if(iter==n_outer)then
       ! momentum matrix and stand-alone solves on it (solv_*.f90)
       call put_r('mom_pl',pl)
@@ -181,9 +185,14 @@ program fvk_ref_driver
         call put_r('mom_bicgr_x'//achar(48+imat),reshape(x3,(/nv*n_dx/)))
       end do
     end if
+    call system_clock(t2)
     call cfd_setup_pp(1,1,1,iter,n_pp=1,urf=urf_p)
+    call system_clock(t3)
     hist(iter)=mx_res_n
     write(*,'(a,2i6,es26.17e3)')' REDUC17:',1,iter,mx_res_n
+    ! wall time of the iteration's cfd_setup_eqn and cfd_setup_pp (the
+    ! stand-alone solves of the last iteration excluded)
+    write(*,'(a,i6,es14.6)')' TIME17:',iter,real(t1-t0+t3-t2,wp)/real(crate,wp)
   end do
   call put_r('hist_REDUC',hist)
 

@@ -103,3 +103,55 @@ def box_records(n, jitter=0.0, seed=0x5EED0006, lid=31, wall=32):
     e_vptr = np.concatenate([[0], np.cumsum(np.concatenate([np.full(nq, 4), np.full(nh, 8)]))]).astype(np.int32)
     e_vrt = np.concatenate([quads.ravel(), hexes.ravel()]).astype(np.int32)
     return x3, e_ty, e_tag, e_vptr, e_vrt
+
+
+def box_bnd(nx, ny, nz, lid=31, wall=32):
+    """Boundary codes per (cell, local Gambit face) of fvk.Mesh.box's cells
+    (cell i + nx (j + ny k), hex faces as gambit.vrt numbers them: 1 y-, 2
+    x+, 3 y+, 4 x-, 5 z-, 6 z+; the lid is z+).  Flat, 6 per cell."""
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    b = np.zeros((nz, ny, nx, 6), np.int32)
+    b[..., 0][j == 0] = wall
+    b[..., 1][i == nx - 1] = wall
+    b[..., 2][j == ny - 1] = wall
+    b[..., 3][i == 0] = wall
+    b[..., 4][k == 0] = wall
+    b[..., 5][k == nz - 1] = lid
+    return b.reshape(-1)
+
+
+def write_neu(path, n_dx, vx, v_ty, v_vptr, v_vrt, bnd, faces_per_cell):
+    """Gambit neutral file in the fixed formats the reference's grd_read
+    expects (grd_read.f90:28,42,59,83,85): nodal coordinates (E20.12),
+    elements (7 vertices a line), one boundary-condition set per code (its
+    name from grd_read.f90:115-128's rule: 11 inlet_1, 21 outlet_1, 31
+    wall_1, ...).  v_vptr 0-based offsets, v_vrt 1-based vertex ids; bnd the
+    code of each (cell, local face), faces_per_cell[i] of them for cell i."""
+    nv = len(v_ty)
+    fptr = np.concatenate([[0], np.cumsum(faces_per_cell)]).astype(np.int64)
+    codes = sorted(set(int(c) for c in bnd if c > 0))
+    kinds = {1: "inlet", 2: "outlet", 3: "wall", 4: "symmetry", 5: "pressure"}
+    L = ["        CONTROL INFO 2.3.16", "** GAMBIT NEUTRAL FILE", "generated", "PROGRAM:  fvk", "",
+         "     NUMNP     NELEM     NGRPS    NBSETS     NDFCD     NDFVL",
+         "".join(" %9d" % v for v in (len(vx), nv, 1, len(codes), n_dx, n_dx)), "ENDOFSECTION",
+         "   NODAL COORDINATES 2.3.16"]
+    for k, x in enumerate(vx):
+        L.append("%10d " % (k + 1) + "".join("%20.12E" % x[d] for d in range(n_dx)))
+    L += ["ENDOFSECTION", "      ELEMENTS/CELLS 2.3.16"]
+    for i in range(nv):
+        vs = v_vrt[v_vptr[i]:v_vptr[i + 1]]
+        head = "%8d %2d %2d " % (i + 1, v_ty[i], len(vs))
+        first, rest = vs[:7], vs[7:]
+        L.append(head + "".join("%8d" % v for v in first))
+        while len(rest):
+            L.append(" " * 15 + "".join("%8d" % v for v in rest[:7]))
+            rest = rest[7:]
+    L.append("ENDOFSECTION")
+    for c in codes:
+        ent = [(i + 1, v_ty[i], k + 1) for i in range(nv) for k, cc in enumerate(bnd[fptr[i]:fptr[i + 1]]) if cc == c]
+        L.append(" BOUNDARY CONDITIONS 2.3.16")
+        L.append("%32s%10d%10d%10d%10d" % (kinds[c // 10] + "_%d" % (c % 10), 1, len(ent), 0, 6))
+        L += ["%10d%5d%5d" % e for e in ent]
+        L.append("ENDOFSECTION")
+    with open(path, "w") as fh:
+        fh.write("\n".join(L) + "\n")

@@ -65,43 +65,10 @@ def box_raw(nx, ny, nz, jitter, seed):
 
 
 def write_neu(path, r):
-    """Gambit neutral file in the fixed formats grd_read expects
-    (grd_read.f90:28,42,59,83,85)."""
-    n_dx, vx, ty, vptr, vrt = r["n_dx"], r["vx"], r["v_ty"], r["v_vptr"], r["v_vrt"]
-    bnd, bptr = r["bnd"], r["bnd_ptr"]
-    nv = len(ty)
-    codes = sorted(set(int(c) for c in bnd if c > 0))
-    names = {c: {1: "inlet", 2: "outlet", 3: "wall", 4: "symmetry", 5: "pressure"}[c // 10] + "_%d" % (c % 10)
-             for c in codes}
-    L = ["        CONTROL INFO 2.3.16", "** GAMBIT NEUTRAL FILE", "generated", "PROGRAM:  fvk", "",
-         "     NUMNP     NELEM     NGRPS    NBSETS     NDFCD     NDFVL",
-         "".join(" %9d" % v for v in (len(vx), nv, 1, len(codes), n_dx, n_dx)), "ENDOFSECTION",
-         "   NODAL COORDINATES 2.3.16"]
-    for k, x in enumerate(vx):
-        L.append("%10d " % (k + 1) + "".join("%20.12E" % x[d] for d in range(n_dx)))
-    L += ["ENDOFSECTION", "      ELEMENTS/CELLS 2.3.16"]
-    for i in range(nv):
-        vs = vrt[vptr[i]:vptr[i + 1]]
-        s = "%8d %2d %2d " % (i + 1, ty[i], len(vs))
-        first, rest = vs[:7], vs[7:]
-        L.append(s + "".join("%8d" % v for v in first))
-        while len(rest):
-            L.append(" " * 15 + "".join("%8d" % v for v in rest[:7]))
-            rest = rest[7:]
-    L.append("ENDOFSECTION")
-    for c in codes:
-        ent = []
-        for i in range(nv):
-            for k, cc in enumerate(bnd[bptr[i]:bptr[i + 1]]):
-                if cc == c:
-                    ent.append((i + 1, ty[i], k + 1))
-        L.append(" BOUNDARY CONDITIONS 2.3.16")
-        L.append("%32s%10d%10d%10d%10d" % (names[c], 1, len(ent), 0, 6))
-        for e in ent:
-            L.append("%10d%5d%5d" % e)
-        L.append("ENDOFSECTION")
-    with open(path, "w") as fh:
-        fh.write("\n".join(L) + "\n")
+    """Gambit neutral file of a raw mesh record (programs_amd.meshgen.write_neu:
+    the fixed formats grd_read expects, grd_read.f90:28,42,59,83,85)."""
+    from programs_amd import meshgen
+    meshgen.write_neu(path, r["n_dx"], r["vx"], r["v_ty"], r["v_vptr"], r["v_vrt"], r["bnd"], np.diff(r["bnd_ptr"]))
 
 
 def run_case(name, spec, tmp):
