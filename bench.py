@@ -416,7 +416,10 @@ def channel_leg(fvk, args):
     log("channel: %d triangles, %d faces (%.1f s)" % (m.n_vol, m.n_fce, time.time() - t0))
     out, fields = {}, {}
     warm, its = args.channel_warm, args.channel_its
-    for name, ps in (("plain_pcg64", 3), ("dc32_learned", 2)):
+    seg = args.channel_seg if args.channel_seg > 0 else its
+    methods = [(n_, p_) for n_, p_ in (("plain_pcg64", 3), ("dc32_learned", 2)) if n_ in args.channel_methods.split(",")]
+    dump = {}
+    for name, ps in methods:
         cfg = fvk.flow_cfg(simpler=0, imat_u=0, imat_p=0, urf_p=0.3, mu=0.002, p_solver=ps, p_tol=args.tol)
         fl = fvk.Flow(c, cfg)
         fl.set_bnd_u(11, [1.0, 0.0])
@@ -428,13 +431,26 @@ def channel_leg(fvk, args):
             log("channel %s: untimed outer iteration %d, p' iterations %d" % (name, 1 + k, h0[-1]["it_pp"]))
         if ps == 2:
             fl.set_model(model, train_steps=0)
+        # the timed iterations in segments of seg replays (a log line each)
         c.sync()
         t1 = time.perf_counter()
-        h = fl.run_graph(1 + warm, its)
+        h = []
+        while len(h) < its:
+            h += fl.run_graph(1 + warm + len(h), min(seg, its - len(h)))
+            if len(h) < its:
+                c.sync()
+                log("channel %s: %d of %d timed outer iterations, %.1f s, p' iterations %d..%d"
+                    % (name, len(h), its, time.perf_counter() - t1, min(x["it_pp"] for x in h),
+                       max(x["it_pp"] for x in h)))
         c.sync()
         dt = (time.perf_counter() - t1) / its
         st = fl.state()
         fields[name] = {k: st[k] for k in ("u_cell", "p")}
+        if args.channel_dump:
+            dump[name + "_u_sample"] = st["u_cell"][::97]
+            dump[name + "_p_sample"] = st["p"][::97]
+            dump[name + "_p_iters"] = np.array([x["it_pp"] for x in h0 + h])
+            dump[name + "_mx_res_n"] = np.array([x["mx_res_n"] for x in h0 + h])
         out["pressure_to_tol_" + name] = dict(ms_per_outer=dt * 1e3, outer_its_per_s=1.0 / dt,
                                               p_iters_untimed=[x["it_pp"] for x in h0],
                                               p_iters=[x["it_pp"] for x in h])
@@ -443,10 +459,13 @@ def channel_leg(fvk, args):
                out["pressure_to_tol_" + name]["p_iters_untimed"]))
         fl.destroy()
     model.destroy()
-    a, b = fields["plain_pcg64"], fields["dc32_learned"]
-    out["field_rel_diff"] = {k: float(np.max(np.abs(a[k] - b[k])) / np.max(np.abs(a[k]))) for k in a}
-    out["speedup_vs_plain"] = (out["pressure_to_tol_plain_pcg64"]["ms_per_outer"]
-                               / out["pressure_to_tol_dc32_learned"]["ms_per_outer"])
+    if args.channel_dump:
+        np.savez_compressed(args.channel_dump, **dump)
+    if len(fields) == 2:
+        a, b = fields["plain_pcg64"], fields["dc32_learned"]
+        out["field_rel_diff"] = {k: float(np.max(np.abs(a[k] - b[k])) / np.max(np.abs(a[k]))) for k in a}
+        out["speedup_vs_plain"] = (out["pressure_to_tol_plain_pcg64"]["ms_per_outer"]
+                                   / out["pressure_to_tol_dc32_learned"]["ms_per_outer"])
     c.destroy()
     m.free()
     out["training"] = dict(seconds=train_s, triangles=args.channel_train_tri, simple=args.channel_pretrain,
@@ -673,6 +692,12 @@ def main():
     ap.add_argument("--channel-pretrain", type=int, default=300, help="SIMPLE outer iterations of the training run")
     ap.add_argument("--channel-pretrain-lognormal", type=int, default=3, help="lognormal systems per outer")
     ap.add_argument("--channel-only", action="store_true", help="run the channel leg alone")
+    ap.add_argument("--channel-methods", default="plain_pcg64,dc32_learned",
+                    help="pressure solvers of the channel leg (comma list)")
+    ap.add_argument("--channel-seg", type=int, default=0,
+                    help="timed outer iterations per graph replay call (0: all in one), a log line after each")
+    ap.add_argument("--channel-dump", default="",
+                    help="npz of every 97th cell's u and p and the residual histories of each method")
     ap.add_argument("--simple-only", action="store_true", help="run the SIMPLE leg alone (no network)")
     ap.add_argument("--dropin", type=int, default=24,
                     help="cells per side of the Fortran drop-in leg (0: off; the reference's own grd_conn is O(n^2) "
