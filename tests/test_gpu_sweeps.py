@@ -94,6 +94,37 @@ def test_factor_and_sweeps_one_launch_bitwise(n, rcm):
     m.free()
 
 
+@pytest.mark.parametrize("rcm", [False, True])
+def test_sweeps_on_tetrahedra_bitwise(rcm, tmp_path):
+    """flowlab's duct_3d.msh read as a spray-simulation mesh (24,258 cells,
+    up to 5 lower neighbours a row: the kernels that hold 8 slots) in every
+    form against one launch per level."""
+    import gzip
+    import os
+    import shutil
+    fvk = require_lib()
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    p = os.path.join(str(tmp_path), "duct_3d.msh")
+    with gzip.open(os.path.join(golden, "duct_3d.msh.gz")) as a, open(p, "wb") as b:
+        shutil.copyfileobj(a, b)
+    say("sweeps: duct_3d rcm %s" % rcm)
+    m = fvk.Mesh.read_gmsh(p, 3)
+    ctx = fvk.grd_solv(m, rcm=rcm)
+    pl, ngp, ngn = random_matrix(m, 17)
+    fvk.set_matrix(ctx, pl, ngp, ngn)
+    b = np.random.default_rng(5).standard_normal(m.n_vol)
+    for imat in (1, 2):
+        f = all_modes(fvk, ctx, lambda: fvk.solv_precond(ctx, imat))
+        for label, (c1, cr1) in f[1:]:
+            assert np.array_equal(f[0][1][0], c1) and np.array_equal(f[0][1][1], cr1), ("factor", imat, label)
+        x = all_modes(fvk, ctx, lambda: fvk.solv_direct(ctx, imat, b))
+        for label, x1 in x[1:]:
+            assert np.array_equal(x[0][1], x1), ("sweeps", imat, label)
+        say("  imat %d: factor and sweeps bitwise, levels %s" % (imat, ctx.levels()))
+    ctx.destroy()
+    m.free()
+
+
 def test_krylov_on_one_launch_sweeps_bitwise():
     fvk = require_lib()
     n = 48
