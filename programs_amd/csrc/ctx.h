@@ -103,7 +103,7 @@ struct Ctx {
   // sweep_part_rows rows consecutive in the reference numbering, each walked
   // level by level by one block
   struct SweepParts {
-    int P = 0, R = 0, W = 0;
+    int P = 0, R = 0, W = 0, wmax = 0;  // wmax: the most rows of one level of a range
     int *rows = nullptr, *first = nullptr, *cnt = nullptr, *aux = nullptr, *lrow = nullptr, *cols = nullptr,
         *lcol = nullptr, *step = nullptr, *sp = nullptr;
   };

@@ -62,7 +62,7 @@ struct FlCtx {
   // order of each row range, for ranges of R1 rows (one component, the
   // factor) and R3 rows (two or three components)
   struct Parts {
-    int P = 0, R = 0, W = 0;
+    int P = 0, R = 0, W = 0, wmax = 0;  // wmax: the most rows of one level of a range
     int *rows = nullptr, *first = nullptr, *cnt = nullptr, *aux = nullptr, *lrow = nullptr, *cols = nullptr,
         *lcol = nullptr;
     int *step = nullptr, *sp = nullptr;
@@ -783,24 +783,24 @@ struct FlPtPc {
   __device__ void done(const SweepDesc &) {}
 };
 
-template <int MW>
-__global__ __launch_bounds__(BS) void k_fl_pc_pt(int n, int R, int P, SweepTabs T, const int *__restrict__ step,
+template <int MW, int TB>
+__global__ __launch_bounds__(TB) void k_fl_pc_pt(int n, int R, int P, SweepTabs T, const int *__restrict__ step,
                                                  const int *__restrict__ sp, const int *__restrict__ tpos,
                                                  const double *__restrict__ a, double *pc, int *ticket, int *err) {
   FlPtPc row;
   row.a = a; row.tpos = tpos;
-  sweep_parts<1, MW>(n, R, P, T, step, sp, pc, ticket, err, row);
+  sweep_parts<1, MW, FlPtPc, TB>(n, R, P, T, step, sp, pc, ticket, err, row);
 }
 
-template <bool FWD, int NC, int MW>
-__global__ __launch_bounds__(BS) void k_fl_sweep_pt(int n, int R, int P, SweepTabs T, const int *__restrict__ step,
+template <bool FWD, int NC, int MW, int TB>
+__global__ __launch_bounds__(TB) void k_fl_sweep_pt(int n, int R, int P, SweepTabs T, const int *__restrict__ step,
                                                     const int *__restrict__ sp, const double *__restrict__ a,
                                                     const double *__restrict__ pc, const double *__restrict__ src,
                                                     double *out, int *ticket, int *err, const FlKS *ks) {
   if (fl_all_done(ks, NC)) return;
   FlPtSweep<FWD, NC> row;
   row.a = a; row.pc = pc; row.src = src; row.n = n;
-  sweep_parts<NC, MW>(n, R, P, T, step, sp, out, ticket, err, row);
+  sweep_parts<NC, MW, FlPtSweep<FWD, NC>, TB>(n, R, P, T, step, sp, out, ticket, err, row);
 }
 
 // ---------------------------------------------------------------- host
@@ -828,6 +828,9 @@ static int fl_parts_build(FlCtx &c, const std::vector<int> &lev, bool fwd, int R
     step.push_back((int)rows.size());
   }
   sp.push_back((int)step.size());
+  pt.wmax = 0;
+  for (int q = 0; q < pt.P; ++q)
+    for (int k = sp[q]; k + 1 < sp[q + 1]; ++k) pt.wmax = std::max(pt.wmax, step[k + 1] - step[k]);
   int W = 1;
   for (int p = 0; p < n; ++p) {
     const int r = rows[p];
@@ -873,13 +876,13 @@ static int fl_parts_setup(FlCtx &c) {
   if (rc) return rc;
   const int bytes = 3 * c.part_rows * 8;
   if (bytes > 65536) {
-    const void *k[] = {(const void *)k_fl_pc_pt<4>, (const void *)k_fl_pc_pt<8>,
-                       (const void *)k_fl_sweep_pt<true, 1, 4>, (const void *)k_fl_sweep_pt<false, 1, 4>,
-                       (const void *)k_fl_sweep_pt<true, 2, 4>, (const void *)k_fl_sweep_pt<false, 2, 4>,
-                       (const void *)k_fl_sweep_pt<true, 3, 4>, (const void *)k_fl_sweep_pt<false, 3, 4>,
-                       (const void *)k_fl_sweep_pt<true, 1, 8>, (const void *)k_fl_sweep_pt<false, 1, 8>,
-                       (const void *)k_fl_sweep_pt<true, 2, 8>, (const void *)k_fl_sweep_pt<false, 2, 8>,
-                       (const void *)k_fl_sweep_pt<true, 3, 8>, (const void *)k_fl_sweep_pt<false, 3, 8>};
+    const void *k[] = {
+#define K_(D, C, MW) (const void *)k_fl_sweep_pt<D, C, MW, 64>, (const void *)k_fl_sweep_pt<D, C, MW, BS>
+#define K2_(C, MW) K_(true, C, MW), K_(false, C, MW)
+        K2_(1, 4), K2_(2, 4), K2_(3, 4), K2_(1, 8), K2_(2, 8), K2_(3, 8)
+#undef K2_
+#undef K_
+    };
     for (const void *f : k) FVK_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
   }
   return 0;
@@ -914,22 +917,52 @@ static int fl_sweep_grid(const FlCtx &c) {
 static int fl_mark_grid(size_t cnt) { return (int)std::max<size_t>(1, std::min<size_t>((cnt + BS - 1) / BS, 4096)); }
 
 // blocks of the partitioned kernels (sweep_blocks 0: as many per CU as the
-// LDS of a block lets reside, at most two)
-static int fl_pt_grid(const FlCtx &c, const FlCtx::Parts &pt, size_t lds) {
-  const int per_cu = std::max(1, std::min<int>(2, (int)(163840 / (lds + 1024))));
+// LDS of a block lets reside, at most four of 64 threads or two of 256)
+static int fl_pt_grid(const FlCtx &c, const FlCtx::Parts &pt, size_t lds, int tb) {
+  const int per_cu = std::max(1, std::min<int>(tb == 64 ? 4 : 2, (int)(163840 / (lds + 1024))));
   return std::max(1, std::min(pt.P, c.sweep_blocks > 0 ? c.sweep_blocks : c.n_cu * per_cu));
+}
+
+// one partitioned sweep: one wave a block when no level of a range has more
+// than 64 rows
+template <bool FWD, int C, int MW>
+static void fl_sweep_launch(FlCtx &c, const FlCtx::Parts &X, size_t lds, const double *a, const double *src,
+                            double *out, int *ticket, const FlKS *ks) {
+  if (X.wmax <= 64)
+    k_fl_sweep_pt<FWD, C, MW, 64><<<fl_pt_grid(c, X, lds, 64), 64, lds, c.stream>>>(
+        c.n, X.R, X.P, X.tabs(), X.step, X.sp, a, c.pc, src, out, ticket, c.sweep_int + 3, ks);
+  else
+    k_fl_sweep_pt<FWD, C, MW, BS><<<fl_pt_grid(c, X, lds, BS), BS, lds, c.stream>>>(
+        c.n, X.R, X.P, X.tabs(), X.step, X.sp, a, c.pc, src, out, ticket, c.sweep_int + 3, ks);
+}
+
+template <bool FWD>
+static void fl_sweep_any(FlCtx &c, int ncmp, const FlCtx::Parts &X, size_t lds, const double *a, const double *src,
+                         double *out, int *ticket, const FlKS *ks) {
+  if (X.W <= 4) {
+    if (ncmp == 1) fl_sweep_launch<FWD, 1, 4>(c, X, lds, a, src, out, ticket, ks);
+    else if (ncmp == 2) fl_sweep_launch<FWD, 2, 4>(c, X, lds, a, src, out, ticket, ks);
+    else fl_sweep_launch<FWD, 3, 4>(c, X, lds, a, src, out, ticket, ks);
+  } else {
+    if (ncmp == 1) fl_sweep_launch<FWD, 1, 8>(c, X, lds, a, src, out, ticket, ks);
+    else if (ncmp == 2) fl_sweep_launch<FWD, 2, 8>(c, X, lds, a, src, out, ticket, ks);
+    else fl_sweep_launch<FWD, 3, 8>(c, X, lds, a, src, out, ticket, ks);
+  }
 }
 
 static int fl_precond(FlCtx &c, const double *a) {
   if (c.one_launch && c.parts) {
     const FlCtx::Parts &P = c.pf;
+    const size_t lds = (size_t)P.R * 8;
     k_fl_mark<<<fl_mark_grid(c.n), BS, 0, c.stream>>>(c.n, c.pc, c.sweep_int + 2, nullptr, 1);
-    if (P.W <= 4)
-      k_fl_pc_pt<4><<<fl_pt_grid(c, P, (size_t)P.R * 8), BS, (size_t)P.R * 8, c.stream>>>(c.n, P.R, P.P, P.tabs(), P.step, P.sp, c.tpos,
-                                                                        a, c.pc, c.sweep_int + 2, c.sweep_int + 3);
-    else
-      k_fl_pc_pt<8><<<fl_pt_grid(c, P, (size_t)P.R * 8), BS, (size_t)P.R * 8, c.stream>>>(c.n, P.R, P.P, P.tabs(), P.step, P.sp, c.tpos,
-                                                                        a, c.pc, c.sweep_int + 2, c.sweep_int + 3);
+#define PC_(MW, TB) k_fl_pc_pt<MW, TB><<<fl_pt_grid(c, P, lds, TB), TB, lds, c.stream>>>(c.n, P.R, P.P, P.tabs(), \
+    P.step, P.sp, c.tpos, a, c.pc, c.sweep_int + 2, c.sweep_int + 3)
+    if (P.W <= 4) {
+      if (P.wmax <= 64) PC_(4, 64); else PC_(4, BS);
+    } else {
+      if (P.wmax <= 64) PC_(8, 64); else PC_(8, BS);
+    }
+#undef PC_
     FVK_HIP(hipGetLastError());
     return 0;
   }
@@ -953,24 +986,11 @@ static int fl_precond(FlCtx &c, const double *a) {
 static int fl_direct(FlCtx &c, int ncmp, const double *a, const double *b, double *t, double *x, const FlKS *ks) {
   if (c.one_launch && c.parts) {
     const size_t cnt = (size_t)ncmp * c.n;
-    const FlCtx::Parts &F = c.pf, &B = c.pb;
     const size_t lds = (size_t)ncmp * c.part_rows * 8;
     k_fl_mark<<<fl_mark_grid(cnt), BS, 0, c.stream>>>(cnt, t, c.sweep_int, ks, ncmp);
-#define SW(D, C, MW, X, S, O, TK) k_fl_sweep_pt<D, C, MW><<<fl_pt_grid(c, X, lds), BS, lds, c.stream>>>(c.n, X.R, X.P, \
-    X.tabs(), X.step, X.sp, a, c.pc, S, O, TK, c.sweep_int + 3, ks)
-#define SWC(D, X, S, O, TK)                                                              \
-    if (X.W <= 4) {                                                                      \
-      if (ncmp == 1) SW(D, 1, 4, X, S, O, TK); else if (ncmp == 2) SW(D, 2, 4, X, S, O, TK); \
-      else SW(D, 3, 4, X, S, O, TK);                                                     \
-    } else {                                                                             \
-      if (ncmp == 1) SW(D, 1, 8, X, S, O, TK); else if (ncmp == 2) SW(D, 2, 8, X, S, O, TK); \
-      else SW(D, 3, 8, X, S, O, TK);                                                     \
-    }
-    SWC(true, F, b, t, c.sweep_int)
+    fl_sweep_any<true>(c, ncmp, c.pf, lds, a, b, t, c.sweep_int, ks);
     k_fl_mark<<<fl_mark_grid(cnt), BS, 0, c.stream>>>(cnt, x, c.sweep_int + 1, ks, ncmp);
-    SWC(false, B, t, x, c.sweep_int + 1)
-#undef SWC
-#undef SW
+    fl_sweep_any<false>(c, ncmp, c.pb, lds, a, t, x, c.sweep_int + 1, ks);
     FVK_HIP(hipGetLastError());
     return 0;
   }

@@ -396,19 +396,19 @@ struct PtSweep {
   __device__ void done(const SweepDesc &) {}
 };
 
-template <int MW>
-__global__ __launch_bounds__(BS) void k_precond_pt(int n, int R, int P, SweepTabs T, const int *__restrict__ step,
+template <int MW, int TB>
+__global__ __launch_bounds__(TB) void k_precond_pt(int n, int R, int P, SweepTabs T, const int *__restrict__ step,
                                                    const int *__restrict__ sp, DMesh M, int imat,
                                                    const double *__restrict__ diag, const double *__restrict__ ngp,
                                                    const double *__restrict__ ngn, double *c, double *__restrict__ cr,
                                                    int *ticket, int *err) {
   PtPrecond row;
   row.spf = M.spf; row.ld = M.ld; row.imat = imat; row.diag = diag; row.ngp = ngp; row.ngn = ngn; row.cr = cr;
-  sweep_parts<1, MW>(n, R, P, T, step, sp, c, ticket, err, row);
+  sweep_parts<1, MW, PtPrecond, TB>(n, R, P, T, step, sp, c, ticket, err, row);
 }
 
-template <bool FWD, int NVC, int MW>
-__global__ __launch_bounds__(BS) void k_sweep_pt(int n, int R, int P, SweepTabs T, const int *__restrict__ step,
+template <bool FWD, int NVC, int MW, int TB>
+__global__ __launch_bounds__(TB) void k_sweep_pt(int n, int R, int P, SweepTabs T, const int *__restrict__ step,
                                                  const int *__restrict__ sp, size_t ld, const double *__restrict__ spv,
                                                  const double *__restrict__ c, const double *__restrict__ cr,
                                                  const double *__restrict__ src, double *out, int *ticket, int *err,
@@ -416,7 +416,7 @@ __global__ __launch_bounds__(BS) void k_sweep_pt(int n, int R, int P, SweepTabs 
   if (all_done(ks, NVC)) return;
   PtSweep<FWD, NVC> row;
   row.spv = spv; row.c = c; row.cr = cr; row.src = src; row.ld = ld; row.n = n;
-  sweep_parts<NVC, MW>(n, R, P, T, step, sp, out, ticket, err, row);
+  sweep_parts<NVC, MW, PtSweep<FWD, NVC>, TB>(n, R, P, T, step, sp, out, ticket, err, row);
 }
 
 template <int WS>
@@ -940,6 +940,9 @@ static int parts_build(Ctx &c, bool fwd, const std::vector<int> &rank, const std
     step.push_back((int)rows.size());
   }
   sp.push_back((int)step.size());
+  pt.wmax = 0;
+  for (int q = 0; q < pt.P; ++q)
+    for (int k = sp[q]; k + 1 < sp[q + 1]; ++k) pt.wmax = std::max(pt.wmax, step[k + 1] - step[k]);
   int W = 1;
   for (int p = 0; p < n; ++p) {
     first[p] = mask[rows[p]];
@@ -993,12 +996,13 @@ int sweep_parts_build(Ctx &c) {
   if (rc) return rc;
   const int bytes = 3 * c.sweep_part_rows * 8;
   if (bytes > 65536) {
-    const void *k[] = {(const void *)k_sweep_pt<true, 1, 4>, (const void *)k_sweep_pt<false, 1, 4>,
-                       (const void *)k_sweep_pt<true, 2, 4>, (const void *)k_sweep_pt<false, 2, 4>,
-                       (const void *)k_sweep_pt<true, 3, 4>, (const void *)k_sweep_pt<false, 3, 4>,
-                       (const void *)k_sweep_pt<true, 1, 8>, (const void *)k_sweep_pt<false, 1, 8>,
-                       (const void *)k_sweep_pt<true, 2, 8>, (const void *)k_sweep_pt<false, 2, 8>,
-                       (const void *)k_sweep_pt<true, 3, 8>, (const void *)k_sweep_pt<false, 3, 8>};
+    const void *k[] = {
+#define K_(D, V, MW) (const void *)k_sweep_pt<D, V, MW, 64>, (const void *)k_sweep_pt<D, V, MW, BS>
+#define K2_(V, MW) K_(true, V, MW), K_(false, V, MW)
+        K2_(1, 4), K2_(2, 4), K2_(3, 4), K2_(1, 8), K2_(2, 8), K2_(3, 8)
+#undef K2_
+#undef K_
+    };
     for (const void *f : k) FVK_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
   }
   return 0;
@@ -1008,11 +1012,39 @@ static SweepTabs tabs_of(const Ctx::SweepParts &p) {
   return SweepTabs{p.rows, p.first, p.cnt, p.aux, p.lrow, p.cols, p.lcol, p.W};
 }
 // blocks of the partitioned kernels: as many per CU as the LDS of a block
-// lets reside, at most two (measured: two blocks of 32 KiB per CU beat one;
-// the ranges a grid cannot hold at once wait for a block to finish)
-static int pt_grid(const Ctx &c, const Ctx::SweepParts &p, size_t lds) {
-  const int per_cu = std::max(1, std::min<int>(2, (int)(163840 / (lds + 1024))));
+// lets reside, at most four of 64 threads or two of 256 (the ranges a grid
+// cannot hold at once wait for a block to finish)
+static int pt_grid(const Ctx &c, const Ctx::SweepParts &p, size_t lds, int tb) {
+  const int per_cu = std::max(1, std::min<int>(tb == 64 ? 4 : 2, (int)(163840 / (lds + 1024))));
   return std::max(1, std::min(p.P, c.n_cu * per_cu));
+}
+
+// one partitioned sweep: one wave a block when no level of a range has more
+// than 64 rows
+template <bool FWD, int V, int MW>
+static void sweep_pt_launch(Ctx &c, const Ctx::SweepParts &X, size_t lds, const double *cr, const double *src,
+                            double *out, int *ticket, const KS *gate) {
+  const size_t ld = (size_t)c.M.ld;
+  if (X.wmax <= 64)
+    k_sweep_pt<FWD, V, MW, 64><<<pt_grid(c, X, lds, 64), 64, lds, c.stream>>>(
+        c.n, X.R, X.P, tabs_of(X), X.step, X.sp, ld, c.spv, c.pc_c, cr, src, out, ticket, c.sweep_int + 3, gate);
+  else
+    k_sweep_pt<FWD, V, MW, BS><<<pt_grid(c, X, lds, BS), BS, lds, c.stream>>>(
+        c.n, X.R, X.P, tabs_of(X), X.step, X.sp, ld, c.spv, c.pc_c, cr, src, out, ticket, c.sweep_int + 3, gate);
+}
+
+template <bool FWD>
+static void sweep_pt_any(Ctx &c, int nvc, const Ctx::SweepParts &X, size_t lds, const double *cr, const double *src,
+                         double *out, int *ticket, const KS *gate) {
+  if (X.W <= 4) {
+    if (nvc == 1) sweep_pt_launch<FWD, 1, 4>(c, X, lds, cr, src, out, ticket, gate);
+    else if (nvc == 2) sweep_pt_launch<FWD, 2, 4>(c, X, lds, cr, src, out, ticket, gate);
+    else sweep_pt_launch<FWD, 3, 4>(c, X, lds, cr, src, out, ticket, gate);
+  } else {
+    if (nvc == 1) sweep_pt_launch<FWD, 1, 8>(c, X, lds, cr, src, out, ticket, gate);
+    else if (nvc == 2) sweep_pt_launch<FWD, 2, 8>(c, X, lds, cr, src, out, ticket, gate);
+    else sweep_pt_launch<FWD, 3, 8>(c, X, lds, cr, src, out, ticket, gate);
+  }
 }
 
 int precond_build(Ctx &c, int imat) {
@@ -1021,15 +1053,16 @@ int precond_build(Ctx &c, int imat) {
     k_precond_jacobi<<<grid_for(n), BS, 0, c.stream>>>(n, c.diag, c.pc_c, c.pc_cr);
   } else if (c.sweep_one_launch && c.sweep_parts) {
     const Ctx::SweepParts &P = c.spf;
+    const size_t lds = (size_t)P.R * 8;
     k_sweep_mark<<<mark_grid(n), BS, 0, c.stream>>>(n, c.pc_c, c.sweep_int + 2, nullptr, 1);
-    if (P.W <= 4)
-      k_precond_pt<4><<<pt_grid(c, P, (size_t)P.R * 8), BS, (size_t)P.R * 8, c.stream>>>(n, P.R, P.P, tabs_of(P), P.step, P.sp, c.M,
-                                                                          imat, c.diag, c.ngp, c.ngn, c.pc_c,
-                                                                          c.pc_cr, c.sweep_int + 2, c.sweep_int + 3);
-    else
-      k_precond_pt<8><<<pt_grid(c, P, (size_t)P.R * 8), BS, (size_t)P.R * 8, c.stream>>>(n, P.R, P.P, tabs_of(P), P.step, P.sp, c.M,
-                                                                          imat, c.diag, c.ngp, c.ngn, c.pc_c,
-                                                                          c.pc_cr, c.sweep_int + 2, c.sweep_int + 3);
+#define PC_(MW, TB) k_precond_pt<MW, TB><<<pt_grid(c, P, lds, TB), TB, lds, c.stream>>>(n, P.R, P.P, tabs_of(P), \
+    P.step, P.sp, c.M, imat, c.diag, c.ngp, c.ngn, c.pc_c, c.pc_cr, c.sweep_int + 2, c.sweep_int + 3)
+    if (P.W <= 4) {
+      if (P.wmax <= 64) PC_(4, 64); else PC_(4, BS);
+    } else {
+      if (P.wmax <= 64) PC_(8, 64); else PC_(8, BS);
+    }
+#undef PC_
   } else if (c.sweep_one_launch) {
     k_sweep_mark<<<mark_grid(n), BS, 0, c.stream>>>(n, c.pc_c, c.sweep_int + 2, nullptr, 1);
     k_precond_sf<<<sweep_grid(c), BS, 0, c.stream>>>(c.M, c.lf.rows, c.sweep_lmask, imat, c.diag, c.ngp, c.ngn,
@@ -1053,24 +1086,11 @@ int precond_apply(Ctx &c, int imat, int nvc, const double *b, double *x, const K
     k_jacobi_apply<<<grid_for(n), BS, 0, c.stream>>>(n, nvc, c.pc_c, b, x, gate);
   } else if (c.sweep_one_launch && c.sweep_parts) {
     const size_t cnt = (size_t)nvc * n;
-    const Ctx::SweepParts &F = c.spf, &B = c.spb;
-    const size_t lds = (size_t)nvc * c.sweep_part_rows * 8, ld = (size_t)c.M.ld;
+    const size_t lds = (size_t)nvc * c.sweep_part_rows * 8;
     k_sweep_mark<<<mark_grid(cnt), BS, 0, c.stream>>>(cnt, c.sweep_t, c.sweep_int, gate, nvc);
-#define SW(D, V, MW, X, CR, S, O, TK) k_sweep_pt<D, V, MW><<<pt_grid(c, X, lds), BS, lds, c.stream>>>(n, X.R, X.P, \
-    tabs_of(X), X.step, X.sp, ld, c.spv, c.pc_c, CR, S, O, TK, c.sweep_int + 3, gate)
-#define SWC(D, X, CR, S, O, TK)                                                                          \
-    if (X.W <= 4) {                                                                                      \
-      if (nvc == 1) SW(D, 1, 4, X, CR, S, O, TK); else if (nvc == 2) SW(D, 2, 4, X, CR, S, O, TK);       \
-      else SW(D, 3, 4, X, CR, S, O, TK);                                                                 \
-    } else {                                                                                             \
-      if (nvc == 1) SW(D, 1, 8, X, CR, S, O, TK); else if (nvc == 2) SW(D, 2, 8, X, CR, S, O, TK);       \
-      else SW(D, 3, 8, X, CR, S, O, TK);                                                                 \
-    }
-    SWC(true, F, nullptr, b, c.sweep_t, c.sweep_int)
+    sweep_pt_any<true>(c, nvc, c.spf, lds, nullptr, b, c.sweep_t, c.sweep_int, gate);
     k_sweep_mark<<<mark_grid(cnt), BS, 0, c.stream>>>(cnt, x, c.sweep_int + 1, gate, nvc);
-    SWC(false, B, c.pc_cr, c.sweep_t, x, c.sweep_int + 1)
-#undef SWC
-#undef SW
+    sweep_pt_any<false>(c, nvc, c.spb, lds, c.pc_cr, c.sweep_t, x, c.sweep_int + 1, gate);
   } else if (c.sweep_one_launch) {
     const size_t cnt = (size_t)nvc * n;
     k_sweep_mark<<<mark_grid(cnt), BS, 0, c.stream>>>(cnt, c.sweep_t, c.sweep_int, gate, nvc);

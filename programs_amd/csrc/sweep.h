@@ -78,18 +78,20 @@ constexpr int kSweepSlots = 8;
 // Each position p has a descriptor: its row, a row-specific first-entry
 // word, the entry count, one more row-specific index, the row's own LDS
 // slot (its index in the range), and per entry (slot-major, W slots of n)
-// the column and the column's LDS slot (-1: another range).  The first BS positions of each level are
-// software-pipelined: the descriptors two levels ahead, the row's
-// coefficients and the values from other ranges one level ahead, all loads
+// the column and the column's LDS slot (-1: another range).  The first TB
+// positions of each level are software-pipelined (descriptors four levels
+// ahead and the row's coefficients and values from other ranges two levels
+// ahead where three stages fit in registers, else two and one), all loads
 // issued without branches (a lane past the level's end loads the level's
 // last position and is not run), so a level waits for memory only when a
 // value from another range has not yet arrived.
 //
 // Row interface: fetch(d, MW) loads the row's coefficients (entries t < MW,
-// entries t >= d.nl from a valid address, unused); begin() does the arithmetic on them that precedes the
-// entries; take(t, v) adds entry t with the NC values v[] (called in entry
-// order, t < d.nl); result(c) is component c of the row's value; done(d)
-// runs after the row's value is stored.
+// entries t >= d.nl from a valid address, unused); begin() does the
+// arithmetic on them that precedes the entries; take(t, v) adds entry t with
+// the NC values v[] (called in entry order, t < d.nl); result(c) is
+// component c of the row's value; done(d) runs after the row's value is
+// stored.
 struct SweepTabs {
   const int *rows, *first, *cnt, *aux, *lrow, *cols, *lcol;
   int W;
@@ -209,10 +211,14 @@ struct SweepStage {
   unsigned long long v[kSweepSlots][NC];
 };
 
-template <int NC, int MW, class Row>
+template <int NC, int MW, class Row, int TB = BS>
 __device__ __forceinline__ void sweep_parts(int n, int R, int P, const SweepTabs &T,
                                             const int *__restrict__ step, const int *__restrict__ sp, double *out,
                                             int *ticket, int *err, const Row &proto) {
+  // two levels of prefetch on both links (descriptors four levels ahead,
+  // coefficients and values from other ranges two) where the three stages
+  // fit in registers, else one
+  constexpr bool deep = NC * MW <= 12;
   extern __shared__ double lv[];
   __shared__ int s_q, s_abort;
   if (threadIdx.x == 0) s_abort = 0;
@@ -235,11 +241,11 @@ __device__ __forceinline__ void sweep_parts(int n, int R, int P, const SweepTabs
       S.row.fetch(S.d, MW);
       sweep_ext<NC, MW>(S.d, n, out, S.v);
     };
-    // positions of level k beyond the first BS, not pipelined
+    // positions of level k beyond the first TB, not pipelined
     auto rest = [&](int k) -> bool {
       const int e = ld_uniform(step, k + 1);
       bool ok = true;
-      for (int p = ld_uniform(step, k) + tid + BS; ok && p < e; p += BS) {
+      for (int p = ld_uniform(step, k) + tid + TB; ok && p < e; p += TB) {
         SweepDesc d;
         SweepStage<NC, Row> Z;
         Z.row = proto;
@@ -249,33 +255,52 @@ __device__ __forceinline__ void sweep_parts(int n, int R, int P, const SweepTabs
       }
       return ok;
     };
-    SweepStage<NC, Row> X, Y;
-    X.row = proto;
-    Y.row = proto;
-    SweepDesc D;
-    desc(k0, D);
-    stage(D, X);
-    desc(k0 + 1, D);
     bool ok = true;
-    for (int k = k0; k < k1; k += 2) {
-      // level k in X; level k+1 into Y; descriptors of level k+2
-      stage(D, Y);
-      desc(k + 2, D);
-      if (ok && X.d.nl >= 0) ok = sweep_row<NC, MW>(X.d, X.row, X.v, R, n, out, lv, err);
-      if (ok) ok = rest(k);
-      if (!ok) s_abort = 1;
-      __syncthreads();
-      if (s_abort) return;
-      if (k + 1 >= k1) break;
-      // level k+1 in Y; level k+2 into X; descriptors of level k+3
+    // one level: stage NEXT from the descriptors in DB, load the descriptors
+    // of level K + AHEAD into DB, run RUN (level K), barrier
+#define SWEEP_LEVEL(RUN, NEXT, DB, K, AHEAD)                                                           \
+    stage(DB, NEXT);                                                                                   \
+    desc((K) + (AHEAD), DB);                                                                           \
+    if (ok && RUN.d.nl >= 0) ok = sweep_row<NC, MW>(RUN.d, RUN.row, RUN.v, R, n, out, lv, err);        \
+    if (ok) ok = rest(K);                                                                              \
+    if (!ok) s_abort = 1;                                                                              \
+    __syncthreads();                                                                                   \
+    if (s_abort) return;                                                                               \
+    if ((K) + 1 >= k1) break;
+    if constexpr (deep) {
+      SweepStage<NC, Row> S0, S1, S2;
+      S0.row = proto;
+      S1.row = proto;
+      S2.row = proto;
+      SweepDesc D0, D1;
+      desc(k0, D0);
+      stage(D0, S0);
+      desc(k0 + 1, D1);
+      stage(D1, S1);
+      desc(k0 + 2, D0);
+      desc(k0 + 3, D1);
+      for (int k = k0; k < k1; k += 6) {
+        SWEEP_LEVEL(S0, S2, D0, k, 4)
+        SWEEP_LEVEL(S1, S0, D1, k + 1, 4)
+        SWEEP_LEVEL(S2, S1, D0, k + 2, 4)
+        SWEEP_LEVEL(S0, S2, D1, k + 3, 4)
+        SWEEP_LEVEL(S1, S0, D0, k + 4, 4)
+        SWEEP_LEVEL(S2, S1, D1, k + 5, 4)
+      }
+    } else {
+      SweepStage<NC, Row> X, Y;
+      X.row = proto;
+      Y.row = proto;
+      SweepDesc D;
+      desc(k0, D);
       stage(D, X);
-      desc(k + 3, D);
-      if (ok && Y.d.nl >= 0) ok = sweep_row<NC, MW>(Y.d, Y.row, Y.v, R, n, out, lv, err);
-      if (ok) ok = rest(k + 1);
-      if (!ok) s_abort = 1;
-      __syncthreads();
-      if (s_abort) return;
+      desc(k0 + 1, D);
+      for (int k = k0; k < k1; k += 2) {
+        SWEEP_LEVEL(X, Y, D, k, 2)
+        SWEEP_LEVEL(Y, X, D, k + 1, 2)
+      }
     }
+#undef SWEEP_LEVEL
     __syncthreads();
   }
 }
