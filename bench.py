@@ -527,7 +527,10 @@ def flowlab_leg(fvk, args):
                cells=m.n_dmn, faces=m.n_fce, nnz=m.nnz, mesh_build_s=t_mesh, context_s=t_ctx)
     # one launch per sweep with row ranges owned by blocks (the default), with
     # rows taken in level order in chunks, and one launch per level
+    forms = args.flowlab_forms.split(",")
     for key, one, parts in (("one_launch", 1, 1), ("one_launch_chunks", 1, 0), ("per_level_launches", 0, 0)):
+        if {"one_launch": "ranges", "one_launch_chunks": "chunks", "per_level_launches": "per_level"}[key] not in forms:
+            continue
         ctx.set_param("sweep_one_launch", one)
         if one:
             ctx.set_param("sweep_parts", parts)
@@ -557,7 +560,8 @@ def flowlab_leg(fvk, args):
                                               "one core) driven by oracle/ref/flowlab_ref_driver.f90 on the same "
                                               "mesh, the iterations the GPU legs time: a %d-iteration run minus a "
                                               "2-iteration run (%.1f s)" % (K + 2, K + 2, setup))
-            out["speedup_vs_cpu_reference"] = dt / (out["one_launch"]["ms_per_outer"] * 1e-3)
+            if "one_launch" in out:
+                out["speedup_vs_cpu_reference"] = dt / (out["one_launch"]["ms_per_outer"] * 1e-3)
             log("flowlab %d^3 reference: %.1f s per outer iteration" % (n, dt))
     ctx.destroy()
     return out
@@ -706,6 +710,8 @@ def main():
     ap.add_argument("--dropin-only", action="store_true", help="run the Fortran drop-in leg alone")
     ap.add_argument("--flowlab", type=int, default=96, help="cells per side of the flowlab CSR SIMPLE leg (0: off)")
     ap.add_argument("--flowlab-its", type=int, default=6)
+    ap.add_argument("--flowlab-forms", default="ranges,chunks,per_level",
+                    help="forms of the factor and sweeps the flowlab leg times (comma list)")
     ap.add_argument("--flowlab-only", action="store_true", help="run the flowlab leg alone")
     args = ap.parse_args()
     if args.gpus != 1:
