@@ -35,16 +35,17 @@ def run(cmd):
 
 
 def build(verbose=False):
+    """Compile the translation units in parallel (one compiler process each,
+    at most as many as there are cores, capped at 8), then link."""
+    from concurrent.futures import ThreadPoolExecutor
     os.makedirs(BUILD, exist_ok=True)
-    objs = []
-    for s in HIP_SOURCES:
-        o = os.path.join(BUILD, s + ".o")
-        run(["hipcc", "-c"] + COMMON + HIPFLAGS + [os.path.join(CSRC, s), "-o", o])
-        objs.append(o)
-    for s in CPP_SOURCES:
-        o = os.path.join(BUILD, s + ".o")
-        run(["g++", "-c"] + COMMON + [os.path.join(CSRC, s), "-o", o])
-        objs.append(o)
+    jobs = [(["hipcc", "-c"] + COMMON + HIPFLAGS, s) for s in HIP_SOURCES] + \
+           [(["g++", "-c"] + COMMON, s) for s in CPP_SOURCES]
+    objs = [os.path.join(BUILD, s + ".o") for _, s in jobs]
+    workers = max(1, min(len(jobs), os.cpu_count() or 1, 8))
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        for f in [pool.submit(run, cmd + [os.path.join(CSRC, s), "-o", o]) for (cmd, s), o in zip(jobs, objs)]:
+            f.result()
     tmp = OUT + ".tmp"
     run(["hipcc", "-shared", "--offload-arch=" + ARCH, "-o", tmp] + objs)
     os.replace(tmp, OUT)
