@@ -1,14 +1,17 @@
 // Device helpers of the one-launch triangular sweeps (krylov.hip: the
 // spray-simulation factor on the slot tables; flowlab.hip: flowlab's factor
-// on CSR rows).  Rows are taken in level order in chunks of BS positions; a
-// resident grid takes the chunks from a ticket counter, so every row a chunk
-// waits for belongs to a chunk taken earlier, by a block that is running or
-// has finished.  An output starts as kSweepMark (a signalling NaN: no
-// arithmetic result has this pattern) and a row waits until each value it
-// needs has been replaced.  Waiting lanes never block their wave: the wave
-// loops until all its lanes are finished.  Every wait is bounded
-// (kSweepSpin passes per chunk); past the bound the kernel records an error
-// and its blocks stop taking chunks.
+// on CSR rows), in two forms: rows taken in level order in chunks
+// (sweep_chunks) and ranges of rows walked level by level by one block each
+// (sweep_parts, the default).  In both, work is taken from a ticket counter,
+// so everything a block waits for was taken earlier by a block that is
+// running or has finished.  An output starts as kSweepMark (a signalling
+// NaN: no arithmetic result has this pattern) and a row waits until each
+// value it needs from another block has been replaced.  Every wait is
+// bounded (kSweepSpin passes); past the bound the kernel records an error
+// and its blocks stop taking work.
+//
+// sweep_chunks: chunks of BS positions of the level order; waiting lanes
+// never block their wave (the wave loops until all its lanes are finished).
 #pragma once
 #include "dev.h"
 
