@@ -104,8 +104,8 @@ struct Ctx {
   // level by level by one block
   struct SweepParts {
     int P = 0, R = 0, W = 0, wmax = 0;  // wmax: the most rows of one level of a range
-    int *rows = nullptr, *first = nullptr, *cnt = nullptr, *aux = nullptr, *lrow = nullptr, *cols = nullptr,
-        *lcol = nullptr, *step = nullptr, *sp = nullptr;
+    int *hd = nullptr, *jc = nullptr, *lc = nullptr;  // descriptors (sweep_pack), 16-byte words
+    int *step = nullptr, *sp = nullptr;
   };
   SweepParts spf, spb;
   bool sweep_parts = true;

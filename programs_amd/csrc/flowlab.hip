@@ -63,10 +63,9 @@ struct FlCtx {
   // factor) and R3 rows (two or three components)
   struct Parts {
     int P = 0, R = 0, W = 0, wmax = 0;  // wmax: the most rows of one level of a range
-    int *rows = nullptr, *first = nullptr, *cnt = nullptr, *aux = nullptr, *lrow = nullptr, *cols = nullptr,
-        *lcol = nullptr;
+    int *hd = nullptr, *jc = nullptr, *lc = nullptr;  // descriptors (sweep_pack), 16-byte words
     int *step = nullptr, *sp = nullptr;
-    SweepTabs tabs() const { return SweepTabs{rows, first, cnt, aux, lrow, cols, lcol, W}; }
+    SweepTabs tabs() const { return SweepTabs{(const int4 *)hd, (const int4 *)jc, (const int4 *)lc}; }
   };
   Parts pf, pb;
   bool parts = false;
@@ -853,13 +852,11 @@ static int fl_parts_build(FlCtx &c, const std::vector<int> &lev, bool fwd, int R
       if (range_lo(j) == lo) lcol[(size_t)t * n + p] = j - lo;
     }
   }
-  int rc = fl_up(c, &pt.rows, rows);
-  if (!rc) rc = fl_up(c, &pt.first, first);
-  if (!rc) rc = fl_up(c, &pt.cnt, cnt);
-  if (!rc) rc = fl_up(c, &pt.aux, aux);
-  if (!rc) rc = fl_up(c, &pt.lrow, lrow);
-  if (!rc) rc = fl_up(c, &pt.cols, cols);
-  if (!rc) rc = fl_up(c, &pt.lcol, lcol);
+  std::vector<int> hd, jc, lc;
+  sweep_pack(n, W, rows, first, cnt, aux, lrow, cols, lcol, hd, jc, lc);
+  int rc = fl_up(c, &pt.hd, hd);
+  if (!rc) rc = fl_up(c, &pt.jc, jc);
+  if (!rc) rc = fl_up(c, &pt.lc, lc);
   if (!rc) rc = fl_up(c, &pt.step, step);
   if (!rc) rc = fl_up(c, &pt.sp, sp);
   return rc;
@@ -867,8 +864,7 @@ static int fl_parts_build(FlCtx &c, const std::vector<int> &lev, bool fwd, int R
 
 static int fl_parts_setup(FlCtx &c) {
   for (FlCtx::Parts *pt : {&c.pf, &c.pb}) {
-    for (int *q : {pt->rows, pt->first, pt->cnt, pt->aux, pt->lrow, pt->cols, pt->lcol, pt->step, pt->sp})
-      fl_free(c, q);
+    for (int *q : {pt->hd, pt->jc, pt->lc, pt->step, pt->sp}) fl_free(c, q);
     *pt = FlCtx::Parts();
   }
   int rc = fl_parts_build(c, c.h_levf, true, c.part_rows, c.pf);
@@ -1977,12 +1973,12 @@ int fvk_fl_ctx_set_param(fvk_fl_ctx *cc, const char *name, double value) {
     return FVK_OK;
   }
   if (!std::strcmp(name, "sweep_parts") && (value == 0 || value == 1)) {
-    if (value == 1 && !c.pf.rows) { set_error("a row has more lower or upper entries than one launch holds"); return FVK_ERR_ARG; }
+    if (value == 1 && !c.pf.hd) { set_error("a row has more lower or upper entries than one launch holds"); return FVK_ERR_ARG; }
     c.parts = value != 0;
     return FVK_OK;
   }
   if (!std::strcmp(name, "sweep_part_rows") && value >= 256 && value <= 6400 && value == (int)value) {
-    if (!c.pf.rows) { set_error("a row has more lower or upper entries than one launch holds"); return FVK_ERR_ARG; }
+    if (!c.pf.hd) { set_error("a row has more lower or upper entries than one launch holds"); return FVK_ERR_ARG; }
     FVK_HIP(hipStreamSynchronize(c.stream));
     c.part_rows = (int)value;
     return fl_parts_setup(c);

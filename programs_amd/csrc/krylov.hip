@@ -963,13 +963,11 @@ static int parts_build(Ctx &c, bool fwd, const std::vector<int> &rank, const std
         ++t;
       }
   }
-  int rc = upload_v(c, &pt.rows, rows);
-  if (!rc) rc = upload_v(c, &pt.first, first);
-  if (!rc) rc = upload_v(c, &pt.cnt, cnt);
-  if (!rc) rc = upload_v(c, &pt.aux, aux);
-  if (!rc) rc = upload_v(c, &pt.lrow, lrow);
-  if (!rc) rc = upload_v(c, &pt.cols, cols);
-  if (!rc) rc = upload_v(c, &pt.lcol, lcol);
+  std::vector<int> hd, jc, lc;
+  sweep_pack(n, W, rows, first, cnt, aux, lrow, cols, lcol, hd, jc, lc);
+  int rc = upload_v(c, &pt.hd, hd);
+  if (!rc) rc = upload_v(c, &pt.jc, jc);
+  if (!rc) rc = upload_v(c, &pt.lc, lc);
   if (!rc) rc = upload_v(c, &pt.step, step);
   if (!rc) rc = upload_v(c, &pt.sp, sp);
   return rc;
@@ -977,8 +975,7 @@ static int parts_build(Ctx &c, bool fwd, const std::vector<int> &rank, const std
 
 int sweep_parts_build(Ctx &c) {
   for (Ctx::SweepParts *pt : {&c.spf, &c.spb}) {
-    for (int *q : {pt->rows, pt->first, pt->cnt, pt->aux, pt->lrow, pt->cols, pt->lcol, pt->step, pt->sp})
-      free_owned(c, q);
+    for (int *q : {pt->hd, pt->jc, pt->lc, pt->step, pt->sp}) free_owned(c, q);
     *pt = Ctx::SweepParts();
   }
   if (!c.sweep_one_launch) return 0;
@@ -1009,7 +1006,7 @@ int sweep_parts_build(Ctx &c) {
 }
 
 static SweepTabs tabs_of(const Ctx::SweepParts &p) {
-  return SweepTabs{p.rows, p.first, p.cnt, p.aux, p.lrow, p.cols, p.lcol, p.W};
+  return SweepTabs{(const int4 *)p.hd, (const int4 *)p.jc, (const int4 *)p.lc};
 }
 // blocks of the partitioned kernels: as many per CU as the LDS of a block
 // lets reside, at most four of 64 threads or two of 256 (the ranges a grid

@@ -13,6 +13,8 @@
 // sweep_chunks: chunks of BS positions of the level order; waiting lanes
 // never block their wave (the wave loops until all its lanes are finished).
 #pragma once
+#include <vector>
+
 #include "dev.h"
 
 namespace fvk {
@@ -80,14 +82,14 @@ constexpr int kSweepSlots = 8;
 //
 // Each position p has a descriptor: its row, a row-specific first-entry
 // word, the entry count, one more row-specific index, the row's own LDS
-// slot (its index in the range), and per entry (slot-major, W slots of n)
-// the column and the column's LDS slot (-1: another range).  The first TB
-// positions of each level are software-pipelined (descriptors four levels
-// ahead and the row's coefficients and values from other ranges two levels
-// ahead where three stages fit in registers, else two and one), all loads
-// issued without branches (a lane past the level's end loads the level's
-// last position and is not run), so a level waits for memory only when a
-// value from another range has not yet arrived.
+// slot (its index in the range), and per entry the column and the column's
+// LDS slot (-1: another range), packed in 16-byte words (SweepTabs).  The
+// first TB positions of each level are software-pipelined (descriptors four
+// levels ahead, the row's coefficients two and its values from other ranges
+// one where three stages fit in registers, else two, one and one), all
+// loads issued without branches (a lane past the level's end loads the
+// level's last position and is not run), so a level waits for memory only
+// when a value from another range has not yet arrived.
 //
 // Row interface: fetch(d, MW) loads the row's coefficients (entries t < MW,
 // entries t >= d.nl from a valid address, unused); begin() does the
@@ -95,10 +97,37 @@ constexpr int kSweepSlots = 8;
 // the NC values v[] (called in entry order, t < d.nl); result(c) is
 // component c of the row's value; done(d) runs after the row's value is
 // stored.
+// Descriptor tables of n positions: hd[p] = {row, first-entry word,
+// count | LDS slot << 4, aux}; for each group g of four entry slots,
+// jc[g n + p] the columns and lc[g n + p] their LDS slots (-1: another
+// range).  Slots past the row's count hold the row and 0 (never read).
 struct SweepTabs {
-  const int *rows, *first, *cnt, *aux, *lrow, *cols, *lcol;
-  int W;
+  const int4 *hd, *jc, *lc;
 };
+
+// Packs per-position arrays (cols / lcol slot-major, W slots of n) into the
+// int words of hd (4 n) and of jc, lc (4 n per group of four slots, MW / 4
+// groups: MW = 4 when W <= 4, else 8).
+inline void sweep_pack(int n, int W, const std::vector<int> &rows, const std::vector<int> &first,
+                       const std::vector<int> &cnt, const std::vector<int> &aux, const std::vector<int> &lrow,
+                       const std::vector<int> &cols, const std::vector<int> &lcol, std::vector<int> &hd,
+                       std::vector<int> &jc, std::vector<int> &lc) {
+  const int groups = W <= 4 ? 1 : 2;
+  hd.assign((size_t)4 * n, 0);
+  jc.assign((size_t)4 * groups * n, 0);
+  lc.assign((size_t)4 * groups * n, 0);
+  for (int p = 0; p < n; ++p) {
+    hd[(size_t)4 * p] = rows[p];
+    hd[(size_t)4 * p + 1] = first[p];
+    hd[(size_t)4 * p + 2] = cnt[p] | lrow[p] << 4;
+    hd[(size_t)4 * p + 3] = aux[p];
+    for (int t = 0; t < 4 * groups; ++t) {
+      const size_t o = ((size_t)(t / 4) * n + p) * 4 + t % 4;
+      jc[o] = t < cnt[p] ? cols[(size_t)t * n + p] : rows[p];
+      lc[o] = t < cnt[p] ? lcol[(size_t)t * n + p] : 0;
+    }
+  }
+}
 
 struct SweepDesc {
   int r, first, nl, aux, lr;
@@ -111,23 +140,29 @@ typedef const __attribute__((address_space(4))) int *const_int_p;
 __device__ __forceinline__ int ld_uniform(const int *p, int i) { return ((const_int_p)p)[i]; }
 
 // descriptor of position p of the level [pb, pe) (p >= pe: a copy of the
-// level's last position with no entries, not run); MW >= W entry slots are
-// loaded, slots t >= W from slot W-1 (unused)
+// level's last position with no entries, not run): 1 + 2 MW / 4 loads of
+// 16 bytes
 template <int MW>
 __device__ __forceinline__ void sweep_desc(const SweepTabs &T, int n, int p, int pe, SweepDesc &d) {
   const bool live = p < pe;
   const int q = live ? p : pe - 1;
-  d.r = T.rows[q];
-  d.first = T.first[q];
-  const int c = T.cnt[q];
-  d.nl = live ? c : -1;
-  d.aux = T.aux[q];
-  d.lr = T.lrow[q];
+  const int4 h = T.hd[q];
+  d.r = h.x;
+  d.first = h.y;
+  d.nl = live ? (h.z & 15) : -1;
+  d.lr = h.z >> 4;
+  d.aux = h.w;
 #pragma unroll
-  for (int t = 0; t < MW; ++t) {
-    const size_t o = (size_t)(t < T.W ? t : T.W - 1) * n + q;
-    d.j[t] = T.cols[o];
-    d.li[t] = T.lcol[o];
+  for (int g = 0; g < MW / 4; ++g) {
+    const int4 J = T.jc[(size_t)g * n + q], L = T.lc[(size_t)g * n + q];
+    d.j[4 * g] = J.x;
+    d.j[4 * g + 1] = J.y;
+    d.j[4 * g + 2] = J.z;
+    d.j[4 * g + 3] = J.w;
+    d.li[4 * g] = L.x;
+    d.li[4 * g + 1] = L.y;
+    d.li[4 * g + 2] = L.z;
+    d.li[4 * g + 3] = L.w;
   }
 }
 
@@ -218,9 +253,8 @@ template <int NC, int MW, class Row, int TB = BS>
 __device__ __forceinline__ void sweep_parts(int n, int R, int P, const SweepTabs &T,
                                             const int *__restrict__ step, const int *__restrict__ sp, double *out,
                                             int *ticket, int *err, const Row &proto) {
-  // two levels of prefetch on both links (descriptors four levels ahead,
-  // coefficients and values from other ranges two) where the three stages
-  // fit in registers, else one
+  // descriptors four levels ahead, coefficients two and values from other
+  // ranges one where the three stages fit in registers, else two, one and one
   constexpr bool deep = NC * MW <= 12;
   extern __shared__ double lv[];
   __shared__ int s_q, s_abort;
@@ -239,10 +273,14 @@ __device__ __forceinline__ void sweep_parts(int n, int R, int P, const SweepTabs
       const int pb = ld_uniform(step, kk), pe = ld_uniform(step, kk + 1);
       sweep_desc<MW>(T, n, in ? pb + tid : pe, pe, d);
     };
-    auto stage = [&](const SweepDesc &d, SweepStage<NC, Row> &S) {
+    auto coef = [&](const SweepDesc &d, SweepStage<NC, Row> &S) {
       S.d = d;
       S.row.fetch(S.d, MW);
-      sweep_ext<NC, MW>(S.d, n, out, S.v);
+    };
+    auto ext = [&](SweepStage<NC, Row> &S) { sweep_ext<NC, MW>(S.d, n, out, S.v); };
+    auto stage = [&](const SweepDesc &d, SweepStage<NC, Row> &S) {
+      coef(d, S);
+      ext(S);
     };
     // positions of level k beyond the first TB, not pipelined
     auto rest = [&](int k) -> bool {
@@ -259,10 +297,11 @@ __device__ __forceinline__ void sweep_parts(int n, int R, int P, const SweepTabs
       return ok;
     };
     bool ok = true;
-    // one level: stage NEXT from the descriptors in DB, load the descriptors
-    // of level K + AHEAD into DB, run RUN (level K), barrier
-#define SWEEP_LEVEL(RUN, NEXT, DB, K, AHEAD)                                                           \
-    stage(DB, NEXT);                                                                                   \
+    // one level: STAGE (loads for the levels ahead, the coefficients from the
+    // descriptors in DB), load the descriptors of level K + AHEAD into DB,
+    // run RUN (level K), barrier
+#define SWEEP_LEVEL(RUN, STAGE, DB, K, AHEAD)                                                          \
+    STAGE;                                                                                             \
     desc((K) + (AHEAD), DB);                                                                           \
     if (ok && RUN.d.nl >= 0) ok = sweep_row<NC, MW>(RUN.d, RUN.row, RUN.v, R, n, out, lv, err);        \
     if (ok) ok = rest(K);                                                                              \
@@ -279,16 +318,17 @@ __device__ __forceinline__ void sweep_parts(int n, int R, int P, const SweepTabs
       desc(k0, D0);
       stage(D0, S0);
       desc(k0 + 1, D1);
-      stage(D1, S1);
+      coef(D1, S1);
       desc(k0 + 2, D0);
       desc(k0 + 3, D1);
+      // level k: coefficients of k + 2, values from other ranges of k + 1
       for (int k = k0; k < k1; k += 6) {
-        SWEEP_LEVEL(S0, S2, D0, k, 4)
-        SWEEP_LEVEL(S1, S0, D1, k + 1, 4)
-        SWEEP_LEVEL(S2, S1, D0, k + 2, 4)
-        SWEEP_LEVEL(S0, S2, D1, k + 3, 4)
-        SWEEP_LEVEL(S1, S0, D0, k + 4, 4)
-        SWEEP_LEVEL(S2, S1, D1, k + 5, 4)
+        SWEEP_LEVEL(S0, (coef(D0, S2), ext(S1)), D0, k, 4)
+        SWEEP_LEVEL(S1, (coef(D1, S0), ext(S2)), D1, k + 1, 4)
+        SWEEP_LEVEL(S2, (coef(D0, S1), ext(S0)), D0, k + 2, 4)
+        SWEEP_LEVEL(S0, (coef(D1, S2), ext(S1)), D1, k + 3, 4)
+        SWEEP_LEVEL(S1, (coef(D0, S0), ext(S2)), D0, k + 4, 4)
+        SWEEP_LEVEL(S2, (coef(D1, S1), ext(S0)), D1, k + 5, 4)
       }
     } else {
       SweepStage<NC, Row> X, Y;
@@ -299,8 +339,8 @@ __device__ __forceinline__ void sweep_parts(int n, int R, int P, const SweepTabs
       stage(D, X);
       desc(k0 + 1, D);
       for (int k = k0; k < k1; k += 2) {
-        SWEEP_LEVEL(X, Y, D, k, 2)
-        SWEEP_LEVEL(Y, X, D, k + 1, 2)
+        SWEEP_LEVEL(X, stage(D, Y), D, k, 2)
+        SWEEP_LEVEL(Y, stage(D, X), D, k + 1, 2)
       }
     }
 #undef SWEEP_LEVEL
