@@ -181,7 +181,9 @@ __device__ __forceinline__ void sweep_ext(const SweepDesc &d, int n, const doubl
 }
 
 // run one row (d.nl >= 0): values inside the range from LDS, the others from
-// v[] (polled again while they are the mark); false if the wait bound ran out
+// v[] (polled again while they are the mark); false if the wait bound ran
+// out.  Every slot t < MW is read and taken, and a slot past the row's count
+// is dropped by a select, so that the entries cost no branches.
 template <int NC, int MW, class Row>
 __device__ __forceinline__ bool sweep_row(const SweepDesc &d, Row &row,
                                           unsigned long long (&v)[kSweepSlots][NC], int R, int n, double *out,
@@ -190,17 +192,17 @@ __device__ __forceinline__ bool sweep_row(const SweepDesc &d, Row &row,
   unsigned need = 0;
 #pragma unroll
   for (int t = 0; t < MW; ++t) {
-    if (t >= d.nl) continue;
     const int l = d.li[t];
-    if (l >= 0) {
+    const bool loc = l >= 0;
+    const int la = loc ? l : 0;
+    bool mk = false;
 #pragma unroll
-      for (int c = 0; c < NC; ++c) v[t][c] = (unsigned long long)__double_as_longlong(lv[c * R + l]);
-    } else {
-      bool ready = true;
-#pragma unroll
-      for (int c = 0; c < NC; ++c) ready = ready && v[t][c] != kSweepMark;
-      if (!ready) need |= 1u << t;
+    for (int c = 0; c < NC; ++c) {
+      const unsigned long long x = (unsigned long long)__double_as_longlong(lv[c * R + la]);
+      v[t][c] = loc ? x : v[t][c];
+      mk = mk | (v[t][c] == kSweepMark);
     }
+    need |= (t < d.nl && mk) ? 1u << t : 0u;
   }
   unsigned spin = 0;
   while (need) {
@@ -225,13 +227,14 @@ __device__ __forceinline__ bool sweep_row(const SweepDesc &d, Row &row,
     }
   }
 #pragma unroll
-  for (int t = 0; t < MW; ++t)
-    if (t < d.nl) {
-      double w[NC];
+  for (int t = 0; t < MW; ++t) {
+    double w[NC];
 #pragma unroll
-      for (int c = 0; c < NC; ++c) w[c] = __longlong_as_double((long long)v[t][c]);
-      row.take(t, w);
-    }
+    for (int c = 0; c < NC; ++c) w[c] = __longlong_as_double((long long)v[t][c]);
+    Row next = row;
+    next.take(t, w);
+    if (t < d.nl) row = next;
+  }
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     const double x = row.result(c);
@@ -299,16 +302,14 @@ __device__ __forceinline__ void sweep_parts(int n, int R, int P, const SweepTabs
     bool ok = true;
     // one level: STAGE (loads for the levels ahead, the coefficients from the
     // descriptors in DB), load the descriptors of level K + AHEAD into DB,
-    // run RUN (level K), barrier
+    // run RUN (level K), barrier.  Levels past the range's last have no live
+    // positions; a failed wait is acted on once per unrolled group.
 #define SWEEP_LEVEL(RUN, STAGE, DB, K, AHEAD)                                                          \
     STAGE;                                                                                             \
     desc((K) + (AHEAD), DB);                                                                           \
     if (ok && RUN.d.nl >= 0) ok = sweep_row<NC, MW>(RUN.d, RUN.row, RUN.v, R, n, out, lv, err);        \
-    if (ok) ok = rest(K);                                                                              \
-    if (!ok) s_abort = 1;                                                                              \
-    __syncthreads();                                                                                   \
-    if (s_abort) return;                                                                               \
-    if ((K) + 1 >= k1) break;
+    if ((K) < k1 && ld_uniform(step, (K) + 1) - ld_uniform(step, K) > TB && ok) ok = rest(K);          \
+    __syncthreads();
     if constexpr (deep) {
       SweepStage<NC, Row> S0, S1, S2;
       S0.row = proto;
@@ -329,6 +330,9 @@ __device__ __forceinline__ void sweep_parts(int n, int R, int P, const SweepTabs
         SWEEP_LEVEL(S0, (coef(D1, S2), ext(S1)), D1, k + 3, 4)
         SWEEP_LEVEL(S1, (coef(D0, S0), ext(S2)), D0, k + 4, 4)
         SWEEP_LEVEL(S2, (coef(D1, S1), ext(S0)), D1, k + 5, 4)
+        if (!ok) s_abort = 1;
+        __syncthreads();
+        if (s_abort) return;
       }
     } else {
       SweepStage<NC, Row> X, Y;
@@ -341,6 +345,9 @@ __device__ __forceinline__ void sweep_parts(int n, int R, int P, const SweepTabs
       for (int k = k0; k < k1; k += 2) {
         SWEEP_LEVEL(X, stage(D, Y), D, k, 2)
         SWEEP_LEVEL(Y, stage(D, X), D, k + 1, 2)
+        if (!ok) s_abort = 1;
+        __syncthreads();
+        if (s_abort) return;
       }
     }
 #undef SWEEP_LEVEL
