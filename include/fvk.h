@@ -258,7 +258,11 @@ int fvk_sys_get_x(fvk_ctx *ctx, double *x);
    4096, 256..6400, and walks them level by level, values inside a range
    passing through LDS; 0: rows taken in level order in chunks by
    "sweep_blocks_per_cu" resident blocks per CU, default 1).  All give the
-   same bits. */
+   same bits.  "sweep_fail" 1 (a test of the failure path): records that a
+   sweep's wait ran past its bound, as a wait that never ends would; rows
+   of the ranges form that wait give up within 256 passes, blocks stop
+   taking ranges, and this and every later factorisation or sweep of the
+   context returns FVK_ERR_HIP. */
 int fvk_ctx_set_param(fvk_ctx *ctx, const char *name, double value);
 
 /* The factorised preconditioner z = G G^T r, G = D^-1/2 (I + L) Delta, with

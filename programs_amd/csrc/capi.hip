@@ -1003,6 +1003,11 @@ int fvk_ctx_set_param(fvk_ctx *cc, const char *name, double value) {
   }
   else if (!std::strcmp(name, "sweep_blocks_per_cu") && value >= 1 && value <= 8 && value == (int)value)
     cc->c.sweep_blocks_per_cu = (int)value;
+  else if (!std::strcmp(name, "sweep_fail") && value == 1 && cc->c.sweep_int) {
+    const int one = 1;
+    FVK_HIP(hipMemcpyAsync(cc->c.sweep_int + 3, &one, sizeof one, hipMemcpyHostToDevice, cc->c.stream));
+    FVK_HIP(hipStreamSynchronize(cc->c.stream));
+  }
   else { set_error(std::string("unknown parameter or value out of range: ") + name); return FVK_ERR_ARG; }
   return FVK_OK;
 }

@@ -30,6 +30,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfvk.so")
 
 FVK_OK, FVK_FALLBACK, FVK_NOT_CONVERGED = 0, 1, 2
+FVK_ERR_ARG, FVK_ERR_HIP, FVK_ERR_STATE, FVK_ERR_DIVERGED, FVK_ERR_IO, FVK_ERR_NODEV = -1, -2, -3, -4, -5, -6
 CTX_RCM = 1
 
 _dp = C.POINTER(C.c_double)

@@ -185,3 +185,32 @@ def test_shipped_simple_one_launch_bitwise(graph):
         assert h0 == h1, label
     say("  residual history %s" % h0)
     m.free()
+
+
+def test_sweep_wait_failure_is_reported():
+    """The failure path of the one-launch factor and sweeps, forced from the
+    host: with the wait-bound flag raised ("sweep_fail"), rows of the ranges
+    form that wait give up within 256 passes, blocks stop taking ranges, the
+    launches end, and the factorisation and sweeps report FVK_ERR_HIP; a new
+    context on the same mesh gives the bits of the first."""
+    fvk = require_lib()
+    n = 32
+    say("sweep failure path: box %d^3" % n)
+    m = fvk.Mesh.box(n, n, n, jitter=0.05, seed=0x5EED0401)
+    pl, ngp, ngn = random_matrix(m, 19)
+    b = np.random.default_rng(9).standard_normal(m.n_vol)
+    ctx = fvk.grd_solv(m)
+    fvk.set_matrix(ctx, pl, ngp, ngn)
+    x0 = fvk.solv_direct(ctx, 2, b)
+    fvk.set_param(ctx, "sweep_fail", 1)
+    for call in (lambda: fvk.solv_precond(ctx, 1), lambda: fvk.solv_direct(ctx, 2, b)):
+        with pytest.raises(fvk.FvkError) as e:
+            call()
+        assert e.value.status == fvk.FVK_ERR_HIP and "exceeded its bound" in str(e.value)
+    ctx.destroy()
+    ctx = fvk.grd_solv(m)
+    fvk.set_matrix(ctx, pl, ngp, ngn)
+    assert np.array_equal(fvk.solv_direct(ctx, 2, b), x0)
+    ctx.destroy()
+    m.free()
+    say("  reported, and a new context sweeps as before")
