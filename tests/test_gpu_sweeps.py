@@ -12,7 +12,8 @@ scheduling differs.  Cases: jittered hex boxes up to 96^3 (884,736 rows,
 three right-hand sides, the non-symmetric DILU matrix, PCG / BiCGStab
 solves built on them, and SIMPLE outer iterations in the reference's
 shipped configuration (SIMPLER, DILU momentum, DIC pressure), eager and
-from HIP graphs.
+from HIP graphs.  The wait-bound failure path is forced from the host and
+must end every launch and be reported.
 """
 import sys
 
