@@ -508,8 +508,9 @@ void fvk_fl_ctx_destroy(fvk_fl_ctx *ctx);
    "sweep_part_rows" consecutive rows (default 4096, 256..6400) and walks them
    level by level, values inside a range passing through LDS; 0: rows taken
    in level order in chunks.  "sweep_blocks" (0: one per CU): blocks of the
-   one-launch kernels.  All give the same bits.  No counterpart in the
-   reference. */
+   one-launch kernels.  All give the same bits.  "sweep_fail" 1 (a test of
+   the failure path): as fvk_ctx_set_param's; the context's solves then
+   return FVK_ERR_HIP.  No counterpart in the reference. */
 int fvk_fl_ctx_set_param(fvk_fl_ctx *ctx, const char *name, double value);
 /* xdmn%x_vc (n_dim x n_dmn), vol, x_fc, norm, area, x_pc, x_nc, l_pn, w_fp
    (per face), ls (n_dim x n_dim x n_dmn); NULL skips */

@@ -1987,6 +1987,12 @@ int fvk_fl_ctx_set_param(fvk_fl_ctx *cc, const char *name, double value) {
     c.sweep_blocks = (int)value;
     return FVK_OK;
   }
+  if (!std::strcmp(name, "sweep_fail") && value == 1 && c.sweep_int) {
+    const int one = 1;
+    FVK_HIP(hipMemcpyAsync(c.sweep_int + 3, &one, sizeof one, hipMemcpyHostToDevice, c.stream));
+    FVK_HIP(hipStreamSynchronize(c.stream));
+    return FVK_OK;
+  }
   set_error(std::string("unknown parameter or value out of range: ") + name);
   return FVK_ERR_ARG;
 }

@@ -234,6 +234,42 @@ def test_one_launch_sweeps_bitwise(mesh, tmp_path):
             bitwise(s1[k], s0[k], label + ": " + k)
 
 
+
+def test_sweep_wait_failure_is_reported():
+    """The one-launch sweeps' failure path, forced from the host ("sweep_fail"
+    raises the wait-bound flag): the outer iteration's solves end and report
+    FVK_ERR_HIP; a new context on the same mesh gives the first run's bits."""
+    fvk = require_lib()
+    from programs_amd import flowlab as fl
+    from programs_amd import meshgen
+    say("flowlab sweep failure path: box 24^3")
+    m = fl.FlMesh.from_records(3, *meshgen.box_records(24, jitter=0.1))
+
+    def run(fail):
+        ctx = fl.FlCtx(m)
+        fw = fl.FlFlow(ctx, 1.0, 0.01, 0.7, 0.3, 0.5)
+        fw.set(vel_bnd=fl.boundary_values(m, ctx.geometry()["norm"], "Wall", 1, (1, 0, 0, 1)))
+        try:
+            if fail:
+                ctx.set_param("sweep_fail", 1)
+                with pytest.raises(fvk.FvkError) as e:
+                    fw.run(1, 0.0)
+                assert e.value.status == fvk.FVK_ERR_HIP and "exceeded its bound" in str(e.value)
+                return None
+            fw.run(1, 0.0)
+            return fw.get()
+        finally:
+            fw.destroy()
+            ctx.destroy()
+
+    s0 = run(False)
+    run(True)
+    s1 = run(False)
+    for k in ("vel", "pres", "pp", "spd"):
+        bitwise(s1[k], s0[k], "after the failed context: " + k)
+    say("  reported, and a new context runs as before")
+
+
 # ---------------------------------------------------------------- Fortran drop-in
 DROPIN = os.path.join(os.path.dirname(GOLDEN), "..", "oracle", "_ref", "dropin_flowlab", "flowlab_ref_driver")
 DROPIN_STAMP = os.path.join(os.path.dirname(DROPIN), "sources.sha256")
